@@ -1,0 +1,102 @@
+"""ctypes binding of libart_hip.so (declared in include/art_hip.h).
+
+The library is built in-tree by ``art._build`` (hipcc, gfx950).  There is no
+fallback: when the library is missing every device entry point raises.
+Validation helpers here run before the library is loaded, mirroring the
+reference's rule that argument errors surface before any GPU import
+(tests/test_cuda_batched_cholesky.py:142-152).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+LIB_DIR = Path(__file__).resolve().with_name("_lib")
+LIB_PATH = LIB_DIR / "libart_hip.so"
+
+ART_OK = 0
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
+
+_lib = None
+
+
+class HipLibraryError(RuntimeError):
+    """Raised when libart_hip.so is missing or a call fails."""
+
+
+_p = ctypes.c_void_p
+_i = ctypes.c_int
+_i64 = ctypes.c_int64
+_d = ctypes.c_double
+
+_PROTOTYPES = {
+    "art_last_error": (ctypes.c_char_p, []),
+    "art_version": (_i, []),
+    "art_feature_count": (_i, [_i, _i, _i, _i]),
+    "art_features_build": (_i, [_p, _i64, _i, _i, _i, _i, _p, _p]),
+    "art_check_finite": (_i, [_p, _p, _i64, _i64, _i, _p, _p]),
+    "art_leaf_plan_create": (_i, [_p, _i, _i, _i, _i, _i, _i, ctypes.POINTER(_p)]),
+    "art_leaf_ridge_fit": (_i, [_p, _p, _p, _p, _d, _d, _i, _p, _p, _p, _p]),
+    "art_gated_plan_create": (_i, [_i, _i64, _i, _i, ctypes.POINTER(_p)]),
+    "art_gated_ridge_fit": (_i, [_p, _p, _p, _p, _p, _p, _i, _d, _d, _p, _p, _p, _p]),
+    "art_plan_info": (_i, [_p, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "art_plan_timing_enable": (_i, [_p, _i]),
+    "art_plan_timing_read": (_i, [_p, _p, _p, _i]),
+    "art_plan_destroy": (None, [_p]),
+}
+
+
+def exported_symbols():
+    """Names every build of libart_hip.so must export."""
+    return tuple(_PROTOTYPES)
+
+
+def load(path: os.PathLike | None = None):
+    """Load (once) and return the ctypes handle of libart_hip.so."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    target = Path(path) if path is not None else LIB_PATH
+    if not target.exists():
+        raise HipLibraryError(
+            f"{target} is missing: build it with `python -c 'import __graft_entry__ as g; "
+            f"g.build()'` (hipcc --offload-arch=gfx950).")
+    lib = ctypes.CDLL(str(target))
+    for name, (res, args) in _PROTOTYPES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.art_version() != 1:
+        raise HipLibraryError("libart_hip.so ABI version mismatch; rebuild it.")
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(rc: int) -> None:
+    """Translate a negative ART_ERR_* return into an exception."""
+    if rc != ART_OK:
+        msg = load().art_last_error().decode(errors="replace")
+        if rc == -1:
+            raise ValueError(msg)
+        raise HipLibraryError(f"libart_hip error {rc}: {msg}")
+
+
+def torch_device():
+    """torch and a HIP device check; raises when no GPU is visible."""
+    import torch
+
+    if not torch.cuda.is_available():
+        raise HipLibraryError("No HIP device is visible; the ART hot path runs only on a GPU.")
+    return torch
+
+
+def stream_handle(stream) -> int:
+    return int(stream.cuda_stream)
+
+
+def ptr(t) -> int:
+    return int(t.data_ptr())

@@ -1,0 +1,114 @@
+"""Batched weighted ridge fits over variable-length leaves (CSR row ranges).
+
+The reference fits every leaf separately on the CPU with
+``solve_weighted_ridge(solver="normal")`` (art/models.py:218-343) after
+``PolynomialRidgeModel.prepare_design`` (art/models.py:602-620); its GPU
+batch (art/_cuda_batched_cholesky.py) needs equal-size slots.  Here one
+resident plan holds the chunking of K uneven leaves and every fit builds the
+features from X on the fly, contracts the weighted Gram by MFMA and solves
+each leaf's system on the device.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _hip
+from .features import _columns
+
+PRECISIONS = {"fp64": _hip.PRECISION_FP64, "mixed": _hip.PRECISION_MIXED}
+
+
+class LeafRidgeBatch:
+    """Resident plan for K leaves given by CSR ``offsets`` (K+1,).
+
+    ``fit(X, y, w, ridge)`` takes device tensors X (N,d), y (N,), w (N,)
+    float64 and returns device tensors ``coefficients (K,p)``, ``status (K,)``
+    and ``ridge_effective (K,)`` (overwritten by the next fit).  Status codes
+    are those of the batched Cholesky workspace; a negative or non-finite
+    weight flags its leaf with -1.
+    """
+
+    def __init__(self, offsets, d, *, degree=2, include_bias=True, bias_last=False,
+                 chunk_rows=0, stream=None):
+        offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        if offsets.ndim != 1 or offsets.size < 2 or offsets[0] != 0:
+            raise ValueError("offsets must be a one-dimensional CSR array starting at 0.")
+        if np.any(np.diff(offsets) < 0):
+            raise ValueError("offsets must be nondecreasing.")
+        if not isinstance(d, (int, np.integer)) or d < 1:
+            raise ValueError("d must be a positive integer.")
+        columns = _columns(int(d), degree, include_bias, bias_last)
+        self.K, self.d, self.p = offsets.size - 1, int(d), columns.shape[1]
+        self.n_rows = int(offsets[-1])
+        self.offsets = offsets
+        self.degree, self.include_bias, self.bias_last = degree, bool(include_bias), bool(bias_last)
+        self.plan = None
+        torch = _hip.torch_device()
+        self.torch, self.lib = torch, _hip.load()
+        self.device_id = torch.cuda.current_device()
+        self.device = torch.device("cuda", self.device_id)
+        self.stream = torch.cuda.current_stream() if stream is None else stream
+        handle = ctypes.c_void_p()
+        _hip.check(self.lib.art_leaf_plan_create(
+            offsets.ctypes.data, self.K, self.d, int(degree), int(include_bias), int(bias_last),
+            int(chunk_rows), ctypes.byref(handle)))
+        self.plan = handle
+        p, nchunks, tiles = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _hip.check(self.lib.art_plan_info(self.plan, ctypes.byref(p), ctypes.byref(nchunks),
+                                          ctypes.byref(tiles)))
+        self.n_chunks, self.tiles = nchunks.value, tiles.value
+        f64 = dict(dtype=torch.float64, device=self.device)
+        self.coefficients = torch.full((self.K, self.p), float("nan"), **f64)
+        self.ridge_effective = torch.zeros(self.K, **f64)
+        self.status = torch.zeros(self.K, dtype=torch.int32, device=self.device)
+
+    def _check_inputs(self, X, y, w):
+        torch = self.torch
+        for name, t, shape in (("X", X, (self.n_rows, self.d)), ("y", y, (self.n_rows,)),
+                               ("w", w, (self.n_rows,))):
+            if (not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != torch.float64
+                    or not t.is_contiguous() or t.device.index != self.device_id):
+                raise ValueError(f"{name} must be a C-contiguous float64 tensor on the device.")
+            if tuple(t.shape) != shape:
+                raise ValueError(f"{name} must have shape {shape}.")
+
+    def fit(self, X, y, w, ridge, weight_floor=1e-12, precision="fp64"):
+        if self.plan is None:
+            raise RuntimeError("The plan is closed.")
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {tuple(PRECISIONS)}.")
+        ridge, weight_floor = float(ridge), float(weight_floor)
+        if not np.isfinite(ridge) or ridge < 0:
+            raise ValueError("ridge must be finite and nonnegative.")
+        if not np.isfinite(weight_floor) or weight_floor < 0:
+            raise ValueError("weight_floor must be finite and nonnegative.")
+        self._check_inputs(X, y, w)
+        _hip.check(self.lib.art_leaf_ridge_fit(
+            self.plan, _hip.ptr(X), _hip.ptr(y), _hip.ptr(w), ridge, weight_floor,
+            PRECISIONS[precision], _hip.ptr(self.coefficients), _hip.ptr(self.status),
+            _hip.ptr(self.ridge_effective), _hip.stream_handle(self.stream)))
+        return self.coefficients, self.status, self.ridge_effective
+
+    def synchronize(self):
+        self.stream.synchronize()
+
+    def close(self):
+        if getattr(self, "plan", None) is not None:
+            self.synchronize()
+            self.lib.art_plan_destroy(self.plan)
+            self.plan = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *args):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,108 @@
+"""Seeded synthetic workloads for the leaf-ridge, split and sampler paths.
+
+These generate inputs of the shapes the hot path is benchmarked on; they are
+plain NumPy and hold no reference arithmetic.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+
+def sigmoid(t):
+    """Logistic gate with the +-50 clip of art/objectives.py:49-51."""
+    return 1.0 / (1.0 + np.exp(-np.clip(t, -50.0, 50.0)))
+
+
+@dataclass
+class BumpMixture:
+    """y(x) = sum_k A_k exp(-||x - c_k||^2 / (2 s_k^2))."""
+
+    centers: np.ndarray
+    widths: np.ndarray
+    amplitudes: np.ndarray
+
+    @classmethod
+    def random(cls, rng, n_bumps, d):
+        return cls(rng.uniform(-1.0, 1.0, size=(n_bumps, d)),
+                   rng.uniform(0.1, 0.5, size=n_bumps),
+                   rng.uniform(-1.0, 1.0, size=n_bumps))
+
+    def __call__(self, X):
+        X = np.asarray(X, dtype=np.float64)
+        out = np.zeros(X.shape[0])
+        for c, s, a in zip(self.centers, self.widths, self.amplitudes):
+            out += a * np.exp(-np.sum((X - c) ** 2, axis=1) / (2.0 * s * s))
+        return out
+
+
+@dataclass
+class LeafRidgeWorkload:
+    """Variable-length weighted leaves in CSR form.
+
+    X (N,d) float64 row-major, y (N,), w (N,) in (0,1], offsets (K+1,) int64.
+    """
+
+    X: np.ndarray
+    y: np.ndarray
+    w: np.ndarray
+    offsets: np.ndarray
+    ridge: float
+    degree: int = 2
+
+    @property
+    def n_leaves(self):
+        return len(self.offsets) - 1
+
+    @property
+    def n_rows(self):
+        return int(self.offsets[-1])
+
+    def leaf(self, j):
+        s, e = int(self.offsets[j]), int(self.offsets[j + 1])
+        return self.X[s:e], self.y[s:e], self.w[s:e]
+
+    def subset(self, leaves):
+        """A new workload holding only the given leaves, in the given order."""
+        leaves = np.asarray(leaves, dtype=np.int64)
+        parts = [self.leaf(j) for j in leaves]
+        sizes = np.array([len(p[1]) for p in parts], dtype=np.int64)
+        offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+        return LeafRidgeWorkload(np.concatenate([p[0] for p in parts]),
+                                 np.concatenate([p[1] for p in parts]),
+                                 np.concatenate([p[2] for p in parts]),
+                                 offsets, self.ridge, self.degree)
+
+
+def leaf_ridge_workload(n_leaves=4096, d=8, min_rows=64, max_rows=16384,
+                        n_bumps=6, ridge=1e-6, seed=2, degree=2):
+    """Batched weighted-ridge leaves (default: 4096 leaves, d=8, ~12M rows).
+
+    Leaf sizes are log-uniform in [min_rows, max_rows]; x ~ U[-1,1]^d; each
+    leaf's weights are the product of 1..6 sigmoid gates
+    sigma((a.x - c)/tau), a a unit Gaussian direction, c ~ U[-.5,.5],
+    tau ~ U[.05,.5]; y is a sum of ``n_bumps`` Gaussian bumps with centers in
+    [-1,1]^d, widths U[.1,.5] and amplitudes U[-1,1].
+    """
+    rng = np.random.default_rng(seed)
+    sizes = np.floor(np.exp(rng.uniform(np.log(min_rows), np.log(max_rows),
+                                        size=n_leaves))).astype(np.int64)
+    sizes = np.clip(sizes, min_rows, max_rows)
+    offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    N = int(offsets[-1])
+    X = rng.uniform(-1.0, 1.0, size=(N, d))
+    target = BumpMixture.random(rng, n_bumps, d)
+    y = target(X)
+    w = np.empty(N)
+    n_gates = rng.integers(1, 7, size=n_leaves)
+    for j in range(n_leaves):
+        s, e = offsets[j], offsets[j + 1]
+        g = n_gates[j]
+        a = rng.normal(size=(g, d))
+        a /= np.linalg.norm(a, axis=1, keepdims=True)
+        c = rng.uniform(-0.5, 0.5, size=g)
+        tau = rng.uniform(0.05, 0.5, size=g)
+        w[s:e] = np.prod(sigmoid((X[s:e] @ a.T - c) / tau), axis=1)
+    return LeafRidgeWorkload(X, y, w, offsets, float(ridge), degree)

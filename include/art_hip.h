@@ -1,0 +1,117 @@
+/* C ABI of libart_hip.so: the MI355X (gfx950) hot path of Adaptive Regression
+ * Trees.  Plain pointers and sizes only; every device pointer is HIP device
+ * memory, every call is enqueued on `stream` and never synchronizes unless its
+ * comment says so.  Return values: ART_OK (0) or a negative ART_ERR_* code;
+ * art_last_error() returns the message of the calling thread's last failure.
+ *
+ * Per-system numerical outcomes are NOT errors: they are written to int32
+ * status arrays with the reference's codes (art/_cuda_batched_cholesky.py:52-55):
+ *   0 ok, +k first non-positive Cholesky pivot (one-based), -1 flagged input
+ *   (bad gate or weight), -2 non-finite assembly/factor, -3 non-finite solve;
+ *   failed coefficients are NaN.
+ */
+#ifndef ART_HIP_H
+#define ART_HIP_H
+
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ART_ABI_VERSION 1
+
+#define ART_OK 0
+#define ART_ERR_ARG (-1)
+#define ART_ERR_HIP (-2)
+#define ART_ERR_ALLOC (-3)
+
+#define ART_PRECISION_FP64 0  /* FP64 Gram + FP64 Cholesky ("plain") */
+#define ART_PRECISION_MIXED 1 /* low-precision MFMA Gram + FP64 refinement */
+
+#define ART_DEFAULT_CHUNK_ROWS 2048
+#define ART_SOLVE_LDS_BYTES 65536
+
+/* Opaque resident workspace: chunk tables and partial-Gram buffers. */
+typedef struct art_ridge_plan art_ridge_plan;
+
+const char* art_last_error(void);
+int art_version(void);
+
+/* Number of feature columns of the basis, or -1 for an unsupported basis.
+ * Replaces FeaturePlan.__init__'s p (art/_cuda_features.py:13-29,56-64). */
+int art_feature_count(int d, int degree, int include_bias, int bias_last);
+
+/* Phi[r, f] = feature f of X[r, :] for rows r < rows; X (rows, d) and
+ * out (rows, p) row-major float64.  Column order: optional bias, x_0..x_{d-1},
+ * then x_i*x_j for i <= j with i outermost; bias_last moves the bias after the
+ * linear terms (degree 1 only).
+ * Replaces FeaturePlan.build / polynomial_features
+ * (art/_cuda_features.py:66-85, art/_cuda_features.cu:5-20). */
+int art_features_build(const double* X, int64_t rows, int d, int degree, int include_bias,
+                       int bias_last, double* out, hipStream_t stream);
+
+/* flags[g] = 1 when rows [g*rows_per_group, (g+1)*rows_per_group) of `a`
+ * (cols values each) or of `y` (one value per row, may be NULL) hold a
+ * non-finite value, else 0.  Replaces batch_check_data
+ * (art/_cuda_batched_cholesky_kernels.cu:7-13, art/_cuda_batched_cholesky.py:177-183). */
+int art_check_finite(const double* a, const double* y, int64_t groups, int64_t rows_per_group,
+                     int cols, int* flags, hipStream_t stream);
+
+/* Variable-length leaves.  offsets (K+1, host memory) delimit leaf j's rows
+ * [offsets[j], offsets[j+1]).  Allocates device buffers (synchronous).
+ * chunk_rows: rows per Gram work item (multiple of 16; 0 = default). */
+int art_leaf_plan_create(const int64_t* offsets, int K, int d, int degree, int include_bias,
+                         int bias_last, int chunk_rows, art_ridge_plan** plan);
+
+/* One weighted ridge fit per leaf with features built from X on the fly:
+ *   beta_j = argmin sum_r max(w_r, floor) (y_r - phi(x_r).beta)^2 + lambda_j |beta|^2,
+ *   lambda_j = ridge * trace(G_j) / p  (0 when ridge == 0).
+ * X (N, d), y (N,), w (N,) float64 device; beta (K, p), status (K,),
+ * lambda (K,) device outputs.  A negative or non-finite weight flags its
+ * leaf (-1).  Replaces the per-leaf solve_weighted_ridge(solver="normal")
+ * (art/models.py:218-343) fed by PolynomialRidgeModel.prepare_design
+ * (art/models.py:602-620). */
+int art_leaf_ridge_fit(art_ridge_plan* plan, const double* X, const double* y, const double* w,
+                       double ridge, double weight_floor, int precision, double* beta,
+                       int* status, double* lambda, hipStream_t stream);
+
+/* Equal-size gated nodes (the BatchedCholeskyWorkspace contract): capacity K
+ * nodes of n rows and p features.  Allocates device buffers (synchronous).
+ * Replaces BatchedCholeskyWorkspace.allocate (art/_cuda_batched_cholesky.py:68-76). */
+int art_gated_plan_create(int K, int64_t n, int p, int chunk_rows, art_ridge_plan** plan);
+
+/* Left/right fits for `count` active nodes: side 0 weights max(1-pi, floor),
+ * side 1 max(pi, floor).  phi (K, n, p), y (K, n), pi (K, n) float64 device;
+ * ids: the active slot indices, both a device copy and a host copy (count
+ * entries each).  Outputs are side-major: beta (2, K, p), status (2, K),
+ * lambda (2, K); slots not in ids keep their previous contents.
+ * Replaces BatchedCholeskyWorkspace.fit (art/_cuda_batched_cholesky.py:273-332). */
+int art_gated_ridge_fit(art_ridge_plan* plan, const double* phi, const double* y, const double* pi,
+                        const int* ids_device, const int* ids_host, int count, double ridge,
+                        double weight_floor, double* beta, int* status, double* lambda,
+                        hipStream_t stream);
+
+/* Plan shape: feature count p, Gram work items, 16-column tiles. */
+int art_plan_info(const art_ridge_plan* plan, int* p, int* nchunks, int* tiles);
+
+/* Kernel timing for benchmarks: keep `slots` rings of HIP events recorded on
+ * each fit's stream around the Gram kernel and the solve kernel (0 disables).
+ * No reference counterpart; the reference times phases with CUDA events in
+ * _Timers (art/_cuda_optimizer.py:76-107). */
+int art_plan_timing_enable(art_ridge_plan* plan, int slots);
+
+/* Waits for and returns up to `max` recorded (gram_ms, solve_ms) pairs;
+ * returns how many were written (or a negative error). */
+int art_plan_timing_read(art_ridge_plan* plan, float* gram_ms, float* solve_ms, int max);
+
+/* Frees a plan's device buffers (synchronizes the device).
+ * Replaces BatchedCholeskyWorkspace.close (art/_cuda_batched_cholesky.py:339-352). */
+void art_plan_destroy(art_ridge_plan* plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ART_HIP_H */

@@ -1,0 +1,321 @@
+"""GPU parity of features, gated batched fits and varlen leaf fits vs the oracle.
+
+Parameter grids and known answers follow tests/test_cuda_features.py and
+tests/test_cuda_batched_cholesky.py of the reference.  Coefficient tolerance
+for the FP64 path: rtol 3e-11 / atol 1e-12 on well-conditioned systems, as the
+reference requires of its GPU path (test_cuda_batched_cholesky.py:196-197).
+"""
+
+import numpy as np
+import pytest
+from sklearn.preprocessing import PolynomialFeatures
+
+from oracle import ridge as R
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(torch, a, dtype=None):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype or torch.float64, device="cuda")
+
+
+# --------------------------------------------------------------------------- features
+
+@pytest.mark.parametrize("K,n,d,degree,bias,last", [
+    (2, 7, 3, 1, True, True), (3, 5, 2, 1, True, False), (2, 5, 3, 1, False, False),
+    (2, 7, 1, 2, False, False), (20, 3, 5, 2, True, False), (3, 3, 10, 2, False, False),
+    (2, 2, 20, 2, True, False), (1, 2, 40, 2, True, False), (2, 259, 2, 2, True, False),
+])
+def test_features_match_sklearn_exactly(torch_gpu, K, n, d, degree, bias, last):
+    from art import build_features
+
+    torch = torch_gpu
+    X = np.random.default_rng(901).normal(size=(K, n, d))
+    X[:, 0] = 0.
+    x = dev(torch, X)
+    ptr = x.data_ptr()
+    phi = build_features(x, degree=degree, include_bias=bias, bias_last=last)
+    torch.cuda.synchronize()
+    expected = PolynomialFeatures(degree=degree, include_bias=bias).fit_transform(X.reshape(-1, d))
+    if last:
+        expected = np.concatenate((expected[:, 1:], expected[:, :1]), axis=1)
+    assert tuple(phi.shape) == (K, n, expected.shape[1]) and phi.is_contiguous()
+    np.testing.assert_array_equal(phi.cpu().numpy(), expected.reshape(phi.shape))
+    np.testing.assert_array_equal(x.cpu().numpy(), X)
+    assert x.data_ptr() == ptr
+
+
+def test_quadratic_feature_known_answer(torch_gpu):
+    from art import build_features
+
+    phi = build_features(dev(torch_gpu, [[[2., 3.], [5., 7.]], [[11., 13.], [17., 19.]]]))
+    np.testing.assert_array_equal(phi.cpu().numpy(), [
+        [[1., 2., 3., 4., 6., 9.], [1., 5., 7., 25., 35., 49.]],
+        [[1., 11., 13., 121., 143., 169.], [1., 17., 19., 289., 323., 361.]],
+    ])
+
+
+def test_feature_plan_reuses_supplied_output(torch_gpu):
+    from art import FeaturePlan
+
+    torch, rng = torch_gpu, np.random.default_rng(903)
+    plan = FeaturePlan(3, degree=2, include_bias=False)
+    x = torch.empty((5, 7, 3), dtype=torch.float64, device="cuda")
+    phi = torch.empty((5, 7, plan.p), dtype=torch.float64, device="cuda")
+    address = phi.data_ptr()
+    for k in (5, 2, 1):
+        X = rng.normal(size=(k, 7, 3))
+        x[:k].copy_(torch.from_numpy(X))
+        phi.fill_(-91.)
+        out = phi[:k]
+        assert plan.build(x[:k], out=out) is out
+        torch.cuda.synchronize()
+        expected = PolynomialFeatures(degree=2, include_bias=False).fit_transform(X.reshape(-1, 3))
+        np.testing.assert_array_equal(phi.cpu().numpy()[:k], expected.reshape(k, 7, -1))
+        np.testing.assert_array_equal(phi.cpu().numpy()[k:], -91.)
+        assert phi.data_ptr() == address
+    with pytest.raises(ValueError, match="out"):
+        plan.build(x, out=phi[:, :, :-1])
+
+
+@pytest.mark.parametrize("bad", ["dtype", "strides", "degree", "bias_last", "shape"])
+def test_feature_builder_rejects_unsupported_inputs(torch_gpu, bad):
+    from art import build_features
+
+    torch = torch_gpu
+    x, kwargs = dev(torch, np.ones((2, 5, 3))), {}
+    if bad == "dtype": x = x.float()
+    if bad == "strides": x = x.transpose(1, 2).contiguous().transpose(1, 2)
+    if bad == "degree": kwargs["degree"] = 3
+    if bad == "bias_last": kwargs["bias_last"] = True
+    if bad == "shape": x = x[0]
+    with pytest.raises(ValueError):
+        build_features(x, **kwargs)
+
+
+# --------------------------------------------------------------------------- gated fits
+
+@pytest.mark.parametrize("K,p,ridge,floor", [
+    (1, 1, 0., 0.), (1, 6, 0., 1e-12), (3, 6, .03, .07), (3, 6, 0., .75),
+    (20, 21, 1e-5, 1e-12), (3, 231, 1e-5, .07), (1, 259, .03, 0.),
+])
+def test_gated_fits_match_oracle_and_reuse_storage(torch_gpu, K, p, ridge, floor):
+    from art import BatchedCholeskyWorkspace
+
+    torch, rng = torch_gpu, np.random.default_rng(603)
+    phi, y = rng.normal(size=(K, 5 * p + 4, p)), rng.normal(size=(K, 5 * p + 4))
+    pi = rng.uniform(size=y.shape)
+    pi[:, :4] = [0., 1., 1e-20, 1. - 1e-15]
+    stream = torch.cuda.Stream()
+    with BatchedCholeskyWorkspace(phi, y, ridge, floor, stream=stream) as ws:
+        addresses = [t.data_ptr() for t in (ws.phi, ws.y, ws._rhs, ws._status)]
+        for iteration, gates in enumerate((pi, 1. - pi)):
+            with torch.cuda.stream(stream):
+                device_pi = dev(torch, gates)
+            coefficients, status = ws.fit(device_pi)
+            assert coefficients is ws.coefficients and status is ws.status
+            ws.synchronize()
+            np.testing.assert_array_equal(status.cpu().numpy(), 0)
+            actual = coefficients.cpu().numpy()
+            if iteration == 0:
+                first = actual.copy()
+            for j in range(K):
+                for side in range(2):
+                    q = 1. - gates[j] if side == 0 else gates[j]
+                    ref, lam = R.solve_weighted_ridge(phi[j], y[j], q, ridge, floor)
+                    np.testing.assert_allclose(actual[j, side], ref, rtol=3e-11, atol=1e-12)
+                    assert float(ws.ridge_effective[j, side]) == pytest.approx(lam, rel=1e-12)
+            assert addresses == [t.data_ptr() for t in (ws.phi, ws.y, ws._rhs, ws._status)]
+        phi.fill(0.); y.fill(0.)
+        ws.fit(pi)  # host gates; device data were copied once
+        ws.synchronize()
+        np.testing.assert_array_equal(ws.status.cpu().numpy(), 0)
+        np.testing.assert_allclose(ws.coefficients.cpu().numpy(), first, rtol=2e-12, atol=1e-13)
+    with pytest.raises(RuntimeError, match="closed"):
+        ws.fit(pi)
+
+
+def test_gated_failure_isolation_and_recovery(torch_gpu):
+    from art import BatchedCholeskyWorkspace
+
+    rng = np.random.default_rng(604)
+    phi, y = rng.normal(size=(3, 20, 3)), rng.normal(size=(3, 20))
+    phi[1, :, 2] = 0.  # Both fits at node 1 fail at pivot 3.
+    pi = np.full((3, 20), .4)
+    pi[2] = 0.  # Only the right fit at node 2 is empty.
+    with BatchedCholeskyWorkspace(phi, y, weight_floor=0.) as ws:
+        coefficients, status = ws.fit(pi)
+        ws.synchronize()
+        np.testing.assert_array_equal(status.cpu().numpy(), [[0, 0], [3, 3], [0, 1]])
+        result = coefficients.cpu().numpy()
+        assert np.isfinite(result[0]).all() and np.isfinite(result[2, 0]).all()
+        assert np.isnan(result[1]).all() and np.isnan(result[2, 1]).all()
+        for side, q in enumerate((1. - pi[0], pi[0])):
+            ref, _ = R.solve_weighted_ridge(phi[0], y[0], q, 0., 0.)
+            np.testing.assert_allclose(result[0, side], ref, rtol=2e-12, atol=1e-13)
+        pi[0, 0] = np.nan
+        ws.fit(pi); ws.synchronize()
+        np.testing.assert_array_equal(ws.status.cpu().numpy()[0], [-1, -1])
+        pi[0, 0] = .4
+        pi[2] = .4
+        ws.fit(pi); ws.synchronize()
+        np.testing.assert_array_equal(ws.status.cpu().numpy(), [[0, 0], [3, 3], [0, 0]])
+
+
+@pytest.mark.parametrize("value", [np.nan, np.inf, -.1, 1.1])
+def test_gated_bad_gate_marks_only_its_node(torch_gpu, value):
+    from art import BatchedCholeskyWorkspace
+
+    pi = np.full((3, 8), .4)
+    pi[1, 2] = value
+    with BatchedCholeskyWorkspace(np.ones((3, 8, 1)), np.ones((3, 8))) as ws:
+        ws.fit(pi); ws.synchronize()
+        np.testing.assert_array_equal(ws.status.cpu().numpy().T, [[0, -1, 0], [0, -1, 0]])
+        pi[1, 2] = .4
+        ws.fit(pi); ws.synchronize()
+        np.testing.assert_array_equal(ws.status.cpu().numpy(), 0)
+
+
+def test_gated_solve_overflow_is_status_minus_three(torch_gpu):
+    from art import BatchedCholeskyWorkspace
+
+    # One-feature system 1e-300 * b = 1e10 -> inf: factor succeeds, solve overflows.
+    phi = np.full((1, 4, 1), 1e-150)
+    y = np.full((1, 4), 1e160)
+    with BatchedCholeskyWorkspace(phi, y, weight_floor=0.) as ws:
+        ws.fit(np.full((1, 4), .5)); ws.synchronize()
+        np.testing.assert_array_equal(ws.status.cpu().numpy(), [[-3, -3]])
+        assert np.isnan(ws.coefficients.cpu().numpy()).all()
+    _, st, _ = R.batched_gate_fit(phi, y, np.full((1, 4), .5), floor=0.)
+    np.testing.assert_array_equal(st, [[-3, -3]])
+
+
+def test_gated_subset_nodes_keep_other_slots(torch_gpu):
+    from art import BatchedCholeskyWorkspace
+
+    rng = np.random.default_rng(902)
+    phi, y, gates = rng.normal(size=(4, 31, 5)), rng.normal(size=(4, 31)), rng.uniform(size=(4, 31))
+    with BatchedCholeskyWorkspace.allocate(6, 31, 5, ridge=.01) as ws:
+        ws.load(phi, y)
+        ws.fit(gates)
+        ws.synchronize()
+        before = ws.coefficients.cpu().numpy()
+        ws.fit(1. - gates, nodes=[2, 0])
+        ws.synchronize()
+        after = ws.coefficients.cpu().numpy()
+        np.testing.assert_array_equal(after[[1, 3]], before[[1, 3]])
+        assert ws.last_batch_size == 4
+        for j in (2, 0):
+            for side, q in enumerate((gates[j], 1. - gates[j])):
+                ref, _ = R.solve_weighted_ridge(phi[j], y[j], q, .01)
+                np.testing.assert_allclose(after[j, side], ref, rtol=3e-11, atol=1e-12)
+        with pytest.raises(ValueError, match="distinct"):
+            ws.fit(gates, nodes=[0, 0])
+        with pytest.raises(ValueError, match="distinct"):
+            ws.fit(gates, nodes=[4])
+
+
+def test_gated_input_validation(torch_gpu):
+    from art import BatchedCholeskyWorkspace
+
+    phi, y = np.ones((2, 8, 2)), np.ones((2, 8))
+    phi[0, 0, 0] = np.nan
+    with pytest.raises(ValueError, match="finite"):
+        BatchedCholeskyWorkspace(phi, y)
+    phi[0, 0, 0], y[0, 0] = 1., np.inf
+    with pytest.raises(ValueError, match="finite"):
+        BatchedCholeskyWorkspace(phi, y)
+    y[0, 0] = 1.
+    with BatchedCholeskyWorkspace(phi, y, ridge=.1) as ws:
+        with pytest.raises(ValueError, match="shape"):
+            ws.fit(np.ones(8))
+        with pytest.raises(ValueError, match="real"):
+            ws.fit(np.ones((2, 8), dtype=complex))
+    with pytest.raises(RuntimeError, match="Load data"):
+        BatchedCholeskyWorkspace.allocate(2, 8, 2).fit(np.ones((2, 8)))
+
+
+def test_borrowed_device_design(torch_gpu):
+    from art import BatchedCholeskyWorkspace, build_features
+
+    torch, rng = torch_gpu, np.random.default_rng(902)
+    X, y = rng.normal(size=(3, 31, 3)), rng.normal(size=(3, 31))
+    gates = rng.uniform(size=y.shape)
+    x, target = dev(torch, X), dev(torch, y)
+    phi = build_features(x)
+    with BatchedCholeskyWorkspace(phi, target, ridge=.01, copy=False) as ws:
+        assert ws.phi is phi and ws.y is target
+        ws.fit(dev(torch, gates), nodes=[2, 0])
+        ws.synchronize()
+        actual = ws.coefficients.cpu().numpy()
+        expected_phi = R.features(X)
+        for j in (2, 0):
+            for side, q in enumerate((1. - gates[j], gates[j])):
+                ref, _ = R.solve_weighted_ridge(expected_phi[j], y[j], q, .01)
+                np.testing.assert_allclose(actual[j, side], ref, rtol=3e-11, atol=1e-12)
+    with BatchedCholeskyWorkspace(phi, target) as ws:
+        assert ws.phi.data_ptr() != phi.data_ptr()
+    with pytest.raises(ValueError, match="copy=False"):
+        BatchedCholeskyWorkspace(phi.cpu(), target, copy=False)
+
+
+# --------------------------------------------------------------------------- varlen leaves
+
+def _rel_err(a, b):
+    return np.linalg.norm(a - b, axis=-1) / np.maximum(np.linalg.norm(b, axis=-1), 1e-300)
+
+
+@pytest.mark.parametrize("d,degree,bias,last,chunk", [
+    (2, 2, True, False, 0), (8, 2, True, False, 0), (8, 2, True, False, 64),
+    (12, 2, True, False, 0), (16, 1, True, True, 0), (20, 1, True, False, 0), (3, 2, False, False, 16),
+])
+def test_leaf_fits_match_oracle(torch_gpu, d, degree, bias, last, chunk):
+    from art import LeafRidgeBatch
+
+    torch, rng = torch_gpu, np.random.default_rng(1000 + d)
+    sizes = np.array([300, 1, 17, 64, 129, 2048 + 7, 0, 500])
+    offsets = np.concatenate([[0], np.cumsum(sizes)])
+    N = offsets[-1]
+    X = rng.uniform(-1, 1, size=(N, d))
+    y, w = np.sin(3 * X[:, 0]) + X[:, -1] ** 2, rng.uniform(0.05, 1.0, size=N)
+    ref, st_ref, lam_ref = R.leaf_fit(X, y, w, offsets, 1e-3, 1e-12, degree, bias, last)
+    with LeafRidgeBatch(offsets, d, degree=degree, include_bias=bias, bias_last=last,
+                        chunk_rows=chunk) as batch:
+        coef, status, lam = batch.fit(dev(torch, X), dev(torch, y), dev(torch, w), 1e-3)
+        batch.synchronize()
+        st = status.cpu().numpy()
+        np.testing.assert_array_equal(st, st_ref)
+        ok = st == 0
+        got = coef.cpu().numpy()
+        assert np.isnan(got[~ok]).all()
+        assert _rel_err(got[ok], ref[ok]).max() < 1e-9
+        np.testing.assert_allclose(lam.cpu().numpy(), lam_ref, rtol=1e-12)
+
+
+def test_leaf_bad_weights_flag_only_their_leaf(torch_gpu):
+    from art import LeafRidgeBatch
+
+    torch, rng = torch_gpu, np.random.default_rng(11)
+    offsets = np.array([0, 40, 80, 120, 160])
+    X, y = rng.uniform(-1, 1, size=(160, 2)), rng.normal(size=160)
+    w = rng.uniform(size=160)
+    w[45], w[130] = np.nan, -1.0
+    with LeafRidgeBatch(offsets, 2) as batch:
+        _, status, _ = batch.fit(dev(torch, X), dev(torch, y), dev(torch, w), 1e-6)
+        np.testing.assert_array_equal(status.cpu().numpy(), [0, -1, 0, -1])
+
+
+def test_leaf_workload_subset_matches_oracle(torch_gpu):
+    """Synthetic leaf-ridge workload (d=8 quadratic): FP64 GPU vs FP64 oracle."""
+    from art import LeafRidgeBatch
+    from art.synthetic import leaf_ridge_workload
+
+    torch = torch_gpu
+    wl = leaf_ridge_workload(n_leaves=64, seed=2)
+    ref, st_ref, _ = R.leaf_fit(wl.X, wl.y, wl.w, wl.offsets, wl.ridge)
+    with LeafRidgeBatch(wl.offsets, 8) as batch:
+        coef, status, _ = batch.fit(dev(torch, wl.X), dev(torch, wl.y), dev(torch, wl.w), wl.ridge)
+        np.testing.assert_array_equal(status.cpu().numpy(), st_ref)
+        # Both sides are FP64; they differ only by rounding amplified by the
+        # leaf condition number (up to ~1e7 here).
+        assert _rel_err(coef.cpu().numpy(), ref).max() < 1e-7

@@ -1,0 +1,63 @@
+"""Argument validation happens before torch or the HIP library is touched.
+
+Mirrors tests/test_cuda_batched_cholesky.py:132-152 of the reference, where
+validation must precede any CuPy import; here it must precede torch.
+"""
+
+import builtins
+
+import numpy as np
+import pytest
+
+
+def _guard(monkeypatch):
+    original = builtins.__import__
+
+    def guarded(name, *args, **kw):
+        if name == "torch" or name.startswith("torch."):
+            raise AssertionError("Validation must not require torch")
+        return original(name, *args, **kw)
+
+    monkeypatch.setattr(builtins, "__import__", guarded)
+
+
+@pytest.mark.parametrize("phi,y,kwargs,message", [
+    (np.ones((2, 3)), np.ones(3), {}, "Phi"),
+    (np.ones((0, 3, 2)), np.ones((0, 3)), {}, "Phi"),
+    (np.ones((2, 3, 2)), np.ones((2, 3, 1)), {}, "y"),
+    (np.ones((2, 3, 2)), np.ones((2, 3)), {"ridge": -1}, "ridge"),
+    (np.ones((2, 3, 2)), np.ones((2, 3)), {"ridge": np.nan}, "ridge"),
+    (np.ones((2, 3, 2)), np.ones((2, 3)), {"weight_floor": np.inf}, "weight_floor"),
+    (np.ones((2, 3, 2)), np.ones((2, 3)), {"weight_floor": -1}, "weight_floor"),
+    (np.ones((2, 3, 2), dtype=complex), np.ones((2, 3)), {}, "real"),
+])
+def test_batched_workspace_validation_precedes_torch(monkeypatch, phi, y, kwargs, message):
+    from art.batched_ridge import BatchedCholeskyWorkspace
+
+    _guard(monkeypatch)
+    with pytest.raises(ValueError, match=message):
+        BatchedCholeskyWorkspace(phi, y, **kwargs)
+
+
+@pytest.mark.parametrize("kwargs,message", [
+    ({"degree": 3}, "degree"), ({"degree": 2, "bias_last": True}, "bias_last"),
+])
+def test_feature_plan_validation_precedes_torch(monkeypatch, kwargs, message):
+    from art.features import FeaturePlan
+
+    _guard(monkeypatch)
+    with pytest.raises(ValueError, match=message):
+        FeaturePlan(3, **kwargs)
+    with pytest.raises(ValueError, match="positive"):
+        FeaturePlan(0)
+
+
+@pytest.mark.parametrize("offsets,message", [
+    ([1, 5], "CSR"), ([0, 5, 3], "nondecreasing"), ([[0, 1]], "CSR"),
+])
+def test_leaf_batch_validation_precedes_torch(monkeypatch, offsets, message):
+    from art.leaf_ridge import LeafRidgeBatch
+
+    _guard(monkeypatch)
+    with pytest.raises(ValueError, match=message):
+        LeafRidgeBatch(offsets, 2)
