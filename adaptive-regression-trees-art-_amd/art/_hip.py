@@ -43,6 +43,8 @@ _PROTOTYPES = {
     "art_gated_plan_create": (_i, [_i, _i64, _i, _i, ctypes.POINTER(_p)]),
     "art_gated_ridge_fit": (_i, [_p, _p, _p, _p, _p, _p, _i, _d, _d, _p, _p, _p, _p]),
     "art_plan_info": (_i, [_p, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "art_leaf_fallback_count": (_i, [_p, ctypes.POINTER(_i)]),
+    "art_leaf_set_refine_tol": (_i, [_p, _d]),
     "art_plan_timing_enable": (_i, [_p, _i]),
     "art_plan_timing_read": (_i, [_p, _p, _p, _i]),
     "art_plan_destroy": (None, [_p]),

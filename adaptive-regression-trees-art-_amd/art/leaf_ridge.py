@@ -76,6 +76,13 @@ class LeafRidgeBatch:
                 raise ValueError(f"{name} must have shape {shape}.")
 
     def fit(self, X, y, w, ridge, weight_floor=1e-12, precision="fp64"):
+        """Enqueue one fit of every leaf.
+
+        precision="fp64": FP64 Gram (MFMA) and FP64 Cholesky.
+        precision="mixed" (p <= 64): split-bf16 MFMA Gram with an exact FP64
+        diagonal and right-hand side, one FP64 residual refinement, and the
+        FP64 path for every leaf whose correction exceeds the tolerance.
+        """
         if self.plan is None:
             raise RuntimeError("The plan is closed.")
         if precision not in PRECISIONS:
@@ -91,6 +98,16 @@ class LeafRidgeBatch:
             PRECISIONS[precision], _hip.ptr(self.coefficients), _hip.ptr(self.status),
             _hip.ptr(self.ridge_effective), _hip.stream_handle(self.stream)))
         return self.coefficients, self.status, self.ridge_effective
+
+    def fallback_count(self):
+        """Leaves that took the exact FP64 path in the last mixed fit (synchronizes)."""
+        count = ctypes.c_int()
+        _hip.check(self.lib.art_leaf_fallback_count(self.plan, ctypes.byref(count)))
+        return count.value
+
+    def set_refine_tol(self, tol):
+        """Acceptance tolerance |delta| <= tol |beta| of the mixed refinement."""
+        _hip.check(self.lib.art_leaf_set_refine_tol(self.plan, float(tol)))
 
     def synchronize(self):
         self.stream.synchronize()

@@ -8,6 +8,7 @@ namespace art {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kWave = 64;
 
@@ -67,10 +68,20 @@ __device__ inline int wave_or(int v) {
   return v;
 }
 
+// Value of v in lane `src` (src uniform across the wave).
+__device__ inline double readlane_f64(double v, int src) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)bits, src);
+  const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), src);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // Row range of one Gram work item (a chunk of one leaf's rows).
 struct LeafChunk {
   int64_t begin;
   int64_t end;
+  int leaf;
+  int pad;
 };
 
 // One ridge system: its chunks [chunk_begin, chunk_end), output slot and side.

@@ -211,6 +211,181 @@ __global__ __launch_bounds__(256) void gram_f64_kernel(Src src, Chunks chunks, i
 }
 
 // ---------------------------------------------------------------------------
+// Row staging for the leaf kernels.  A chunk's rows stream through LDS in
+// blocks of rb rows, double buffered: the direct global->LDS copies
+// (global_load_lds_dword, LDS slot = wave base + 4 * lane) of block i+1 are
+// in flight while block i is computed, with one barrier per block.  One LDS
+// buffer holds, in doubles:
+//   [0, rb*d) X rows, [rb*d, rb*(d+1)) y, [rb*(d+1), rb*(d+2)) w, then 1.0, 0.0.
+// Every column of the augmented design [phi | y | 0-padding] is the product
+// of two buffer entries addressed as off + r * stride (stride 0 for the
+// constants), so a lane builds a feature with two LDS reads and one multiply.
+constexpr int kMaxStagedD = 24;
+
+typedef __attribute__((address_space(3))) void* lds_void_ptr;
+
+__host__ __device__ inline int stage_rows_per_block(int d) { return d <= 10 ? 256 : 128; }
+__host__ __device__ inline int stage_buffer_doubles(int d) {
+  return stage_rows_per_block(d) * (d + 2) + 2;
+}
+
+size_t leaf_stage_smem_bytes(int d) {
+  const size_t n = 2 * (size_t)stage_buffer_doubles(d);
+  return (n > 1024 ? n : 1024) * sizeof(double);
+}
+
+__device__ inline void copy_dwords_async(double* dst, const void* src, int n, int wave, int lane) {
+  unsigned* d32 = reinterpret_cast<unsigned*>(dst);
+  const unsigned* s32 = reinterpret_cast<const unsigned*>(src);
+  for (int k = wave; k * 64 < n; k += 4) {
+    const int idx = k * 64 + lane;
+    if (idx < n)
+      __builtin_amdgcn_global_load_lds((const void*)(s32 + idx), (lds_void_ptr)(d32 + k * 64), 4,
+                                       0, 0);
+  }
+}
+
+__device__ inline void stage_rows_async(double* buf, const double* X, const double* y,
+                                        const double* w, int d, int rb, int64_t r0, int nrows) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  copy_dwords_async(buf, X + r0 * d, nrows * d * 2, wave, lane);
+  copy_dwords_async(buf + rb * d, y + r0, nrows * 2, wave, lane);
+  copy_dwords_async(buf + rb * (d + 1), w + r0, nrows * 2, wave, lane);
+}
+
+// Zero both buffers (stale rows past a chunk's end must stay finite) and set
+// the constant slots.
+__device__ inline void stage_init(double* smem, int d) {
+  const int n = 2 * stage_buffer_doubles(d);
+  for (int e = threadIdx.x; e < n; e += blockDim.x) smem[e] = 0.0;
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    const int one = stage_buffer_doubles(d) - 2;
+    smem[one + threadIdx.x] = threadIdx.x == 0 ? 1.0 : 0.0;
+    smem[stage_buffer_doubles(d) + one + threadIdx.x] = threadIdx.x == 0 ? 1.0 : 0.0;
+  }
+  __syncthreads();
+}
+
+// Buffer-relative address (offset, row stride) of one factor of column f.
+struct Factor {
+  int off, stride;
+};
+
+__device__ inline void column_factors(const Basis& B, int rb, int f, Factor& fa, Factor& fb) {
+  const int d = B.d;
+  const int one = rb * (d + 2), zero = one + 1;
+  int a = -1, b = -1;
+  if (f < B.p) {
+    basis_factors(B, f, a, b);
+    fa = a >= 0 ? Factor{a, d} : Factor{one, 0};
+    fb = b >= 0 ? Factor{b, d} : Factor{one, 0};
+  } else if (f == B.p) {
+    fa = Factor{rb * d, 1};  // target column y
+    fb = Factor{one, 0};
+  } else {
+    fa = Factor{zero, 0};
+    fb = Factor{one, 0};
+  }
+}
+
+__device__ inline double staged_feature(const double* buf, const Factor& fa, const Factor& fb,
+                                        int r) {
+  return buf[fa.off + r * fa.stride] * buf[fb.off + r * fb.stride];
+}
+
+// Floored weight of staged row r; 0 for rows past the block's end and for
+// invalid weights (negative, NaN or inf), which set `bad`.
+// Branch-free so that the LDS reads of consecutive rows can be batched; the
+// slot of a row past the end holds stale finite data and is masked out.
+__device__ inline double staged_weight(const double* buf, int d, int rb, int r, int nrows,
+                                       double floor, int& bad) {
+  const double v = buf[rb * (d + 1) + r];
+  const bool inside = r < nrows;
+  const bool ok = (v >= 0.0) & !isinf(v);
+  bad |= (int)(inside & !ok);
+  return (inside & ok) ? fmax(v, floor) : 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 Gram for leaves whose features are built from X (the plain leaf-ridge
+// path and the exact fallback of the mixed path).  Lane (col, kq) of wave w
+// takes row 16 s + 4 w + kq of step s and features 16 t + col; each lane
+// builds its T feature values once per step and feeds all T(T+1)/2 tile
+// pairs through v_mfma_f64_16x16x4.  `mask`, when given, skips chunks whose
+// leaf is not flagged.
+template <int T>
+__global__ __launch_bounds__(256) void gram_leaf_f64_kernel(
+    const double* __restrict__ X, const double* __restrict__ y, const double* __restrict__ w,
+    Basis B, double floor, const LeafChunk* __restrict__ chunks, const int* __restrict__ mask,
+    double* __restrict__ partials, int* __restrict__ flags) {
+  constexpr int NP = T * (T + 1) / 2;
+  if (mask != nullptr && !mask[chunks[blockIdx.x].leaf]) return;
+  extern __shared__ double smem[];
+  const int d = B.d, rb = stage_rows_per_block(d), bsz = stage_buffer_doubles(d);
+  const int c = blockIdx.x;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int col = lane & 15, kq = lane >> 4;
+  const int64_t begin = chunks[c].begin, end = chunks[c].end;
+
+  Factor fa[T], fb[T];
+#pragma unroll
+  for (int t = 0; t < T; ++t) column_factors(B, rb, 16 * t + col, fa[t], fb[t]);
+  f64x4 acc[NP];
+#pragma unroll
+  for (int q = 0; q < NP; ++q) acc[q] = f64x4{0.0, 0.0, 0.0, 0.0};
+
+  stage_init(smem, d);
+  int bad = 0;
+  int64_t r0 = begin;
+  int nrows = (int)(end - r0 < rb ? end - r0 : rb);
+  if (nrows > 0) stage_rows_async(smem, X, y, w, d, rb, r0, nrows);
+  for (int blk = 0; r0 < end; ++blk) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    const int64_t n0 = r0 + rb;
+    const int nn = n0 < end ? (int)(end - n0 < rb ? end - n0 : rb) : 0;
+    if (nn > 0) stage_rows_async(smem + ((blk + 1) & 1) * bsz, X, y, w, d, rb, n0, nn);
+    const double* buf = smem + (blk & 1) * bsz;
+    const int steps = (nrows + 15) >> 4;
+    for (int s = 0; s < steps; ++s) {
+      const int r = 16 * s + 4 * wave + kq;
+      const double wt = staged_weight(buf, d, rb, r, nrows, floor, bad);
+      double phi[T], a[T];
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        phi[t] = staged_feature(buf, fa[t], fb[t], r);
+        a[t] = wt * phi[t];
+      }
+      int q = 0;
+#pragma unroll
+      for (int t1 = 0; t1 < T; ++t1)
+#pragma unroll
+        for (int t2 = t1; t2 < T; ++t2) {
+          acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t1], phi[t2], acc[q], 0, 0, 0);
+          ++q;
+        }
+    }
+    r0 = n0;
+    nrows = nn;
+  }
+
+  double* red = smem;
+  const size_t base = (size_t)c * NP;
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) red[wave * 256 + (kq + 4 * r) * 16 + col] = acc[q][r];
+    __syncthreads();
+    const double v = ((red[tid] + red[256 + tid]) + red[512 + tid]) + red[768 + tid];
+    partials[(base + q) * 256 + tid] = v;
+  }
+  bad = __syncthreads_or(bad);
+  if (tid == 0) flags[c] = bad;
+}
+
+// ---------------------------------------------------------------------------
 // Reduce + regularize + factor + solve.  One wave per system.
 //
 // Status semantics follow art/_cuda_batched_cholesky_kernels.cu:29-105:
@@ -329,6 +504,423 @@ __global__ __launch_bounds__(64) void solve_f64_kernel(const SystemDesc* __restr
 }
 
 // ---------------------------------------------------------------------------
+// Row-per-thread systems (p <= kRowsMaxP): thread i owns row i of the packed
+// lower triangle in LDS.  The Cholesky runs in Crout order -- column k is
+// s_i = A[i][k] - sum_{j<k} L[i][j] L[k][j], a dot product of the thread's own
+// row with row k (a broadcast read) -- and the substitutions are
+// column-oriented with one barrier per step.  The code is compact and its
+// loops are not unrolled over p.
+constexpr int kRowsMaxP = 120;
+
+__device__ inline int rows_threads(int p) { return 64 * ((p + 63) / 64); }
+
+// Crout Cholesky of rows 0..p-1 of packed M; returns info (uniform).
+__device__ inline int rows_cholesky(double* M, double* piv, int p, int i) {
+  for (int k = 0; k < p; ++k) {
+    double s = 0.0;
+    if (i >= k && i < p) {
+      const double* ri = M + tri(i, 0);
+      const double* rk = M + tri(k, 0);
+      double s0 = ri[k], s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      int j = 0;
+      for (; j + 3 < k; j += 4) {
+        s0 -= ri[j] * rk[j];
+        s1 -= ri[j + 1] * rk[j + 1];
+        s2 -= ri[j + 2] * rk[j + 2];
+        s3 -= ri[j + 3] * rk[j + 3];
+      }
+      for (; j < k; ++j) s0 -= ri[j] * rk[j];
+      s = (s0 + s1) + (s2 + s3);
+    }
+    if (i == k) *piv = s;
+    __syncthreads();
+    const double dkk = *piv;
+    if (!(dkk > 0.0)) return k + 1;
+    const double lkk = sqrt(dkk);
+    if (i == k) M[tri(k, k)] = lkk;
+    else if (i > k && i < p) M[tri(i, k)] = s / lkk;
+    __syncthreads();
+  }
+  return 0;
+}
+
+// Solve L L^T x = b with L in packed M; b and the result are thread i's entry.
+__device__ inline double rows_solve(const double* M, double* bc, int p, int i, double b) {
+  double z = i < p ? b : 0.0;
+  for (int k = 0; k < p; ++k) {
+    if (i == k) {
+      z = z / M[tri(k, k)];
+      bc[k & 1] = z;
+    }
+    __syncthreads();
+    const double zk = bc[k & 1];
+    if (i > k && i < p) z -= M[tri(i, k)] * zk;
+  }
+  __syncthreads();
+  double x = z;
+  for (int k = p - 1; k >= 0; --k) {
+    if (i == k) {
+      x = x / M[tri(k, k)];
+      bc[k & 1] = x;
+    }
+    __syncthreads();
+    const double xk = bc[k & 1];
+    if (i < k) x -= M[tri(k, i)] * xk;
+  }
+  return x;
+}
+
+__device__ inline int block_or(int v, int* scratch) {
+  v = wave_or(v);
+  if (blockDim.x == 64) return v;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) scratch[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int r = 0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r |= scratch[w];
+  __syncthreads();
+  return r;
+}
+
+__device__ inline double block_sum(double v, double* scratch) {
+  v = wave_sum(v);
+  if (blockDim.x == 64) return v;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) scratch[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = 0.0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += scratch[w];
+  __syncthreads();
+  return r;
+}
+
+// Reduce + regularize + factor + solve, same contract as solve_f64_kernel;
+// optionally stores the factor rows (packed) and skips unmasked systems.
+__global__ __launch_bounds__(128) void solve_rows_kernel(
+    const SystemDesc* __restrict__ systems, int nchunks, int T, int npairs, int p, double ridge,
+    const double* __restrict__ partials, const int* __restrict__ flags,
+    const double* __restrict__ tpart, const int* __restrict__ mask, double* __restrict__ factor,
+    double* __restrict__ beta, int* __restrict__ status, double* __restrict__ lambda) {
+  extern __shared__ double M[];  // packed lower (p+1) x (p+1): G and the row h
+  __shared__ double bc[2];
+  __shared__ double red[2];
+  __shared__ int ired[2];
+  const SystemDesc S = systems[blockIdx.x];
+  if (mask != nullptr && !mask[S.out]) return;
+  const int i = threadIdx.x, nt = blockDim.x;
+  const int nel = (p + 1) * (p + 2) / 2;
+  for (int e = i; e < nel; e += nt) M[e] = 0.0;
+  int bad = 0;
+  for (int ch = S.chunk_begin + i; ch < S.chunk_end; ch += nt)
+    bad |= flags[(size_t)S.side * nchunks + ch];
+  __syncthreads();
+  // Tile (t1, t2) element (a, b) is G[16 t1 + a][16 t2 + b] with t1 <= t2;
+  // it lands in the packed lower slot (16 t2 + b, 16 t1 + a).  Loads are
+  // issued in batches of 8 before their LDS accumulations.
+  const int total = npairs * 256;
+  for (int ch = S.chunk_begin; ch < S.chunk_end; ++ch) {
+    const double* base = partials + ((size_t)S.side * nchunks + ch) * total;
+    for (int e0 = i; e0 < total; e0 += 8 * nt) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + u * nt;
+        v[u] = e < total ? base[e] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + u * nt;
+        if (e < total) {
+          int t1, t2;
+          pair_tiles(e >> 8, T, t1, t2);
+          const int r = 16 * t1 + ((e >> 4) & 15), c = 16 * t2 + (e & 15);
+          if (r <= c && c <= p) M[tri(c, r)] += v[u];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  bad = block_or(bad, ired);
+  // The trace that fixes lambda: the Gram diagonal, or the exact per-chunk
+  // traces when the Gram itself is approximate.
+  double tpiece = 0.0;
+  if (tpart != nullptr) {
+    for (int ch = S.chunk_begin + i; ch < S.chunk_end; ch += nt) tpiece += tpart[ch];
+  } else if (i < p) {
+    tpiece = M[tri(i, i)];
+  }
+  const double tr = block_sum(tpiece, red);
+  const double lam = ridge == 0.0 ? 0.0 : ridge * (tr / p);
+  int st = bad ? -1 : ((!isfinite(tr) || !isfinite(lam)) ? -2 : 0);
+  double x = 0.0;
+  if (st == 0) {
+    if (i < p) M[tri(i, i)] += lam;
+    __syncthreads();
+    st = rows_cholesky(M, bc, p, i);
+    if (st == 0) {
+      int nf = 0;
+      if (i < p)
+        for (int j = 0; j <= i; ++j) nf |= !isfinite(M[tri(i, j)]);
+      if (i < p) nf |= !isfinite(M[tri(p, i)]);
+      if (block_or(nf, ired)) st = -2;
+    }
+    if (st == 0 && factor != nullptr && i < p) {
+      double* row = factor + (size_t)S.out * (p * (p + 1) / 2) + tri(i, 0);
+      for (int j = 0; j <= i; ++j) row[j] = M[tri(i, j)];
+    }
+    if (st == 0) {
+      x = rows_solve(M, bc, p, i, i < p ? M[tri(p, i)] : 0.0);
+      if (block_or(i < p && !isfinite(x), ired)) st = -3;
+    }
+  }
+  if (i < p) beta[(size_t)S.out * p + i] = st == 0 ? x : __builtin_nan("");
+  if (i == 0) {
+    status[S.out] = st;
+    lambda[S.out] = lam;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision leaf fits.
+//
+// 1. gram_leaf_mixed_kernel: the augmented Gram [Phi | y]^T W [Phi | y] by
+//    split-bf16 MFMA (v = hi + lo, hi*hi + hi*lo + lo*hi, 32 rows per
+//    v_mfma_f32_16x16x32_bf16), accumulated in FP32 over one 32-row step and
+//    then in FP64, plus the exact FP64 trace per chunk.  Partials have the
+//    FP64 kernel's layout, so the same solve kernel factors them.
+// 2. solve_rows_kernel (with factor output): beta0 = (G~ + lambda I)^-1 h~,
+//    lambda from the exact trace.
+// 3. residual_leaf_kernel: FP64 data-space residual Phi^T W (y - Phi beta0)
+//    per chunk (a second pass over X, y, w).
+// 4. correct_rows_kernel: beta1 = beta0 + (L L^T)^-1 (r - lambda beta0); a
+//    leaf whose correction exceeds tau * |beta1| (or whose mixed solve
+//    failed) is flagged for the exact FP64 path, which then runs only on the
+//    flagged leaves' chunks.
+
+__device__ inline void split_bf16(float v, __bf16& hi, __bf16& lo) {
+  hi = (__bf16)v;
+  lo = (__bf16)(v - (float)hi);
+}
+
+// Split-bf16 Gram.  Wave w takes the 32-row steps s = w, w+4, ...; lane
+// (col, kq) holds rows 32 s + 8 kq + e (e < 8) of features 16 t + col, the
+// operand layout of v_mfma_f32_16x16x32_bf16.  Each step's FP32 tile is added
+// into FP64 accumulators.  The trace of the Gram (which fixes lambda) is
+// accumulated exactly in FP64, one thread per row, from
+// |phi|^2 = bias + s1 [+ (s1^2 + s2) / 2 for the quadratic basis],
+// s1 = sum x_i^2, s2 = sum x_i^4; it goes to tpart[chunk].
+template <int T>
+__global__ __launch_bounds__(256) void gram_leaf_mixed_kernel(
+    const double* __restrict__ X, const double* __restrict__ y, const double* __restrict__ w,
+    Basis B, double floor, const LeafChunk* __restrict__ chunks, double* __restrict__ partials,
+    double* __restrict__ tpart, int* __restrict__ flags) {
+  constexpr int NP = T * (T + 1) / 2;
+  extern __shared__ double smem[];
+  __shared__ double tred[4];
+  const int d = B.d, rb = stage_rows_per_block(d), bsz = stage_buffer_doubles(d);
+  const int c = blockIdx.x;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int col = lane & 15, kq = lane >> 4;
+  const int64_t begin = chunks[c].begin, end = chunks[c].end;
+  const double bias = B.bias ? 1.0 : 0.0;
+  const double quad = B.degree == 2 ? 0.5 : 0.0;
+
+  Factor fa[T], fb[T];
+#pragma unroll
+  for (int t = 0; t < T; ++t) column_factors(B, rb, 16 * t + col, fa[t], fb[t]);
+  f64x4 acc[NP];
+#pragma unroll
+  for (int q = 0; q < NP; ++q) acc[q] = f64x4{0.0, 0.0, 0.0, 0.0};
+
+  stage_init(smem, d);
+  int bad = 0;
+  double tr = 0.0;
+  int64_t r0 = begin;
+  int nrows = (int)(end - r0 < rb ? end - r0 : rb);
+  if (nrows > 0) stage_rows_async(smem, X, y, w, d, rb, r0, nrows);
+  for (int blk = 0; r0 < end; ++blk) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    const int64_t n0 = r0 + rb;
+    const int nn = n0 < end ? (int)(end - n0 < rb ? end - n0 : rb) : 0;
+    if (nn > 0) stage_rows_async(smem + ((blk + 1) & 1) * bsz, X, y, w, d, rb, n0, nn);
+    const double* buf = smem + (blk & 1) * bsz;
+    // exact trace, one thread per row
+    for (int r = tid; r < nrows; r += 256) {
+      const double wt = staged_weight(buf, d, rb, r, nrows, floor, bad);
+      double s1 = 0.0, s2 = 0.0;
+      for (int i = 0; i < d; ++i) {
+        const double x2 = buf[r * d + i] * buf[r * d + i];
+        s1 += x2;
+        s2 += x2 * x2;
+      }
+      tr += wt * (bias + s1 + quad * (s1 * s1 + s2));
+    }
+    const int steps = (nrows + 31) >> 5;
+    for (int s = wave; s < steps; s += 4) {
+      bf16x8 ah[T], al[T], bh[T], bl[T];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int r = 32 * s + 8 * kq + e;
+        int unused = 0;
+        const float wt = (float)staged_weight(buf, d, rb, r, nrows, floor, unused);
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+          const float phi = (float)staged_feature(buf, fa[t], fb[t], r);
+          __bf16 h0, l0, h1, l1;
+          split_bf16(wt * phi, h0, l0);
+          split_bf16(phi, h1, l1);
+          ah[t][e] = h0;
+          al[t][e] = l0;
+          bh[t][e] = h1;
+          bl[t][e] = l1;
+        }
+      }
+      int q = 0;
+#pragma unroll
+      for (int t1 = 0; t1 < T; ++t1)
+#pragma unroll
+        for (int t2 = t1; t2 < T; ++t2) {
+          f32x4 z = {0.f, 0.f, 0.f, 0.f};
+          z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[t1], bl[t2], z, 0, 0, 0);
+          z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[t1], bh[t2], z, 0, 0, 0);
+          z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[t1], bh[t2], z, 0, 0, 0);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc[q][r] += (double)z[r];
+          ++q;
+        }
+    }
+    r0 = n0;
+    nrows = nn;
+  }
+
+  double* red = smem;
+  const size_t base = (size_t)c * NP;
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) red[wave * 256 + (4 * kq + r) * 16 + col] = acc[q][r];
+    __syncthreads();
+    const double v = ((red[tid] + red[256 + tid]) + red[512 + tid]) + red[768 + tid];
+    partials[(base + q) * 256 + tid] = v;
+  }
+  tr = wave_sum(tr);
+  if (lane == 0) tred[wave] = tr;
+  bad = __syncthreads_or(bad);
+  if (tid == 0) {
+    flags[c] = bad;
+    tpart[c] = ((tred[0] + tred[1]) + tred[2]) + tred[3];
+  }
+}
+
+// r_chunk = sum over the chunk's rows of w (y - phi . beta) phi, in FP64.
+// Lane (col, kq) holds features 16t+col of row 16 s + 4 w + kq; the dot
+// product is reduced over the 16 lanes that share a row.
+template <int T>
+__global__ __launch_bounds__(256) void residual_leaf_kernel(
+    const double* __restrict__ X, const double* __restrict__ y, const double* __restrict__ w,
+    Basis B, double floor, const LeafChunk* __restrict__ chunks, const double* __restrict__ beta,
+    const int* __restrict__ status, double* __restrict__ rpart) {
+  const int c = blockIdx.x;
+  const int leaf = chunks[c].leaf;
+  if (status[leaf] != 0) return;
+  extern __shared__ double smem[];
+  __shared__ double rred[4][16 * T];
+  const int d = B.d, p = B.p, rb = stage_rows_per_block(d), bsz = stage_buffer_doubles(d);
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int col = lane & 15, kq = lane >> 4;
+  const int64_t begin = chunks[c].begin, end = chunks[c].end;
+
+  Factor fa[T], fb[T];
+  double bt[T], rv[T];
+#pragma unroll
+  for (int t = 0; t < T; ++t) {
+    const int f = 16 * t + col;
+    column_factors(B, rb, f, fa[t], fb[t]);
+    bt[t] = f < p ? beta[(size_t)leaf * p + f] : 0.0;
+    rv[t] = 0.0;
+  }
+  stage_init(smem, d);
+  int unused = 0;
+  int64_t r0 = begin;
+  int nrows = (int)(end - r0 < rb ? end - r0 : rb);
+  if (nrows > 0) stage_rows_async(smem, X, y, w, d, rb, r0, nrows);
+  for (int blk = 0; r0 < end; ++blk) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    const int64_t n0 = r0 + rb;
+    const int nn = n0 < end ? (int)(end - n0 < rb ? end - n0 : rb) : 0;
+    if (nn > 0) stage_rows_async(smem + ((blk + 1) & 1) * bsz, X, y, w, d, rb, n0, nn);
+    const double* buf = smem + (blk & 1) * bsz;
+    const int steps = (nrows + 15) >> 4;
+    for (int s = 0; s < steps; ++s) {
+      const int r = 16 * s + 4 * wave + kq;
+      const double wt = staged_weight(buf, d, rb, r, nrows, floor, unused);
+      double phi[T];
+      double dot = 0.0;
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        phi[t] = staged_feature(buf, fa[t], fb[t], r);
+        dot += phi[t] * bt[t];
+      }
+      dot += __shfl_xor(dot, 1, kWave);
+      dot += __shfl_xor(dot, 2, kWave);
+      dot += __shfl_xor(dot, 4, kWave);
+      dot += __shfl_xor(dot, 8, kWave);
+      const double e = wt * (buf[rb * d + r] - dot);
+#pragma unroll
+      for (int t = 0; t < T; ++t) rv[t] += e * phi[t];
+    }
+    r0 = n0;
+    nrows = nn;
+  }
+#pragma unroll
+  for (int t = 0; t < T; ++t) {
+    rv[t] += __shfl_xor(rv[t], 16, kWave);
+    rv[t] += __shfl_xor(rv[t], 32, kWave);
+    if (kq == 0) rred[wave][16 * t + col] = rv[t];
+  }
+  __syncthreads();
+  if (tid < 16 * T)
+    rpart[(size_t)c * 16 * T + tid] = ((rred[0][tid] + rred[1][tid]) + rred[2][tid]) + rred[3][tid];
+}
+
+// One refinement step per leaf with the stored factor; flags leaves that
+// need the exact FP64 path.
+__global__ __launch_bounds__(128) void correct_rows_kernel(
+    const SystemDesc* __restrict__ systems, int p, int rstride, const double* __restrict__ rpart,
+    const double* __restrict__ factor, const double* __restrict__ lambda,
+    double* __restrict__ beta, const int* __restrict__ status, int* __restrict__ need,
+    double tau) {
+  extern __shared__ double M[];
+  __shared__ double bc[2];
+  __shared__ double red[2];
+  const SystemDesc S = systems[blockIdx.x];
+  const int leaf = S.out, i = threadIdx.x, nt = blockDim.x;
+  if (status[leaf] != 0) {
+    if (i == 0) need[leaf] = 1;
+    return;
+  }
+  const int nel = p * (p + 1) / 2;
+  const double* L = factor + (size_t)leaf * nel;
+  for (int e = i; e < nel; e += nt) M[e] = L[e];
+  double r = 0.0, b0 = 0.0;
+  if (i < p) {
+    for (int ch = S.chunk_begin; ch < S.chunk_end; ++ch) r += rpart[(size_t)ch * rstride + i];
+    b0 = beta[(size_t)leaf * p + i];
+    r -= lambda[leaf] * b0;
+  }
+  __syncthreads();
+  const double delta = rows_solve(M, bc, p, i, r);
+  const double b1 = b0 + delta;
+  const double nd = block_sum(i < p ? delta * delta : 0.0, red);
+  const double nb = block_sum(i < p ? b1 * b1 : 0.0, red);
+  if (i < p) beta[(size_t)leaf * p + i] = b1;
+  if (i == 0) need[leaf] = !(nd <= tau * tau * nb);
+}
+
+// ---------------------------------------------------------------------------
 // Feature materialization and input checks.
 
 __global__ __launch_bounds__(256) void features_kernel(const double* __restrict__ X, int64_t rows,
@@ -408,6 +1000,12 @@ struct art_ridge_plan {
   std::vector<SystemDesc> h_sys;
   SystemDesc* h_sys_pinned = nullptr;
   hipEvent_t sys_ready = nullptr;
+  // mixed-precision buffers (allocated on first use)
+  double* d_factor = nullptr;
+  double* d_rpart = nullptr;
+  double* d_tpart = nullptr;
+  int* d_need = nullptr;
+  double refine_tol = ART_MIXED_REFINE_TOL;
   // optional per-fit kernel timing: 3 events per slot (before Gram, after
   // Gram, after solve), recorded on the fit's stream
   int timing_slots = 0;
@@ -477,6 +1075,10 @@ static void plan_free(art_ridge_plan* P) {
   if (P->h_sys_pinned) (void)hipHostFree(P->h_sys_pinned);
   if (P->sys_ready) (void)hipEventDestroy(P->sys_ready);
   for (hipEvent_t e : P->tev) (void)hipEventDestroy(e);
+  (void)hipFree(P->d_factor);
+  (void)hipFree(P->d_rpart);
+  (void)hipFree(P->d_tpart);
+  (void)hipFree(P->d_need);
   delete P;
 }
 
@@ -519,12 +1121,12 @@ int art_leaf_plan_create(const int64_t* offsets, int K, int d, int degree, int i
     }
     sys[j].chunk_begin = (int)chunks.size();
     for (int64_t r = b; r < e; r += chunk_rows)
-      chunks.push_back(LeafChunk{r, r + chunk_rows < e ? r + chunk_rows : e});
+      chunks.push_back(LeafChunk{r, r + chunk_rows < e ? r + chunk_rows : e, j, 0});
     sys[j].chunk_end = (int)chunks.size();
     sys[j].out = j;
     sys[j].side = 0;
   }
-  if (chunks.empty()) chunks.push_back(LeafChunk{0, 0});
+  if (chunks.empty()) chunks.push_back(LeafChunk{0, 0, 0, 0});
   P->nchunks = (int)chunks.size();
   P->nsys = K;
   int rc;
@@ -591,10 +1193,17 @@ int art_plan_info(const art_ridge_plan* P, int* p, int* nchunks, int* tiles) {
 }
 
 static int launch_solve(const art_ridge_plan* P, int nsys, int nchunks_stride, double ridge,
-                        double* beta, int* status, double* lambda, hipStream_t stream) {
+                        double* beta, int* status, double* lambda, hipStream_t stream,
+                        const int* mask = nullptr, double* factor = nullptr,
+                        const double* tpart = nullptr) {
   const int P1 = P->p + 1;
   const size_t bytes = (size_t)P1 * (P1 + 1) / 2 * sizeof(double);
-  if (bytes <= ART_SOLVE_LDS_BYTES) {
+  if (P->p <= kRowsMaxP) {
+    const size_t lds = (size_t)(P->p + 1) * (P->p + 2) / 2 * sizeof(double);
+    solve_rows_kernel<<<dim3(nsys), dim3(64 * ((P->p + 63) / 64)), lds, stream>>>(
+        P->d_sys, nchunks_stride, P->T, P->npairs, P->p, ridge, P->d_partials, P->d_flags, tpart,
+        mask, factor, beta, status, lambda);
+  } else if (bytes <= ART_SOLVE_LDS_BYTES) {
     solve_f64_kernel<true><<<dim3(nsys), dim3(64), bytes, stream>>>(
         P->d_sys, nchunks_stride, P->T, P->npairs, P->p, ridge, P->d_partials, P->d_flags,
         nullptr, beta, status, lambda);
@@ -607,6 +1216,79 @@ static int launch_solve(const art_ridge_plan* P, int nsys, int nchunks_stride, d
   return ART_OK;
 }
 
+
+// Launch helpers dispatching on the compile-time tile count / padded width.
+static void launch_gram_leaf_f64(const art_ridge_plan* P, const double* X, const double* y,
+                                 const double* w, double floor, const int* mask,
+                                 hipStream_t stream) {
+  const size_t smem = leaf_stage_smem_bytes(P->basis.d);
+  const dim3 grid(P->nchunks), block(256);
+  switch (P->T) {
+#define ART_CASE(TT)                                                                         \
+  case TT:                                                                                   \
+    gram_leaf_f64_kernel<TT><<<grid, block, smem, stream>>>(X, y, w, P->basis, floor,        \
+                                                            P->d_chunks, mask,               \
+                                                            P->d_partials, P->d_flags);      \
+    break;
+    ART_CASE(1) ART_CASE(2) ART_CASE(3) ART_CASE(4) ART_CASE(5) ART_CASE(6)
+#undef ART_CASE
+  }
+}
+
+static int leaf_fit_mixed(art_ridge_plan* P, const double* X, const double* y, const double* w,
+                          double ridge, double floor, double* beta, int* status,
+                          double* lambda, hipStream_t stream) {
+  if (P->p > 64 || P->T > 5 || P->basis.d > kMaxStagedD)
+    return fail(ART_ERR_ARG, "art_leaf_ridge_fit: mixed precision needs p <= 64 and d <= 24");
+  const int p = P->p, T = P->T;
+  if (P->d_factor == nullptr) {
+    ART_HIP(hipMalloc(&P->d_factor, (size_t)P->K * (p * (p + 1) / 2) * sizeof(double)));
+    ART_HIP(hipMalloc(&P->d_rpart, (size_t)P->nchunks * 16 * T * sizeof(double)));
+    ART_HIP(hipMalloc(&P->d_tpart, (size_t)P->nchunks * sizeof(double)));
+    ART_HIP(hipMalloc(&P->d_need, (size_t)P->K * sizeof(int)));
+  }
+  const size_t smem = leaf_stage_smem_bytes(P->basis.d);
+  const dim3 cgrid(P->nchunks), sgrid(P->nsys), block(256);
+  timing_mark(P, 0, stream);
+  switch (T) {
+#define ART_CASE(TT)                                                                           \
+  case TT:                                                                                     \
+    gram_leaf_mixed_kernel<TT><<<cgrid, block, smem, stream>>>(X, y, w, P->basis, floor,       \
+                                                               P->d_chunks, P->d_partials,     \
+                                                               P->d_tpart, P->d_flags);        \
+    break;
+    ART_CASE(1) ART_CASE(2) ART_CASE(3) ART_CASE(4) ART_CASE(5)
+#undef ART_CASE
+  }
+  ART_HIP(hipGetLastError());
+  timing_mark(P, 1, stream);
+  int rc = launch_solve(P, P->nsys, P->nchunks, ridge, beta, status, lambda, stream, nullptr,
+                        P->d_factor, P->d_tpart);
+  if (rc != ART_OK) return rc;
+  switch (T) {
+#define ART_CASE(TT)                                                                           \
+  case TT:                                                                                     \
+    residual_leaf_kernel<TT><<<cgrid, block, smem, stream>>>(X, y, w, P->basis, floor,         \
+                                                             P->d_chunks, beta, status,        \
+                                                             P->d_rpart);                      \
+    break;
+    ART_CASE(1) ART_CASE(2) ART_CASE(3) ART_CASE(4) ART_CASE(5)
+#undef ART_CASE
+  }
+  ART_HIP(hipGetLastError());
+  correct_rows_kernel<<<sgrid, dim3(64 * ((p + 63) / 64)), (size_t)p * (p + 1) / 2 * sizeof(double),
+                        stream>>>(P->d_sys, p, 16 * T, P->d_rpart, P->d_factor, lambda, beta,
+                                  status, P->d_need, P->refine_tol);
+  ART_HIP(hipGetLastError());
+  // Exact FP64 path for the flagged leaves only.
+  launch_gram_leaf_f64(P, X, y, w, floor, P->d_need, stream);
+  ART_HIP(hipGetLastError());
+  rc = launch_solve(P, P->nsys, P->nchunks, ridge, beta, status, lambda, stream, P->d_need,
+                    nullptr);
+  timing_mark(P, 2, stream);
+  return rc;
+}
+
 int art_leaf_ridge_fit(art_ridge_plan* P, const double* X, const double* y, const double* w,
                        double ridge, double weight_floor, int precision, double* beta,
                        int* status, double* lambda, hipStream_t stream) {
@@ -615,14 +1297,36 @@ int art_leaf_ridge_fit(art_ridge_plan* P, const double* X, const double* y, cons
     return fail(ART_ERR_ARG, "art_leaf_ridge_fit: null pointer");
   if (!(ridge >= 0.0) || std::isinf(ridge) || !(weight_floor >= 0.0) || std::isinf(weight_floor))
     return fail(ART_ERR_ARG, "art_leaf_ridge_fit: ridge and weight_floor must be finite >= 0");
+  if (precision == ART_PRECISION_MIXED)
+    return leaf_fit_mixed(P, X, y, w, ridge, weight_floor, beta, status, lambda, stream);
   if (precision != ART_PRECISION_FP64)
     return fail(ART_ERR_ARG, "art_leaf_ridge_fit: unsupported precision");
-  PolySource src{X, y, w, P->basis, weight_floor};
-  TableChunks map{P->d_chunks};
-  const int groups = (P->npairs + kGramMaxQ - 1) / kGramMaxQ;
   timing_mark(P, 0, stream);
-  gram_f64_kernel<<<dim3(P->nchunks, groups, 1), dim3(256), 0, stream>>>(
-      src, map, P->nchunks, P->T, P->npairs, P->d_partials, P->d_flags);
+  if (P->T <= 6 && P->basis.d <= kMaxStagedD) {
+    const size_t smem = leaf_stage_smem_bytes(P->basis.d);
+    const dim3 grid(P->nchunks), block(256);
+    switch (P->T) {
+#define ART_LEAF_CASE(TT)                                                                    \
+  case TT:                                                                                   \
+    gram_leaf_f64_kernel<TT><<<grid, block, smem, stream>>>(X, y, w, P->basis, weight_floor, \
+                                                            P->d_chunks, nullptr,            \
+                                                            P->d_partials, P->d_flags);      \
+    break;
+      ART_LEAF_CASE(1)
+      ART_LEAF_CASE(2)
+      ART_LEAF_CASE(3)
+      ART_LEAF_CASE(4)
+      ART_LEAF_CASE(5)
+      ART_LEAF_CASE(6)
+#undef ART_LEAF_CASE
+    }
+  } else {
+    PolySource src{X, y, w, P->basis, weight_floor};
+    TableChunks map{P->d_chunks};
+    const int groups = (P->npairs + kGramMaxQ - 1) / kGramMaxQ;
+    gram_f64_kernel<<<dim3(P->nchunks, groups, 1), dim3(256), 0, stream>>>(
+        src, map, P->nchunks, P->T, P->npairs, P->d_partials, P->d_flags);
+  }
   ART_HIP(hipGetLastError());
   timing_mark(P, 1, stream);
   const int rc = launch_solve(P, P->nsys, P->nchunks, ridge, beta, status, lambda, stream);
@@ -668,6 +1372,23 @@ int art_gated_ridge_fit(art_ridge_plan* P, const double* phi, const double* y, c
   const int rc = launch_solve(P, 2 * count, nchunks, ridge, beta, status, lambda, stream);
   timing_mark(P, 2, stream);
   return rc;
+}
+
+int art_leaf_fallback_count(art_ridge_plan* P, int* count) {
+  if (!P || P->kind != 0 || !count) return fail(ART_ERR_ARG, "art_leaf_fallback_count: bad arguments");
+  *count = 0;
+  if (P->d_need == nullptr) return ART_OK;
+  std::vector<int> need(P->K);
+  ART_HIP(hipMemcpy(need.data(), P->d_need, P->K * sizeof(int), hipMemcpyDeviceToHost));
+  for (int v : need) *count += v != 0;
+  return ART_OK;
+}
+
+int art_leaf_set_refine_tol(art_ridge_plan* P, double tol) {
+  if (!P || P->kind != 0 || !(tol > 0.0) || std::isinf(tol))
+    return fail(ART_ERR_ARG, "art_leaf_set_refine_tol: tol must be finite and > 0");
+  P->refine_tol = tol;
+  return ART_OK;
 }
 
 int art_plan_timing_enable(art_ridge_plan* P, int slots) {

@@ -34,7 +34,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", default="fp64", choices=("fp64", "mixed"))
+    ap.add_argument("--precision", default="mixed", choices=("fp64", "mixed"))
     ap.add_argument("--leaves", type=int, default=4096)
     ap.add_argument("--chunk-rows", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -69,6 +69,39 @@ def cpu_baseline(wl, max_seconds=30.0):
     }, sample, coef, status, stride
 
 
+def timed_fits(batch, lib, X, y, w, ridge, precision, steps, warmup):
+    """Warm up, then time `steps` fits with per-step events on the fit stream and
+    the library's own events around its first kernel and the rest."""
+    import ctypes
+
+    import torch
+
+    for _ in range(warmup):
+        batch.fit(X, y, w, ridge, precision=precision)
+    torch.cuda.synchronize()
+    lib.art_plan_timing_enable(batch.plan, steps)
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    torch.cuda.synchronize()
+    ev[0].record(stream)
+    for i in range(steps):
+        batch.fit(X, y, w, ridge, precision=precision)
+        ev[i + 1].record(stream)
+    torch.cuda.synchronize()
+    step_ms = np.array([ev[i].elapsed_time(ev[i + 1]) for i in range(steps)])
+    total_ms = ev[0].elapsed_time(ev[-1])
+    first = (ctypes.c_float * steps)()
+    rest = (ctypes.c_float * steps)()
+    n = lib.art_plan_timing_read(batch.plan, first, rest, steps)
+    lib.art_plan_timing_enable(batch.plan, 0)
+    return {"ms_per_step": total_ms / steps, "step_ms": step_ms,
+            "first_kernel_ms": float(np.mean(first[:n])), "rest_ms": float(np.mean(rest[:n]))}
+
+
+def rel_err(a, b):
+    return np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-300)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -90,43 +123,43 @@ def main():
     batch = LeafRidgeBatch(wl.offsets, 8, degree=2, chunk_rows=args.chunk_rows)
     p, N, K = batch.p, wl.n_rows, wl.n_leaves
     lib = batch.lib
-    print(f"[bench] leaves={K} rows={N} p={p} chunks={batch.n_chunks} "
-          f"precision={args.precision}", file=sys.stderr, flush=True)
+    print(f"[bench] leaves={K} rows={N} p={p} chunks={batch.n_chunks}", file=sys.stderr, flush=True)
 
-    for _ in range(args.warmup):
-        batch.fit(X, y, w, wl.ridge, precision=args.precision)
-    torch.cuda.synchronize()
+    # FP64 plain form: the accuracy reference and the comparison point.
+    plain = timed_fits(batch, lib, X, y, w, wl.ridge, "fp64", args.steps, args.warmup)
+    ref = batch.coefficients.cpu().numpy()
+    ref_status = batch.status.cpu().numpy()
+    print(f"[bench] fp64 plain {plain['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+    run = plain
+    if args.precision == "mixed":
+        run = timed_fits(batch, lib, X, y, w, wl.ridge, "mixed", args.steps, args.warmup)
+        print(f"[bench] mixed {run['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+    coef = batch.coefficients.cpu().numpy()
+    st = batch.status.cpu().numpy()
+    ok = (st == 0) & (ref_status == 0)
+    err = rel_err(coef[ok], ref[ok])
+    fallback = batch.fallback_count() if args.precision == "mixed" else 0
 
-    # Timed region: K steps, per-step events for latency, library events
-    # around the Gram and solve kernels on the same stream.
-    import ctypes
-    lib.art_plan_timing_enable(batch.plan, args.steps)
-    stream = torch.cuda.current_stream()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev[0].record(stream)
-    for i in range(args.steps):
-        batch.fit(X, y, w, wl.ridge, precision=args.precision)
-        ev[i + 1].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    step_ms = np.array([ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps)])
-    total_ms = ev[0].elapsed_time(ev[-1])
-    gram = (ctypes.c_float * args.steps)()
-    solve = (ctypes.c_float * args.steps)()
-    n = lib.art_plan_timing_read(batch.plan, gram, solve, args.steps)
-    gram_ms = float(np.mean(gram[:n]))
-    solve_ms = float(np.mean(solve[:n]))
-    lib.art_plan_timing_enable(batch.plan, 0)
-
-    coef, status, _ = batch.coefficients, batch.status, batch.ridge_effective
-    st = status.cpu().numpy()
-    ms_per_step = total_ms / args.steps
-    # Algorithmic flops of the Gram kernel per launch: G (2 p^2 per row) + h (2 p per row).
-    flops = 2.0 * p * (p + 1) * N
-    achieved = flops / (gram_ms * 1e-3) / 1e12
-    peak = PEAK_TFLOPS[args.precision]
+    ms_per_step = run["ms_per_step"]
+    kernel_ms = run["first_kernel_ms"]
+    if args.precision == "mixed":
+        # Dominant kernel: the split-bf16 Gram pass, bound by HBM.  Algorithmic
+        # bytes: X (8 doubles), y and w per row, read once.
+        traffic_bytes = N * (8 + 2) * 8
+        achieved = traffic_bytes / (kernel_ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                    "kernel": "gram_leaf_mixed_kernel<3>", "algorithmic_bytes_per_launch": traffic_bytes,
+                    "kernel_ms": kernel_ms}
+    else:
+        # FP64 Gram on MFMA: symmetric entries p(p+1)/2 plus p right-hand-side
+        # entries, 2 flops each per row.
+        flops = (p * (p + 1) + 2.0 * p) * N
+        achieved = flops / (kernel_ms * 1e-3) / 1e12
+        roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS["fp64"],
+                    "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["fp64"], "traffic": None,
+                    "kernel": "gram_leaf_f64_kernel<3>", "algorithmic_flops_per_launch": flops,
+                    "kernel_ms": kernel_ms}
     result = {
         "metric": METRIC,
         "value": K / (ms_per_step * 1e-3),
@@ -138,7 +171,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "bf16x3/f32+f64",
+        "dtype": "f64" if args.precision == "fp64" else "bf16x3-mfma/f32-acc/f64-refine",
         "data": "synthetic (seeded, generated on host, resident in HBM before timing)",
         "config": {
             "workload": ("batched weighted ridge: 4096 leaves, d=8, quadratic p=45, "
@@ -146,29 +179,29 @@ def main():
             "leaves": K, "rows": N, "p": p, "precision": args.precision,
             "chunk_rows": args.chunk_rows or 2048, "chunks": batch.n_chunks,
         },
-        "latency_ms": {"p50": float(np.percentile(step_ms, 50)),
-                       "p99": float(np.percentile(step_ms, 99)),
+        "latency_ms": {"p50": float(np.percentile(run["step_ms"], 50)),
+                       "p99": float(np.percentile(run["step_ms"], 99)),
                        "unit": "one 4096-leaf batch fit"},
-        "kernels_ms": {"gram": gram_ms, "solve": solve_ms},
+        "rel_err_vs_fp64_plain": {"max": float(err.max()) if err.size else 0.0,
+                                  "p99": float(np.percentile(err, 99)) if err.size else 0.0},
+        "fp64_plain": {"ms_per_step": plain["ms_per_step"],
+                       "leaf_fits_per_s": K / (plain["ms_per_step"] * 1e-3),
+                       "gram_ms": plain["first_kernel_ms"], "solve_ms": plain["rest_ms"]},
+        "speedup_vs_fp64_plain": plain["ms_per_step"] / ms_per_step,
+        "fallback_leaves": fallback,
+        "kernels_ms": {"first": kernel_ms, "rest": run["rest_ms"]},
         "failed_leaves": int((st != 0).sum()),
-        "wall_s": wall,
-        "roofline": {
-            "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-            "frac": achieved / peak, "traffic": None,
-            "kernel": "gram_f64_kernel", "algorithmic_flops_per_launch": flops,
-            "hbm_gbs_min_bytes": N * 80 / (gram_ms * 1e-3) / 1e9,
-        },
+        "roofline": roofline,
     }
     if not args.no_cpu:
         sys.path.insert(0, str(ROOT))
-        cpu, sample, ref, ref_status, stride = cpu_baseline(wl)
+        cpu, sample, cref, cref_status, stride = cpu_baseline(wl)
         result["cpu_baseline"] = cpu
-        got = coef.cpu().numpy()[::stride][: sample.n_leaves]
-        ok = (ref_status == 0) & (st[::stride][: sample.n_leaves] == 0)
-        err = (np.linalg.norm(got[ok] - ref[ok], axis=1)
-               / np.maximum(np.linalg.norm(ref[ok], axis=1), 1e-300))
-        result["rel_err_vs_cpu_fp64"] = {"max": float(err.max()),
-                                         "p99": float(np.percentile(err, 99))}
+        got = coef[::stride][: sample.n_leaves]
+        both = (cref_status == 0) & (st[::stride][: sample.n_leaves] == 0)
+        cerr = rel_err(got[both], cref[both])
+        result["rel_err_vs_cpu_fp64"] = {"max": float(cerr.max()),
+                                         "p99": float(np.percentile(cerr, 99))}
     batch.close()
     print(json.dumps(result))
 

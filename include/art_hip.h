@@ -31,6 +31,7 @@ extern "C" {
 #define ART_PRECISION_MIXED 1 /* low-precision MFMA Gram + FP64 refinement */
 
 #define ART_DEFAULT_CHUNK_ROWS 2048
+#define ART_MIXED_REFINE_TOL 1e-4 /* accept |delta| <= tol*|beta| after one refinement */
 #define ART_SOLVE_LDS_BYTES 65536
 
 /* Opaque resident workspace: chunk tables and partial-Gram buffers. */
@@ -76,6 +77,16 @@ int art_leaf_plan_create(const int64_t* offsets, int K, int d, int degree, int i
 int art_leaf_ridge_fit(art_ridge_plan* plan, const double* X, const double* y, const double* w,
                        double ridge, double weight_floor, int precision, double* beta,
                        int* status, double* lambda, hipStream_t stream);
+
+/* ART_PRECISION_MIXED (p <= 64): split-bf16 MFMA Gram with exact FP64
+ * diagonal/right-hand side, one FP64 data-space refinement pass, and the exact
+ * FP64 path for every leaf whose refinement correction exceeds the tolerance
+ * (or whose mixed factorization fails).  Number of leaves that took the FP64
+ * path in the last mixed fit (synchronizes the device): */
+int art_leaf_fallback_count(art_ridge_plan* plan, int* count);
+
+/* Acceptance tolerance of the mixed refinement (default ART_MIXED_REFINE_TOL). */
+int art_leaf_set_refine_tol(art_ridge_plan* plan, double tol);
 
 /* Equal-size gated nodes (the BatchedCholeskyWorkspace contract): capacity K
  * nodes of n rows and p features.  Allocates device buffers (synchronous).

@@ -319,3 +319,64 @@ def test_leaf_workload_subset_matches_oracle(torch_gpu):
         # Both sides are FP64; they differ only by rounding amplified by the
         # leaf condition number (up to ~1e7 here).
         assert _rel_err(coef.cpu().numpy(), ref).max() < 1e-7
+
+
+# --------------------------------------------------------------------------- mixed precision
+
+@pytest.mark.parametrize("n_leaves,seed", [(256, 2), (64, 5)])
+def test_mixed_leaf_fits_within_1e6_of_fp64(torch_gpu, n_leaves, seed):
+    """Mixed precision (split-bf16 Gram + FP64 refinement + FP64 fallback) stays
+    within 1e-6 relative coefficient error of the FP64 path on every leaf."""
+    from art import LeafRidgeBatch
+    from art.synthetic import leaf_ridge_workload
+
+    torch = torch_gpu
+    wl = leaf_ridge_workload(n_leaves=n_leaves, seed=seed)
+    X, y, w = dev(torch, wl.X), dev(torch, wl.y), dev(torch, wl.w)
+    with LeafRidgeBatch(wl.offsets, 8) as batch:
+        c64, s64, l64 = (t.clone() for t in batch.fit(X, y, w, wl.ridge, precision="fp64"))
+        cm, sm, lm = batch.fit(X, y, w, wl.ridge, precision="mixed")
+        np.testing.assert_array_equal(sm.cpu().numpy(), s64.cpu().numpy())
+        np.testing.assert_allclose(lm.cpu().numpy(), l64.cpu().numpy(), rtol=1e-12)
+        err = _rel_err(cm.cpu().numpy(), c64.cpu().numpy())
+        assert err.max() <= 1e-6, err.max()
+        assert 0 <= batch.fallback_count() < n_leaves
+    ref, st_ref, _ = R.leaf_fit(wl.X, wl.y, wl.w, wl.offsets, wl.ridge)
+    np.testing.assert_array_equal(sm.cpu().numpy(), st_ref)
+    assert _rel_err(cm.cpu().numpy(), ref).max() <= 1e-6
+
+
+def test_mixed_edge_leaves_match_fp64(torch_gpu):
+    """Empty, one-row, flagged and tiny-weight leaves take the exact path."""
+    from art import LeafRidgeBatch
+
+    torch, rng = torch_gpu, np.random.default_rng(17)
+    sizes = np.array([0, 1, 50, 3000, 64, 700])
+    offsets = np.concatenate([[0], np.cumsum(sizes)])
+    N = offsets[-1]
+    X = rng.uniform(-1, 1, size=(N, 2))
+    y = np.cos(2 * X[:, 0]) * X[:, 1]
+    w = rng.uniform(size=N)
+    w[offsets[2] + 3] = np.nan            # leaf 2 flagged (-1)
+    w[offsets[4]:offsets[5]] = 1e-13      # leaf 4: every weight below the floor
+    Xd, yd, wd = dev(torch, X), dev(torch, y), dev(torch, w)
+    with LeafRidgeBatch(offsets, 2) as batch:
+        c64, s64, _ = (t.clone() for t in batch.fit(Xd, yd, wd, 1e-6, precision="fp64"))
+        cm, sm, _ = batch.fit(Xd, yd, wd, 1e-6, precision="mixed")
+        np.testing.assert_array_equal(sm.cpu().numpy(), s64.cpu().numpy())
+        ok = s64.cpu().numpy() == 0
+        assert not ok[0] and not ok[2]
+        assert _rel_err(cm.cpu().numpy()[ok], c64.cpu().numpy()[ok]).max() <= 1e-6
+        assert np.isnan(cm.cpu().numpy()[~ok]).all()
+
+
+def test_mixed_rejects_wide_bases(torch_gpu):
+    from art import LeafRidgeBatch
+
+    torch = torch_gpu
+    offsets = np.array([0, 200])
+    X = np.random.default_rng(3).uniform(-1, 1, size=(200, 12))
+    with LeafRidgeBatch(offsets, 12) as batch:  # p = 91 > 64
+        with pytest.raises(ValueError, match="mixed"):
+            batch.fit(dev(torch, X), dev(torch, X[:, 0]), dev(torch, np.ones(200)), 1e-6,
+                      precision="mixed")
