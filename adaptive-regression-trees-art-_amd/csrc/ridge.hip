@@ -224,7 +224,9 @@ constexpr int kMaxStagedD = 24;
 
 typedef __attribute__((address_space(3))) void* lds_void_ptr;
 
-__host__ __device__ inline int stage_rows_per_block(int d) { return d <= 10 ? 256 : 128; }
+__host__ __device__ inline int stage_rows_per_block(int d) {
+  return d <= 10 ? 256 : (d <= 16 ? 128 : 64);
+}
 __host__ __device__ inline int stage_buffer_doubles(int d) {
   return stage_rows_per_block(d) * (d + 2) + 2;
 }
@@ -232,6 +234,17 @@ __host__ __device__ inline int stage_buffer_doubles(int d) {
 size_t leaf_stage_smem_bytes(int d) {
   const size_t n = 2 * (size_t)stage_buffer_doubles(d);
   return (n > 1024 ? n : 1024) * sizeof(double);
+}
+
+// The mixed Gram kernel stages rows through registers (one row per thread)
+// into an FP32 table [x_0..x_{d-1}, 1, y, 0] (stride d + 3 floats) plus FP32
+// weights; it has no FP64 LDS buffers.
+constexpr int kMixedRows = 256;
+constexpr int kMixedMaxD = 16;
+__host__ __device__ inline int rows32_stride(int d) { return d + 3; }
+size_t mixed_stage_smem_bytes(int d) {
+  const size_t f32 = (size_t)kMixedRows * (rows32_stride(d) + 1) * sizeof(float);
+  return f32 > 1024 * sizeof(double) ? f32 : 1024 * sizeof(double);
 }
 
 __device__ inline void copy_dwords_async(double* dst, const void* src, int n, int wave, int lane) {
@@ -697,27 +710,52 @@ __global__ __launch_bounds__(128) void solve_rows_kernel(
 //    failed) is flagged for the exact FP64 path, which then runs only on the
 //    flagged leaves' chunks.
 
-__device__ inline void split_bf16(float v, __bf16& hi, __bf16& lo) {
-  hi = (__bf16)v;
-  lo = (__bf16)(v - (float)hi);
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// Round two floats to bf16 (RNE) with one v_cvt_pk_bf16_f32; the asm barrier
+// keeps the compiler from fusing an earlier double->float conversion into a
+// double->bf16 one (which it emulates with a round-to-odd sequence).
+__device__ inline unsigned pack_bf16(float a, float b) {
+  asm volatile("" : "+v"(a), "+v"(b));
+  const f32x2 v = {a, b};
+  return __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2));
 }
 
-// Split-bf16 Gram.  Wave w takes the 32-row steps s = w, w+4, ...; lane
-// (col, kq) holds rows 32 s + 8 kq + e (e < 8) of features 16 t + col, the
-// operand layout of v_mfma_f32_16x16x32_bf16.  Each step's FP32 tile is added
-// into FP64 accumulators.  The trace of the Gram (which fixes lambda) is
-// accumulated exactly in FP64, one thread per row, from
-// |phi|^2 = bias + s1 [+ (s1^2 + s2) / 2 for the quadratic basis],
-// s1 = sum x_i^2, s2 = sum x_i^4; it goes to tpart[chunk].
-template <int T>
-__global__ __launch_bounds__(256) void gram_leaf_mixed_kernel(
+// Split a pair of floats into packed bf16 hi and lo words: v = hi + lo + O(2^-18 v).
+__device__ inline void split_pair(float a, float b, unsigned& hi, unsigned& lo) {
+  hi = pack_bf16(a, b);
+  const float ha = __uint_as_float(hi << 16), hb = __uint_as_float(hi & 0xffff0000u);
+  lo = pack_bf16(a - ha, b - hb);
+}
+
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+// Split-bf16 Gram.  One thread per row loads a block of kMixedRows rows
+// straight into registers (the next block's loads are issued before the
+// current block's MFMA phase, so HBM latency hides behind it), accumulates
+// the exact FP64 trace of the Gram (which fixes lambda) from
+// |phi|^2 = bias + s1 [+ (s1^2 + s2)/2 for the quadratic basis],
+// s1 = sum x_i^2, s2 = sum x_i^4, checks the weight, and writes the row to
+// an FP32 table [x, 1, y, 0] with its floored FP32 weight.  Wave w then takes
+// the 32-row steps s = w, w+4, ...: lane (col, kq) holds rows 32 s + 4 e + kq
+// (e < 8; the four row groups of one LDS read are adjacent rows) of features
+// 16 t + col as a = w phi and b = phi, each split v = hi + lo in bf16, in the
+// operand layout of v_mfma_f32_16x16x32_bf16.  A tile pair accumulates
+// hi*lo + lo*hi + hi*hi in FP32, added into FP64 every second step.
+template <int T, int DC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void gram_leaf_mixed_kernel(
     const double* __restrict__ X, const double* __restrict__ y, const double* __restrict__ w,
     Basis B, double floor, const LeafChunk* __restrict__ chunks, double* __restrict__ partials,
     double* __restrict__ tpart, int* __restrict__ flags) {
   constexpr int NP = T * (T + 1) / 2;
+  constexpr int MD = DC > 0 ? DC : kMixedMaxD;
   extern __shared__ double smem[];
   __shared__ double tred[4];
-  const int d = B.d, rb = stage_rows_per_block(d), bsz = stage_buffer_doubles(d);
+  const int d = DC > 0 ? DC : B.d;
+  const int S32 = rows32_stride(d);
+  float* rows32 = reinterpret_cast<float*>(smem);
+  float* w32 = rows32 + kMixedRows * S32;
   const int c = blockIdx.x;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int col = lane & 15, kq = lane >> 4;
@@ -725,77 +763,146 @@ __global__ __launch_bounds__(256) void gram_leaf_mixed_kernel(
   const double bias = B.bias ? 1.0 : 0.0;
   const double quad = B.degree == 2 ? 0.5 : 0.0;
 
-  Factor fa[T], fb[T];
+  int fa[T], fb[T];
 #pragma unroll
-  for (int t = 0; t < T; ++t) column_factors(B, rb, 16 * t + col, fa[t], fb[t]);
+  for (int t = 0; t < T; ++t) {
+    const int f = 16 * t + col;
+    int a = -1, b2 = -1;
+    if (f < B.p) {
+      basis_factors(B, f, a, b2);
+      fa[t] = a >= 0 ? a : d;
+      fb[t] = b2 >= 0 ? b2 : d;
+    } else {
+      fa[t] = f == B.p ? d + 1 : d + 2;
+      fb[t] = d;
+    }
+    fa[t] += kq * S32;
+    fb[t] += kq * S32;
+  }
   f64x4 acc[NP];
+  f32x4 z[NP];
 #pragma unroll
-  for (int q = 0; q < NP; ++q) acc[q] = f64x4{0.0, 0.0, 0.0, 0.0};
-
-  stage_init(smem, d);
-  int bad = 0;
-  double tr = 0.0;
-  int64_t r0 = begin;
-  int nrows = (int)(end - r0 < rb ? end - r0 : rb);
-  if (nrows > 0) stage_rows_async(smem, X, y, w, d, rb, r0, nrows);
-  for (int blk = 0; r0 < end; ++blk) {
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    const int64_t n0 = r0 + rb;
-    const int nn = n0 < end ? (int)(end - n0 < rb ? end - n0 : rb) : 0;
-    if (nn > 0) stage_rows_async(smem + ((blk + 1) & 1) * bsz, X, y, w, d, rb, n0, nn);
-    const double* buf = smem + (blk & 1) * bsz;
-    // exact trace, one thread per row
-    for (int r = tid; r < nrows; r += 256) {
-      const double wt = staged_weight(buf, d, rb, r, nrows, floor, bad);
-      double s1 = 0.0, s2 = 0.0;
-      for (int i = 0; i < d; ++i) {
-        const double x2 = buf[r * d + i] * buf[r * d + i];
-        s1 += x2;
-        s2 += x2 * x2;
-      }
-      tr += wt * (bias + s1 + quad * (s1 * s1 + s2));
-    }
-    const int steps = (nrows + 31) >> 5;
-    for (int s = wave; s < steps; s += 4) {
-      bf16x8 ah[T], al[T], bh[T], bl[T];
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int r = 32 * s + 8 * kq + e;
-        int unused = 0;
-        const float wt = (float)staged_weight(buf, d, rb, r, nrows, floor, unused);
-#pragma unroll
-        for (int t = 0; t < T; ++t) {
-          const float phi = (float)staged_feature(buf, fa[t], fb[t], r);
-          __bf16 h0, l0, h1, l1;
-          split_bf16(wt * phi, h0, l0);
-          split_bf16(phi, h1, l1);
-          ah[t][e] = h0;
-          al[t][e] = l0;
-          bh[t][e] = h1;
-          bl[t][e] = l1;
-        }
-      }
-      int q = 0;
-#pragma unroll
-      for (int t1 = 0; t1 < T; ++t1)
-#pragma unroll
-        for (int t2 = t1; t2 < T; ++t2) {
-          f32x4 z = {0.f, 0.f, 0.f, 0.f};
-          z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[t1], bl[t2], z, 0, 0, 0);
-          z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[t1], bh[t2], z, 0, 0, 0);
-          z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[t1], bh[t2], z, 0, 0, 0);
-#pragma unroll
-          for (int r = 0; r < 4; ++r) acc[q][r] += (double)z[r];
-          ++q;
-        }
-    }
-    r0 = n0;
-    nrows = nn;
+  for (int q = 0; q < NP; ++q) {
+    acc[q] = f64x4{0.0, 0.0, 0.0, 0.0};
+    z[q] = f32x4{0.f, 0.f, 0.f, 0.f};
   }
 
+  // Register staging of this thread's row of the current block.
+  double xr[MD], yr = 0.0, wr = 0.0;
+  auto load_row = [&](int64_t row) {
+    if (row < end) {
+#pragma unroll
+      for (int i = 0; i < MD; ++i)
+        if (DC > 0 || i < d) xr[i] = X[row * d + i];
+      yr = y[row];
+      wr = w[row];
+    }
+  };
+  int bad = 0, pending = 0;
+  double tr = 0.0;
+  int64_t r0 = begin;
+  load_row(r0 + tid);
+  for (; r0 < end; r0 += kMixedRows) {
+    const int nrows = (int)(end - r0 < kMixedRows ? end - r0 : kMixedRows);
+    __syncthreads();  // the previous block's MFMA phase is done with the table
+    {
+      float* row = rows32 + tid * S32;
+      if (tid < nrows) {
+        const bool ok = (wr >= 0.0) & !isinf(wr);
+        bad |= !ok;
+        const double wt = ok ? fmax(wr, floor) : 0.0;
+        double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+        for (int i = 0; i < MD; ++i)
+          if (DC > 0 || i < d) {
+            const double x2 = xr[i] * xr[i];
+            s1 += x2;
+            s2 += x2 * x2;
+            row[i] = (float)xr[i];
+          }
+        tr += wt * (bias + s1 + quad * (s1 * s1 + s2));
+        row[d] = 1.0f;
+        row[d + 1] = (float)yr;
+        row[d + 2] = 0.0f;
+        w32[tid] = (float)wt;
+      } else {
+        for (int i = 0; i < S32; ++i) row[i] = 0.0f;
+        w32[tid] = 0.0f;
+      }
+    }
+    load_row(r0 + kMixedRows + tid);  // next block, in flight during the MFMA phase
+    __syncthreads();
+    const int steps = (nrows + 31) >> 5;
+    for (int s = wave; s < steps; s += 4) {
+      const float* base = rows32 + 32 * s * S32;
+      const float* wb = w32 + 32 * s + kq;
+      // Rows 8k + kq and 8k + 4 + kq share one packed word of each operand.
+      u32x4 ahw[T], alw[T], bhw[T], blw[T];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const float w0 = wb[8 * k], w1 = wb[8 * k + 4];
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+          const float p0 = base[fa[t] + 8 * k * S32] * base[fb[t] + 8 * k * S32];
+          const float p1 = base[fa[t] + (8 * k + 4) * S32] * base[fb[t] + (8 * k + 4) * S32];
+          unsigned hi, lo;
+          split_pair(w0 * p0, w1 * p1, hi, lo);
+          ahw[t][k] = hi;
+          alw[t][k] = lo;
+          split_pair(p0, p1, hi, lo);
+          bhw[t][k] = hi;
+          blw[t][k] = lo;
+        }
+      }
+      bf16x8 ah[T], al[T], bh[T], bl[T];
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        ah[t] = __builtin_bit_cast(bf16x8, ahw[t]);
+        al[t] = __builtin_bit_cast(bf16x8, alw[t]);
+        bh[t] = __builtin_bit_cast(bf16x8, bhw[t]);
+        bl[t] = __builtin_bit_cast(bf16x8, blw[t]);
+      }
+      // Three passes over the tile pairs keep NP independent accumulator
+      // chains in flight; small terms first.
+      {
+        int q = 0;
+#pragma unroll
+        for (int t1 = 0; t1 < T; ++t1)
+#pragma unroll
+          for (int t2 = t1; t2 < T; ++t2, ++q)
+            z[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[t1], bl[t2], z[q], 0, 0, 0);
+        q = 0;
+#pragma unroll
+        for (int t1 = 0; t1 < T; ++t1)
+#pragma unroll
+          for (int t2 = t1; t2 < T; ++t2, ++q)
+            z[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[t1], bh[t2], z[q], 0, 0, 0);
+        q = 0;
+#pragma unroll
+        for (int t1 = 0; t1 < T; ++t1)
+#pragma unroll
+          for (int t2 = t1; t2 < T; ++t2, ++q)
+            z[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[t1], bh[t2], z[q], 0, 0, 0);
+      }
+      if (++pending == 2) {
+#pragma unroll
+        for (int q = 0; q < NP; ++q)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            acc[q][r] += (double)z[q][r];
+            z[q][r] = 0.f;
+          }
+        pending = 0;
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NP; ++q)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[q][r] += (double)z[q][r];
+
   double* red = smem;
-  const size_t base = (size_t)c * NP;
+  const size_t pbase = (size_t)c * NP;
 #pragma unroll
   for (int q = 0; q < NP; ++q) {
     __syncthreads();
@@ -803,7 +910,7 @@ __global__ __launch_bounds__(256) void gram_leaf_mixed_kernel(
     for (int r = 0; r < 4; ++r) red[wave * 256 + (4 * kq + r) * 16 + col] = acc[q][r];
     __syncthreads();
     const double v = ((red[tid] + red[256 + tid]) + red[512 + tid]) + red[768 + tid];
-    partials[(base + q) * 256 + tid] = v;
+    partials[(pbase + q) * 256 + tid] = v;
   }
   tr = wave_sum(tr);
   if (lane == 0) tred[wave] = tr;
@@ -886,10 +993,105 @@ __global__ __launch_bounds__(256) void residual_leaf_kernel(
     rpart[(size_t)c * 16 * T + tid] = ((rred[0][tid] + rred[1][tid]) + rred[2][tid]) + rred[3][tid];
 }
 
+// Quadratic-basis residual, one row per thread.  With the input dimension D
+// known at compile time the row lives in registers, and
+//   phi . beta = b0 + sum_i x_i (b_i + sum_{j>=i} b_ij x_j),
+//   r += e [1, x_i, x_i x_j]   (e = w (y - phi . beta)),
+// cost about (D + 1)(D + 2) FP64 FMAs per row, with no cross-lane traffic
+// until the chunk ends.  Chunks here are up to kResidualChunkRows long.
+constexpr int kResidualChunkRows = 8192;
+
+// Wave reduce-scatter of N <= 64 values per lane (entries >= N count as 0):
+// afterwards lane l holds the sum over the wave of value l (63 shuffles
+// instead of N full reductions).
+template <int N>
+__device__ inline double reduce_scatter(double (&v)[N], int lane) {
+  static_assert(N <= 64, "one value per lane");
+  double u[32];
+#pragma unroll
+  for (int j = 0; j < 32; ++j) {  // h = 32 stage reads the N inputs directly
+    const double lo = j < N ? v[j] : 0.0, hi = j + 32 < N ? v[j + 32] : 0.0;
+    const bool upper = (lane & 32) != 0;
+    u[j] = (upper ? hi : lo) + __shfl_xor(upper ? lo : hi, 32, kWave);
+  }
+#pragma unroll
+  for (int h = 16; h >= 1; h >>= 1) {
+    const bool upper = (lane & h) != 0;
+#pragma unroll
+    for (int j = 0; j < h; ++j) {
+      const double send = upper ? u[j] : u[j + h];
+      const double keep = upper ? u[j + h] : u[j];
+      u[j] = keep + __shfl_xor(send, h, kWave);
+    }
+  }
+  return u[0];
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void residual_quad_kernel(
+    const double* __restrict__ X, const double* __restrict__ y, const double* __restrict__ w,
+    Basis B, double floor, const LeafChunk* __restrict__ rchunks,
+    const double* __restrict__ beta, const int* __restrict__ status,
+    double* __restrict__ rpart) {
+  constexpr int NQ = D * (D + 1) / 2;
+  constexpr int NF = 1 + D + NQ;  // [bias, linear, quadratic]
+  static_assert(NF <= 64, "reduce-scatter holds one value per lane");
+  const int c = blockIdx.x;
+  const int leaf = rchunks[c].leaf;
+  if (status[leaf] != 0) return;
+  __shared__ double bs[NF];
+  __shared__ double red[4][64];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int bias = B.bias, p = B.p;
+  if (tid < NF) {
+    const int f = tid - 1 + bias;  // coefficient index in the leaf's order
+    bs[tid] = (tid == 0 && !bias) ? 0.0 : beta[(size_t)leaf * p + f];
+  }
+  __syncthreads();
+  double acc[NF];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) acc[k] = 0.0;
+  const int64_t begin = rchunks[c].begin, end = rchunks[c].end;
+  for (int64_t row = begin + tid; row < end; row += 256) {
+    double x[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) x[i] = X[row * D + i];
+    const double wv = w[row];
+    const double wt = ((wv >= 0.0) & !isinf(wv)) ? fmax(wv, floor) : 0.0;
+    double dot = bs[0];
+    int k = 0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      double u = bs[1 + i];
+#pragma unroll
+      for (int j = i; j < D; ++j, ++k) u += bs[1 + D + k] * x[j];
+      dot += x[i] * u;
+    }
+    const double e = wt * (y[row] - dot);
+    acc[0] += e;
+    k = 0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      const double ex = e * x[i];
+      acc[1 + i] += ex;
+#pragma unroll
+      for (int j = i; j < D; ++j, ++k) acc[1 + D + k] += ex * x[j];
+    }
+  }
+  const double total = reduce_scatter<NF>(acc, lane);
+  red[wave][lane] = total;
+  __syncthreads();
+  if (tid < NF) {
+    const double v = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
+    if (tid > 0 || bias) rpart[(size_t)c * NF + tid - 1 + bias] = v;
+  }
+}
+
 // One refinement step per leaf with the stored factor; flags leaves that
 // need the exact FP64 path.
 __global__ __launch_bounds__(128) void correct_rows_kernel(
-    const SystemDesc* __restrict__ systems, int p, int rstride, const double* __restrict__ rpart,
+    const SystemDesc* __restrict__ systems, const int* __restrict__ roff, int p, int rstride,
+    const double* __restrict__ rpart,
     const double* __restrict__ factor, const double* __restrict__ lambda,
     double* __restrict__ beta, const int* __restrict__ status, int* __restrict__ need,
     double tau) {
@@ -907,7 +1109,7 @@ __global__ __launch_bounds__(128) void correct_rows_kernel(
   for (int e = i; e < nel; e += nt) M[e] = L[e];
   double r = 0.0, b0 = 0.0;
   if (i < p) {
-    for (int ch = S.chunk_begin; ch < S.chunk_end; ++ch) r += rpart[(size_t)ch * rstride + i];
+    for (int ch = roff[leaf]; ch < roff[leaf + 1]; ++ch) r += rpart[(size_t)ch * rstride + i];
     b0 = beta[(size_t)leaf * p + i];
     r -= lambda[leaf] * b0;
   }
@@ -1005,6 +1207,12 @@ struct art_ridge_plan {
   double* d_rpart = nullptr;
   double* d_tpart = nullptr;
   int* d_need = nullptr;
+  // residual chunking: Gram chunks per leaf (goff) and the coarser chunks of
+  // the quadratic residual kernel (rchunks, roff)
+  int* d_goff = nullptr;
+  int* d_roff = nullptr;
+  LeafChunk* d_rchunks = nullptr;
+  int nrchunks = 0;
   double refine_tol = ART_MIXED_REFINE_TOL;
   // optional per-fit kernel timing: 3 events per slot (before Gram, after
   // Gram, after solve), recorded on the fit's stream
@@ -1079,6 +1287,9 @@ static void plan_free(art_ridge_plan* P) {
   (void)hipFree(P->d_rpart);
   (void)hipFree(P->d_tpart);
   (void)hipFree(P->d_need);
+  (void)hipFree(P->d_goff);
+  (void)hipFree(P->d_roff);
+  (void)hipFree(P->d_rchunks);
   delete P;
 }
 
@@ -1129,6 +1340,30 @@ int art_leaf_plan_create(const int64_t* offsets, int K, int d, int degree, int i
   if (chunks.empty()) chunks.push_back(LeafChunk{0, 0, 0, 0});
   P->nchunks = (int)chunks.size();
   P->nsys = K;
+  std::vector<int> goff(K + 1), roff(K + 1);
+  std::vector<LeafChunk> rchunks;
+  for (int j = 0; j < K; ++j) {
+    goff[j] = sys[j].chunk_begin;
+    roff[j] = (int)rchunks.size();
+    for (int64_t r = offsets[j]; r < offsets[j + 1]; r += kResidualChunkRows)
+      rchunks.push_back(LeafChunk{r, r + kResidualChunkRows < offsets[j + 1] ? r + kResidualChunkRows
+                                                                          : offsets[j + 1],
+                                  j, 0});
+  }
+  goff[K] = sys[K - 1].chunk_end;
+  roff[K] = (int)rchunks.size();
+  if (rchunks.empty()) rchunks.push_back(LeafChunk{0, 0, 0, 0});
+  P->nrchunks = (int)rchunks.size();
+  if (hipMalloc(&P->d_goff, (K + 1) * sizeof(int)) != hipSuccess ||
+      hipMalloc(&P->d_roff, (K + 1) * sizeof(int)) != hipSuccess ||
+      hipMalloc(&P->d_rchunks, rchunks.size() * sizeof(LeafChunk)) != hipSuccess ||
+      hipMemcpy(P->d_goff, goff.data(), (K + 1) * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(P->d_roff, roff.data(), (K + 1) * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(P->d_rchunks, rchunks.data(), rchunks.size() * sizeof(LeafChunk),
+                hipMemcpyHostToDevice) != hipSuccess) {
+    plan_free(P);
+    return fail(ART_ERR_HIP, "art_leaf_plan_create: residual table upload failed");
+  }
   int rc;
   if (hipMalloc(&P->d_chunks, chunks.size() * sizeof(LeafChunk)) != hipSuccess ||
       hipMalloc(&P->d_sys, sys.size() * sizeof(SystemDesc)) != hipSuccess) {
@@ -1238,47 +1473,75 @@ static void launch_gram_leaf_f64(const art_ridge_plan* P, const double* X, const
 static int leaf_fit_mixed(art_ridge_plan* P, const double* X, const double* y, const double* w,
                           double ridge, double floor, double* beta, int* status,
                           double* lambda, hipStream_t stream) {
-  if (P->p > 64 || P->T > 5 || P->basis.d > kMaxStagedD)
-    return fail(ART_ERR_ARG, "art_leaf_ridge_fit: mixed precision needs p <= 64 and d <= 24");
+  if (P->p > 64 || P->T > 5 || P->basis.d > kMixedMaxD)
+    return fail(ART_ERR_ARG, "art_leaf_ridge_fit: mixed precision needs p <= 64 and d <= 16");
   const int p = P->p, T = P->T;
   if (P->d_factor == nullptr) {
     ART_HIP(hipMalloc(&P->d_factor, (size_t)P->K * (p * (p + 1) / 2) * sizeof(double)));
-    ART_HIP(hipMalloc(&P->d_rpart, (size_t)P->nchunks * 16 * T * sizeof(double)));
+    const size_t rp = std::max((size_t)P->nchunks * 16 * T, (size_t)P->nrchunks * p);
+    ART_HIP(hipMalloc(&P->d_rpart, rp * sizeof(double)));
     ART_HIP(hipMalloc(&P->d_tpart, (size_t)P->nchunks * sizeof(double)));
     ART_HIP(hipMalloc(&P->d_need, (size_t)P->K * sizeof(int)));
   }
   const size_t smem = leaf_stage_smem_bytes(P->basis.d);
   const dim3 cgrid(P->nchunks), sgrid(P->nsys), block(256);
   timing_mark(P, 0, stream);
-  switch (T) {
-#define ART_CASE(TT)                                                                           \
-  case TT:                                                                                     \
-    gram_leaf_mixed_kernel<TT><<<cgrid, block, smem, stream>>>(X, y, w, P->basis, floor,       \
-                                                               P->d_chunks, P->d_partials,     \
-                                                               P->d_tpart, P->d_flags);        \
+  const size_t msmem = mixed_stage_smem_bytes(P->basis.d);
+  if (T == 3 && P->basis.d == 8) {
+    gram_leaf_mixed_kernel<3, 8><<<cgrid, block, msmem, stream>>>(X, y, w, P->basis, floor,
+                                                                 P->d_chunks, P->d_partials,
+                                                                 P->d_tpart, P->d_flags);
+  } else {
+    switch (T) {
+#define ART_CASE(TT)                                                                          \
+  case TT:                                                                                    \
+    gram_leaf_mixed_kernel<TT, 0><<<cgrid, block, msmem, stream>>>(X, y, w, P->basis, floor,  \
+                                                                  P->d_chunks, P->d_partials, \
+                                                                  P->d_tpart, P->d_flags);    \
     break;
-    ART_CASE(1) ART_CASE(2) ART_CASE(3) ART_CASE(4) ART_CASE(5)
+      ART_CASE(1) ART_CASE(2) ART_CASE(3) ART_CASE(4) ART_CASE(5)
 #undef ART_CASE
+    }
   }
   ART_HIP(hipGetLastError());
   timing_mark(P, 1, stream);
   int rc = launch_solve(P, P->nsys, P->nchunks, ridge, beta, status, lambda, stream, nullptr,
                         P->d_factor, P->d_tpart);
   if (rc != ART_OK) return rc;
-  switch (T) {
-#define ART_CASE(TT)                                                                           \
-  case TT:                                                                                     \
-    residual_leaf_kernel<TT><<<cgrid, block, smem, stream>>>(X, y, w, P->basis, floor,         \
-                                                             P->d_chunks, beta, status,        \
-                                                             P->d_rpart);                      \
+  // Residual: the register kernel for quadratic bases with bias-first or no
+  // bias (2 <= d <= 9), the feature-layout kernel otherwise.
+  const bool quad = P->basis.degree == 2 && !P->basis.bias_last && P->basis.d >= 2 &&
+                    P->basis.d <= 9;
+  if (quad) {
+    const dim3 rgrid(P->nrchunks);
+    switch (P->basis.d) {
+#define ART_CASE(DD)                                                                     \
+  case DD:                                                                               \
+    residual_quad_kernel<DD><<<rgrid, block, 0, stream>>>(X, y, w, P->basis, floor,       \
+                                                          P->d_rchunks, beta, status,    \
+                                                          P->d_rpart);                   \
     break;
-    ART_CASE(1) ART_CASE(2) ART_CASE(3) ART_CASE(4) ART_CASE(5)
+      ART_CASE(2) ART_CASE(3) ART_CASE(4) ART_CASE(5) ART_CASE(6) ART_CASE(7) ART_CASE(8)
+      ART_CASE(9)
 #undef ART_CASE
+    }
+  } else {
+    switch (T) {
+#define ART_CASE(TT)                                                                         \
+  case TT:                                                                                   \
+    residual_leaf_kernel<TT><<<cgrid, block, smem, stream>>>(X, y, w, P->basis, floor,       \
+                                                             P->d_chunks, beta, status,      \
+                                                             P->d_rpart);                    \
+    break;
+      ART_CASE(1) ART_CASE(2) ART_CASE(3) ART_CASE(4) ART_CASE(5)
+#undef ART_CASE
+    }
   }
   ART_HIP(hipGetLastError());
   correct_rows_kernel<<<sgrid, dim3(64 * ((p + 63) / 64)), (size_t)p * (p + 1) / 2 * sizeof(double),
-                        stream>>>(P->d_sys, p, 16 * T, P->d_rpart, P->d_factor, lambda, beta,
-                                  status, P->d_need, P->refine_tol);
+                        stream>>>(P->d_sys, quad ? P->d_roff : P->d_goff, p, quad ? p : 16 * T,
+                                  P->d_rpart, P->d_factor, lambda, beta, status, P->d_need,
+                                  P->refine_tol);
   ART_HIP(hipGetLastError());
   // Exact FP64 path for the flagged leaves only.
   launch_gram_leaf_f64(P, X, y, w, floor, P->d_need, stream);
