@@ -32,6 +32,32 @@ _i = ctypes.c_int
 _i64 = ctypes.c_int64
 _d = ctypes.c_double
 
+class Basis(ctypes.Structure):
+    """art_basis of include/art_hip.h."""
+
+    _fields_ = [("d", ctypes.c_int), ("p", ctypes.c_int), ("degree", ctypes.c_int),
+                ("bias", ctypes.c_int), ("bias_last", ctypes.c_int)]
+
+
+class SplitArgs(ctypes.Structure):
+    """art_split_args of include/art_hip.h."""
+
+    _fields_ = [
+        ("X", ctypes.c_void_p), ("y", ctypes.c_void_p), ("temperature", ctypes.c_void_p),
+        ("theta0", ctypes.c_void_p), ("node", ctypes.c_void_p),
+        ("n", ctypes.c_int), ("d", ctypes.c_int), ("basis", Basis),
+        *[(name, ctypes.c_double) for name in (
+            "ridge", "floor", "grad_atol", "grad_rtol", "alpha0", "rho", "armijo_c",
+            "alpha_min", "alpha_max", "grow", "recovery")],
+        *[(name, ctypes.c_int) for name in (
+            "max_iters", "max_backtracks", "heavy", "max_failures", "adaptive", "required",
+            "fallback", "hard_refit", "H", "mode", "anneal_iters")],
+        ("t0", ctypes.c_double), ("t1", ctypes.c_double), ("step", ctypes.c_double),
+        ("theta_out", ctypes.c_void_p), ("beta_out", ctypes.c_void_p), ("hard", ctypes.c_void_p),
+        ("info", ctypes.c_void_p), ("history", ctypes.c_void_p), ("scalars", ctypes.c_void_p),
+    ]
+
+
 _PROTOTYPES = {
     "art_last_error": (ctypes.c_char_p, []),
     "art_version": (_i, []),
@@ -48,6 +74,7 @@ _PROTOTYPES = {
     "art_plan_timing_enable": (_i, [_p, _i]),
     "art_plan_timing_read": (_i, [_p, _p, _p, _i]),
     "art_plan_destroy": (None, [_p]),
+    "art_split_optimize": (_i, [ctypes.POINTER(SplitArgs), _i, _p]),
 }
 
 

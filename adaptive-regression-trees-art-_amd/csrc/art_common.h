@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "art_hip.h"
+
 namespace art {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
@@ -12,17 +14,17 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kWave = 64;
 
+// Host-side error reporting shared by every entry point: the message is
+// returned by art_last_error() on the calling thread.
+int fail(int code, const char* msg);
+int hip_fail(hipError_t e, const char* where);
+
 // Feature-basis descriptor shared by every kernel that builds leaf features
-// from raw inputs.  Column order (art/_cuda_features.py:13-29):
+// from raw inputs (the C ABI's art_basis).  Column order
+// (art/_cuda_features.py:13-29):
 //   degree 2: [1?, x_0..x_{d-1}, x_i*x_j for i<=j with i outermost]
 //   degree 1: [1?, x_0..x_{d-1}] or, with bias_last, [x_0..x_{d-1}, 1]
-struct Basis {
-  int d;
-  int p;
-  int degree;
-  int bias;       // include_bias
-  int bias_last;  // affine order (degree 1 only)
-};
+using Basis = art_basis;
 
 // Factor pair of feature column f: -1 stands for the constant 1.
 __host__ __device__ inline void basis_factors(const Basis& B, int f, int& a, int& b) {
@@ -94,5 +96,108 @@ struct SystemDesc {
 
 // Index of element (i, j), j <= i, in packed lower-triangular row storage.
 __host__ __device__ inline int tri(int i, int j) { return i * (i + 1) / 2 + j; }
+
+// Tile pairs (t1 <= t2) of a T-tile symmetric matrix, enumerated t1 outermost.
+__host__ __device__ inline int pair_index(int t1, int t2, int T) {
+  return t1 * T - t1 * (t1 - 1) / 2 + (t2 - t1);
+}
+
+__device__ inline void pair_tiles(int q, int T, int& t1, int& t2) {
+  t1 = 0;
+  while (q >= T - t1) {
+    q -= T - t1;
+    ++t1;
+  }
+  t2 = t1 + q;
+}
+
+
+// ---------------------------------------------------------------------------
+// Row-per-thread dense SPD systems in LDS (thread i owns row i of the packed
+// lower triangle).  The Cholesky runs in Crout order -- column k is
+// s_i = A[i][k] - sum_{j<k} L[i][j] L[k][j], a dot product of the thread's own
+// row with row k (a broadcast read) -- and the substitutions are
+// column-oriented with one barrier per step.  Every thread of the block calls
+// these (they synchronize the block).
+
+// Crout Cholesky of rows 0..p-1 of packed M; returns info (uniform).
+__device__ inline int rows_cholesky(double* M, double* piv, int p, int i) {
+  for (int k = 0; k < p; ++k) {
+    double s = 0.0;
+    if (i >= k && i < p) {
+      const double* ri = M + tri(i, 0);
+      const double* rk = M + tri(k, 0);
+      double s0 = ri[k], s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      int j = 0;
+      for (; j + 3 < k; j += 4) {
+        s0 -= ri[j] * rk[j];
+        s1 -= ri[j + 1] * rk[j + 1];
+        s2 -= ri[j + 2] * rk[j + 2];
+        s3 -= ri[j + 3] * rk[j + 3];
+      }
+      for (; j < k; ++j) s0 -= ri[j] * rk[j];
+      s = (s0 + s1) + (s2 + s3);
+    }
+    if (i == k) *piv = s;
+    __syncthreads();
+    const double dkk = *piv;
+    if (!(dkk > 0.0)) return k + 1;
+    const double lkk = sqrt(dkk);
+    if (i == k) M[tri(k, k)] = lkk;
+    else if (i > k && i < p) M[tri(i, k)] = s / lkk;
+    __syncthreads();
+  }
+  return 0;
+}
+
+// Solve L L^T x = b with L in packed M; b and the result are thread i's entry.
+__device__ inline double rows_solve(const double* M, double* bc, int p, int i, double b) {
+  double z = i < p ? b : 0.0;
+  for (int k = 0; k < p; ++k) {
+    if (i == k) {
+      z = z / M[tri(k, k)];
+      bc[k & 1] = z;
+    }
+    __syncthreads();
+    const double zk = bc[k & 1];
+    if (i > k && i < p) z -= M[tri(i, k)] * zk;
+  }
+  __syncthreads();
+  double x = z;
+  for (int k = p - 1; k >= 0; --k) {
+    if (i == k) {
+      x = x / M[tri(k, k)];
+      bc[k & 1] = x;
+    }
+    __syncthreads();
+    const double xk = bc[k & 1];
+    if (i < k) x -= M[tri(k, i)] * xk;
+  }
+  return x;
+}
+
+__device__ inline int block_or(int v, int* scratch) {
+  v = wave_or(v);
+  if (blockDim.x == 64) return v;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) scratch[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int r = 0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r |= scratch[w];
+  __syncthreads();
+  return r;
+}
+
+__device__ inline double block_sum(double v, double* scratch) {
+  v = wave_sum(v);
+  if (blockDim.x == 64) return v;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) scratch[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = 0.0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += scratch[w];
+  __syncthreads();
+  return r;
+}
 
 }  // namespace art

@@ -123,19 +123,6 @@ struct EqualChunks {  // equal-size nodes: active ids, fixed chunks per node
   }
 };
 
-__host__ __device__ inline int pair_index(int t1, int t2, int T) {
-  return t1 * T - t1 * (t1 - 1) / 2 + (t2 - t1);
-}
-
-__device__ inline void pair_tiles(int q, int T, int& t1, int& t2) {
-  t1 = 0;
-  while (q >= T - t1) {
-    q -= T - t1;
-    ++t1;
-  }
-  t2 = t1 + q;
-}
-
 // ---------------------------------------------------------------------------
 // FP64 Gram kernel.  grid = (chunks, pair groups, sides), block = 4 waves.
 // Each wave takes every 4th group of 4 rows; one v_mfma_f64_16x16x4 per tile
@@ -526,86 +513,6 @@ __global__ __launch_bounds__(64) void solve_f64_kernel(const SystemDesc* __restr
 constexpr int kRowsMaxP = 120;
 
 __device__ inline int rows_threads(int p) { return 64 * ((p + 63) / 64); }
-
-// Crout Cholesky of rows 0..p-1 of packed M; returns info (uniform).
-__device__ inline int rows_cholesky(double* M, double* piv, int p, int i) {
-  for (int k = 0; k < p; ++k) {
-    double s = 0.0;
-    if (i >= k && i < p) {
-      const double* ri = M + tri(i, 0);
-      const double* rk = M + tri(k, 0);
-      double s0 = ri[k], s1 = 0.0, s2 = 0.0, s3 = 0.0;
-      int j = 0;
-      for (; j + 3 < k; j += 4) {
-        s0 -= ri[j] * rk[j];
-        s1 -= ri[j + 1] * rk[j + 1];
-        s2 -= ri[j + 2] * rk[j + 2];
-        s3 -= ri[j + 3] * rk[j + 3];
-      }
-      for (; j < k; ++j) s0 -= ri[j] * rk[j];
-      s = (s0 + s1) + (s2 + s3);
-    }
-    if (i == k) *piv = s;
-    __syncthreads();
-    const double dkk = *piv;
-    if (!(dkk > 0.0)) return k + 1;
-    const double lkk = sqrt(dkk);
-    if (i == k) M[tri(k, k)] = lkk;
-    else if (i > k && i < p) M[tri(i, k)] = s / lkk;
-    __syncthreads();
-  }
-  return 0;
-}
-
-// Solve L L^T x = b with L in packed M; b and the result are thread i's entry.
-__device__ inline double rows_solve(const double* M, double* bc, int p, int i, double b) {
-  double z = i < p ? b : 0.0;
-  for (int k = 0; k < p; ++k) {
-    if (i == k) {
-      z = z / M[tri(k, k)];
-      bc[k & 1] = z;
-    }
-    __syncthreads();
-    const double zk = bc[k & 1];
-    if (i > k && i < p) z -= M[tri(i, k)] * zk;
-  }
-  __syncthreads();
-  double x = z;
-  for (int k = p - 1; k >= 0; --k) {
-    if (i == k) {
-      x = x / M[tri(k, k)];
-      bc[k & 1] = x;
-    }
-    __syncthreads();
-    const double xk = bc[k & 1];
-    if (i < k) x -= M[tri(k, i)] * xk;
-  }
-  return x;
-}
-
-__device__ inline int block_or(int v, int* scratch) {
-  v = wave_or(v);
-  if (blockDim.x == 64) return v;
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) scratch[threadIdx.x >> 6] = v;
-  __syncthreads();
-  int r = 0;
-  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r |= scratch[w];
-  __syncthreads();
-  return r;
-}
-
-__device__ inline double block_sum(double v, double* scratch) {
-  v = wave_sum(v);
-  if (blockDim.x == 64) return v;
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) scratch[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double r = 0.0;
-  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += scratch[w];
-  __syncthreads();
-  return r;
-}
 
 // Reduce + regularize + factor + solve, same contract as solve_f64_kernel;
 // optionally stores the factor rows (packed) and skips unmasked systems.
@@ -1163,9 +1070,10 @@ __global__ __launch_bounds__(256) void check_finite_kernel(const double* __restr
 using namespace art;
 
 namespace {
-
 thread_local char g_err[512];
+}  // namespace
 
+namespace art {
 int fail(int code, const char* msg) {
   std::snprintf(g_err, sizeof(g_err), "%s", msg);
   return code;
@@ -1175,6 +1083,9 @@ int hip_fail(hipError_t e, const char* where) {
   std::snprintf(g_err, sizeof(g_err), "%s: %s", where, hipGetErrorString(e));
   return ART_ERR_HIP;
 }
+}  // namespace art
+
+namespace {
 
 #define ART_HIP(call)                                  \
   do {                                                 \

@@ -1,0 +1,741 @@
+// Soft-oblique split optimizer: one persistent workgroup per problem
+// (node x restart) runs the whole projected-gradient loop on the device.
+//
+// Hot-path pieces a4-a6: soft objective, gradient and tangent direction
+// (art/objectives.py:132-340, art/splitters.py:71-98), the frozen-model Armijo
+// search with AdaptiveAlpha and its retry policy (art/optimizers.py:20-121,
+// art/splitters.py:400-460), commit/retraction, stop reasons, the
+// last-feasible min-side fallback and the hard refit
+// (art/splitters.py:461-690, art/_cuda_batched_optimizer.py:390-526).  The
+// reference drives these from the host with a device->host transfer every
+// iteration and every backtrack batch (art/_cuda_batched_optimizer.py:323,360);
+// here the control flow, the AdaptiveAlpha state and every history live on
+// the device, and one launch runs all problems of a cohort to completion.
+//
+// Per iteration a workgroup makes three passes over its node's rows:
+//   fit       both sides' augmented Grams [phi | y]^T diag(q) [phi | y] by
+//             v_mfma_f64_16x16x4 (lane (col, kq) builds feature 16t+col of row
+//             4s+kq), then trace-scaled ridge, Cholesky and solve in LDS;
+//   stats     one thread per row: residuals, loss, gradient coefficients and
+//             X^T c (block reductions);
+//   search    one thread per row evaluates H Armijo candidates at once with
+//             the frozen residuals recomputed from the two coefficient vectors.
+// Layout: X (K, n, d), y (K, n) float64 row-major; one node per problem index.
+
+#include "art_common.h"
+#include "art_hip.h"
+
+#include <cmath>
+#include <cstdio>
+#include <new>
+
+namespace art {
+
+constexpr int kSplitMaxP = 32;
+constexpr int kSplitMaxD = 31;
+constexpr int kSplitMaxH = 8;  // candidates per trial pass
+constexpr int kSplitScalars = 6;
+
+// Stop reasons (strings in the Python layer; order fixed by the C ABI).
+enum StopReason {
+  kMaxIters = 0,
+  kGradientTolerance = 1,
+  kNonDescent = 2,
+  kLineSearchFailed = 3,
+  kMaxLineSearchFailures = 4,
+  kAlphaMinReached = 5,
+  kCholeskyFailure = 6,
+  kMinSidePoints = 7,
+  kNonfiniteSplit = 8,
+};
+
+__device__ inline double gate(double score) {
+  return 1.0 / (1.0 + exp(-fmin(50.0, fmax(-50.0, score))));
+}
+
+struct SplitShared {
+  double theta[kSplitMaxD + 1];
+  double pdir[kSplitMaxD + 1];
+  double last[kSplitMaxD + 1];
+  double beta[2][kSplitMaxP];
+  double lam[2];
+  int status[2];
+  double geometry[4];
+  double packet[7];
+  double red[4 * 256];        // MFMA tile reduction; reused by block reductions
+  double M[2][(kSplitMaxP + 1) * (kSplitMaxP + 2) / 2];
+  double piv[2];
+  double bc[2];
+  int ired[4];
+  double dred[4 * (kSplitMaxD + 4)];
+  double tot[kSplitMaxD + 4];
+};
+
+struct SplitCtx {
+  const double* X;
+  const double* y;
+  int n, d, p;
+  Basis B;
+  double T;
+  double ridge;
+};
+
+// Feature f of the augmented row [phi(x) | y | 0...].
+__device__ inline double aug_feature(const Basis& B, const double* x, double yv, int fa, int fb,
+                                     int kind) {
+  if (kind == 1) return yv;
+  if (kind == 2) return 0.0;
+  double v = fa >= 0 ? x[fa] : 1.0;
+  if (fb >= 0) v *= x[fb];
+  return v;
+}
+
+__device__ inline void column_kind(const Basis& B, int f, int& fa, int& fb, int& kind) {
+  fa = fb = -1;
+  kind = 2;
+  if (f < B.p) {
+    basis_factors(B, f, fa, fb);
+    kind = 0;
+  } else if (f == B.p) {
+    kind = 1;
+  }
+}
+
+// Row feature vector into registers (p <= PM).
+template <int PM>
+__device__ inline void row_features(const Basis& B, const double* x, double (&phi)[PM]) {
+#pragma unroll
+  for (int f = 0; f < PM; ++f) {
+    if (f < B.p) {
+      int a, b;
+      basis_factors(B, f, a, b);
+      double v = a >= 0 ? x[a] : 1.0;
+      if (b >= 0) v *= x[b];
+      phi[f] = v;
+    } else {
+      phi[f] = 0.0;
+    }
+  }
+}
+
+__device__ inline double dotw(const double* x, const double* w, int d) {
+  double u = 0.0;
+  for (int j = 0; j < d; ++j) u += x[j] * w[j];
+  return u;
+}
+
+// Soft (hard = false) or hard (hard = true) weighted fits of both sides at
+// the current theta.  Writes beta, lam and status (0 / -1 bad gate /
+// pivot / -2 non-finite / -3 solve) for both sides into shared state.
+template <int T>
+__device__ void fit_sides(const SplitCtx& C, SplitShared& S, bool hard, double floor,
+                          bool half = false) {
+  constexpr int NP = T * (T + 1) / 2;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int col = lane & 15, kq = lane >> 4;
+  const int d = C.d, p = C.p, n = C.n;
+  int fa[T], fb[T], kind[T];
+#pragma unroll
+  for (int t = 0; t < T; ++t) column_kind(C.B, 16 * t + col, fa[t], fb[t], kind[t]);
+  f64x4 acc0[NP], acc1[NP];
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+    acc0[q] = f64x4{0.0, 0.0, 0.0, 0.0};
+    acc1[q] = f64x4{0.0, 0.0, 0.0, 0.0};
+  }
+  const double z = S.theta[d];
+  int bad = 0;
+  for (int s = 0; 16 * s < n; ++s) {
+    const int r = 16 * s + 4 * wave + kq;
+    const bool valid = r < n;
+    const double* x = C.X + (size_t)(valid ? r : 0) * d;
+    const double yv = valid ? C.y[r] : 0.0;
+    const double u = dotw(x, S.theta, d);
+    double pi;
+    if (half) {
+      pi = 0.5;
+    } else if (hard) {
+      pi = (isfinite(u) && isfinite(z)) ? (double)(u >= z) : __builtin_nan("");
+    } else {
+      const double score = (u - z) / C.T;
+      pi = (C.T > 0.0 && isfinite(C.T) && isfinite(score)) ? gate(score) : __builtin_nan("");
+    }
+    const bool okg = isfinite(pi) && pi >= 0.0 && pi <= 1.0;
+    bad |= valid && !okg;
+    const double q0 = (valid && okg) ? fmax(1.0 - pi, floor) : 0.0;
+    const double q1 = (valid && okg) ? fmax(pi, floor) : 0.0;
+    double ph[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) ph[t] = valid ? aug_feature(C.B, x, yv, fa[t], fb[t], kind[t]) : 0.0;
+    int q = 0;
+#pragma unroll
+    for (int t1 = 0; t1 < T; ++t1)
+#pragma unroll
+      for (int t2 = t1; t2 < T; ++t2, ++q) {
+        acc0[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(q0 * ph[t1], ph[t2], acc0[q], 0, 0, 0);
+        acc1[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(q1 * ph[t1], ph[t2], acc1[q], 0, 0, 0);
+      }
+  }
+  // Tiles into packed lower augmented systems: element (a, b) of tile (t1, t2)
+  // is G[16 t1 + a][16 t2 + b] and lands in slot (16 t2 + b, 16 t1 + a).
+  const int nel = (p + 1) * (p + 2) / 2;
+  for (int e = tid; e < nel; e += blockDim.x) {
+    S.M[0][e] = 0.0;
+    S.M[1][e] = 0.0;
+  }
+  for (int side = 0; side < 2; ++side) {
+    int q = 0;
+#pragma unroll
+    for (int t1 = 0; t1 < T; ++t1)
+#pragma unroll
+      for (int t2 = t1; t2 < T; ++t2, ++q) {
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          S.red[wave * 256 + (kq + 4 * r) * 16 + col] = side == 0 ? acc0[q][r] : acc1[q][r];
+        __syncthreads();
+        const double v = ((S.red[tid] + S.red[256 + tid]) + S.red[512 + tid]) + S.red[768 + tid];
+        const int rr = 16 * t1 + (tid >> 4), cc = 16 * t2 + (tid & 15);
+        if (rr <= cc && cc <= p) S.M[side][tri(cc, rr)] = v;
+      }
+  }
+  __syncthreads();
+  bad = block_or(bad, S.ired);
+  for (int side = 0; side < 2; ++side) {
+    double* M = S.M[side];
+    const double tr = block_sum(tid < p ? M[tri(tid, tid)] : 0.0, S.red);
+    const double ridge = C.ridge;
+    const double lam = ridge == 0.0 ? 0.0 : ridge * (tr / p);
+    int st = bad ? -1 : ((!isfinite(tr) || !isfinite(lam)) ? -2 : 0);
+    double x = 0.0;
+    if (st == 0) {
+      if (tid < p) M[tri(tid, tid)] += lam;
+      __syncthreads();
+      st = rows_cholesky(M, &S.piv[side], p, tid);
+      if (st == 0) {
+        int nf = 0;
+        if (tid < p) {
+          for (int j = 0; j <= tid; ++j) nf |= !isfinite(M[tri(tid, j)]);
+          nf |= !isfinite(M[tri(p, tid)]);
+        }
+        if (block_or(nf, S.ired)) st = -2;
+      }
+      if (st == 0) {
+        x = rows_solve(M, S.bc, p, tid, tid < p ? M[tri(p, tid)] : 0.0);
+        if (block_or(tid < p && !isfinite(x), S.ired)) st = -3;
+      }
+    }
+    if (tid < p) S.beta[side][tid] = st == 0 ? x : __builtin_nan("");
+    if (tid == 0) {
+      S.status[side] = st;
+      S.lam[side] = lam;
+    }
+    __syncthreads();
+  }
+}
+
+// Residual statistics at the current fit; then the tangent direction,
+// geometry and control packet (art/_cuda_optimizer_kernels.cu:108-195):
+// packet = [loss, |projected grad|, slope, n_right, lambda_L, lambda_R, bad].
+template <int T>
+__device__ void stats_and_direction(const SplitCtx& C, SplitShared& S) {
+  constexpr int PM = 16 * T;  // p + 1 <= 16 T, and d <= p
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int d = C.d, p = C.p, n = C.n;
+  const double z = S.theta[d];
+  const double inv = 1.0 / (n * C.T);
+  double acc[PM + 3];
+#pragma unroll
+  for (int k = 0; k < PM + 3; ++k) acc[k] = 0.0;  // loss, gz, count, gw[0..d)
+  for (int r = tid; r < n; r += blockDim.x) {
+    const double* x = C.X + (size_t)r * d;
+    double phi[PM];
+    row_features<PM>(C.B, x, phi);
+    double pl = 0.0, pr = 0.0;
+#pragma unroll
+    for (int f = 0; f < PM; ++f)
+      if (f < p) {
+        pl += phi[f] * S.beta[0][f];
+        pr += phi[f] * S.beta[1][f];
+      }
+    const double u = dotw(x, S.theta, d);
+    const double pi = gate((u - z) / C.T);
+    double l = C.y[r] - pl, rr = C.y[r] - pr;
+    l *= l;
+    rr *= rr;
+    const double c = (rr - l) * pi * (1.0 - pi) * inv;
+    acc[0] += ((1.0 - pi) * l + pi * rr) / n;
+    acc[1] -= c;
+    acc[2] += pi >= 0.5 ? 1.0 : 0.0;
+#pragma unroll
+    for (int j = 0; j < PM; ++j)
+      if (j < d) acc[3 + j] += c * x[j];
+  }
+  constexpr int W = kSplitMaxD + 4;
+#pragma unroll
+  for (int k = 0; k < PM + 3; ++k) {
+    if (k < 3 + d) {
+      const double v = wave_sum(acc[k]);
+      if (lane == 0) S.dred[wave * W + k] = v;
+    }
+  }
+  __syncthreads();
+  if (tid < 3 + d)
+    S.tot[tid] = ((S.dred[tid] + S.dred[W + tid]) + S.dred[2 * W + tid]) + S.dred[3 * W + tid];
+  __syncthreads();
+  if (tid == 0) {
+    const double* w = S.theta;
+    const double* g = S.tot + 3;
+    double ww = 0.0, wg = 0.0;
+    for (int j = 0; j < d; ++j) {
+      ww += w[j] * w[j];
+      wg += w[j] * g[j];
+    }
+    const double proj = wg / ww;
+    double pp = 0.0, wp = 0.0, gp = 0.0;
+    for (int j = 0; j < d; ++j) {
+      const double v = -g[j] + proj * w[j];
+      S.pdir[j] = v;
+      pp += v * v;
+      wp += w[j] * v;
+      gp += g[j] * v;
+    }
+    const double gzt = S.tot[1], norm = sqrt(pp + gzt * gzt);
+    S.pdir[d] = -gzt;
+    S.geometry[0] = ww;
+    S.geometry[1] = wp;
+    S.geometry[2] = pp;
+    S.geometry[3] = -gzt;
+    S.packet[0] = S.tot[0];
+    S.packet[1] = norm;
+    S.packet[2] = gp - gzt * gzt;
+    S.packet[3] = S.tot[2];
+    S.packet[4] = S.lam[0];
+    S.packet[5] = S.lam[1];
+    const bool failed = S.status[0] != 0 || S.status[1] != 0;
+    S.packet[6] = (failed || !(ww > 0.0) || !isfinite(ww) || !isfinite(norm) ||
+                   !isfinite(S.packet[2]) || !isfinite(S.tot[0]))
+                      ? 1.0
+                      : 0.0;
+  }
+  __syncthreads();
+}
+
+// Frozen-model trial losses of `count` candidates alphas[0..count).
+template <int T>
+__device__ void trial_losses(const SplitCtx& C, SplitShared& S, const double* alphas, int count,
+                             double* out) {
+  constexpr int PM = 16 * T;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int d = C.d, p = C.p, n = C.n;
+  const double z = S.theta[d], temp = C.T;
+  const double g0 = S.geometry[0], g1 = S.geometry[1], g2 = S.geometry[2], g3 = S.geometry[3];
+  double f[kSplitMaxH];
+  double rinv[kSplitMaxH];
+  bool okc[kSplitMaxH];
+#pragma unroll
+  for (int k = 0; k < kSplitMaxH; ++k) {
+    f[k] = 0.0;
+    const double al = k < count ? alphas[k] : 0.0;
+    const double r2 = g0 + 2.0 * al * g1 + al * al * g2;
+    okc[k] = (r2 > 0.0) && isfinite(r2) && (temp > 0.0) && isfinite(temp);
+    rinv[k] = okc[k] ? 1.0 / sqrt(r2) : 0.0;
+  }
+  for (int r = tid; r < n; r += blockDim.x) {
+    const double* x = C.X + (size_t)r * d;
+    double phi[PM];
+    row_features<PM>(C.B, x, phi);
+    double pl = 0.0, pr = 0.0;
+#pragma unroll
+    for (int ff = 0; ff < PM; ++ff)
+      if (ff < p) {
+        pl += phi[ff] * S.beta[0][ff];
+        pr += phi[ff] * S.beta[1][ff];
+      }
+    double l = C.y[r] - pl, rr = C.y[r] - pr;
+    l *= l;
+    rr *= rr;
+    const double u = dotw(x, S.theta, d);
+    const double v = dotw(x, S.pdir, d);
+#pragma unroll
+    for (int k = 0; k < kSplitMaxH; ++k) {
+      if (k < count) {
+        const double al = alphas[k];
+        const double score = ((u + al * v) * rinv[k] - z - al * g3) / temp;
+        const double pi = gate(score);
+        const double fk = ((1.0 - pi) * l + pi * rr) / n;
+        f[k] += (okc[k] && isfinite(score) && isfinite(fk)) ? fk : INFINITY;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kSplitMaxH; ++k) {
+    if (k < count) {
+      const double v = wave_sum(f[k]);
+      if (lane == 0) S.dred[wave * (kSplitMaxD + 4) + k] = v;
+    }
+  }
+  __syncthreads();
+  if (tid < count)
+    out[tid] = ((S.dred[tid] + S.dred[(kSplitMaxD + 4) + tid]) +
+                S.dred[2 * (kSplitMaxD + 4) + tid]) +
+               S.dred[3 * (kSplitMaxD + 4) + tid];
+  __syncthreads();
+}
+
+// Hard partition statistics after a hard fit (art/_cuda_batched_optimizer.cu:92-120):
+// [MSE, MSE_L, MSE_R, n_right, lambda_L, lambda_R, bad].
+template <int T>
+__device__ void hard_stats(const SplitCtx& C, SplitShared& S, bool refit, double* out,
+                           bool all_right = false) {
+  constexpr int PM = 16 * T;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int d = C.d, p = C.p, n = C.n;
+  const double z = S.theta[d];
+  double sl = 0.0, sr = 0.0, cnt = 0.0;
+  int bad = 0;
+  for (int r = tid; r < n; r += blockDim.x) {
+    const double* x = C.X + (size_t)r * d;
+    const double u = dotw(x, S.theta, d);
+    const double pi = all_right ? 1.0
+                      : ((isfinite(u) && isfinite(z)) ? (double)(u >= z) : __builtin_nan(""));
+    bad |= !isfinite(pi);
+    const bool right = pi >= 0.5;
+    if (refit) {
+      double phi[PM];
+      row_features<PM>(C.B, x, phi);
+      double pred = 0.0;
+#pragma unroll
+      for (int f = 0; f < PM; ++f)
+        if (f < p) pred += phi[f] * S.beta[right ? 1 : 0][f];
+      const double e = C.y[r] - pred;
+      if (right) sr += e * e;
+      else sl += e * e;
+    }
+    cnt += right ? 1.0 : 0.0;
+  }
+  {
+    const double v0 = wave_sum(sl), v1 = wave_sum(sr), v2 = wave_sum(cnt);
+    if (lane == 0) {
+      S.dred[wave * (kSplitMaxD + 4) + 0] = v0;
+      S.dred[wave * (kSplitMaxD + 4) + 1] = v1;
+      S.dred[wave * (kSplitMaxD + 4) + 2] = v2;
+    }
+  }
+  bad = block_or(bad, S.ired);
+  __syncthreads();
+  if (tid == 0) {
+    constexpr int W = kSplitMaxD + 4;
+    const double t[3] = {((S.dred[0] + S.dred[W]) + S.dred[2 * W]) + S.dred[3 * W],
+                         ((S.dred[1] + S.dred[W + 1]) + S.dred[2 * W + 1]) + S.dred[3 * W + 1],
+                         ((S.dred[2] + S.dred[W + 2]) + S.dred[2 * W + 2]) + S.dred[3 * W + 2]};
+    out[3] = t[2];
+    if (refit) {
+      out[0] = (t[0] + t[1]) / n;
+      out[1] = t[2] < n ? t[0] / (n - t[2]) : 0.0;
+      out[2] = t[2] > 0 ? t[1] / t[2] : 0.0;
+      out[4] = S.lam[0];
+      out[5] = S.lam[1];
+      out[6] = (S.status[0] != 0 || S.status[1] != 0 || !isfinite(out[0])) ? 1.0 : 0.0;
+    } else {
+      out[0] = out[1] = out[2] = 0.0;
+      out[4] = out[5] = 0.0;
+      out[6] = bad ? 1.0 : 0.0;
+    }
+  }
+  __syncthreads();
+}
+
+__device__ inline double clip_alpha(const art_split_args* a, double v) {
+  return fmin(fmax(v, a->alpha_min), a->alpha_max);
+}
+
+template <int T>
+__global__ __launch_bounds__(256) void split_optimize_kernel(art_split_args args) {
+  __shared__ SplitShared S;
+  __shared__ double alphas[kSplitMaxH];
+  __shared__ double losses[kSplitMaxH];
+  __shared__ int sflag[8];
+  const int prob = blockIdx.x;
+  const int tid = threadIdx.x;
+  const art_split_args* a = &args;
+  const int node = a->node[prob];
+  const int n = a->n, d = a->d;
+  SplitCtx C;
+  C.X = a->X + (size_t)node * n * d;
+  C.y = a->y + (size_t)node * n;
+  C.n = n;
+  C.d = d;
+  C.B = a->basis;
+  C.p = a->basis.p;
+  C.T = a->temperature[node];
+  C.ridge = a->ridge;
+  const int p = C.p;
+  const int MI = a->max_iters;
+  double* hist = a->history + (size_t)prob * 5 * (MI + 1);
+  if (tid <= d) S.theta[tid] = a->theta0[(size_t)prob * (d + 1) + tid];
+  __syncthreads();
+  // Unit-normalize the start (art/splitters.py:71-83 is applied by the caller too).
+  if (tid == 0) {
+    double nw = 0.0;
+    for (int j = 0; j < d; ++j) nw += S.theta[j] * S.theta[j];
+    nw = sqrt(nw);
+    for (int j = 0; j < d; ++j) S.theta[j] /= nw;
+  }
+  __syncthreads();
+
+  if (a->mode == 2) {
+    // Parent fit (art/_cuda_batched_optimizer.py:403-411): pi = 1/2 on both
+    // sides is the unweighted fit (the factor cancels in G, h and the
+    // trace-scaled ridge); its MSE over all rows is the parent loss.
+    fit_sides<T>(C, S, false, 0.0, true);
+    double* hard = a->hard + (size_t)prob * 8;
+    hard_stats<T>(C, S, true, hard, true);
+    if (tid < p) {
+      a->beta_out[((size_t)prob * 2 + 0) * p + tid] = S.beta[1][tid];
+      a->beta_out[((size_t)prob * 2 + 1) * p + tid] = S.beta[1][tid];
+    }
+    if (tid == 0) {
+      hard[7] = 0.0;
+      int* info = a->info + (size_t)prob * 8;
+      for (int k = 0; k < 8; ++k) info[k] = 0;
+      info[0] = hard[6] != 0.0 ? kCholeskyFailure : kMaxIters;
+      info[6] = hard[6] != 0.0;
+    }
+    return;
+  }
+
+  int reason = kMaxIters, converged = 0, it = 0, feasible_it = -1, error = 0;
+  long physical = 0, logical = 0;
+  double alpha_ctrl = clip_alpha(a, a->alpha0);
+  const int required = a->required;
+  const int S_ = a->max_backtracks + 1;
+  const int H = a->H;
+
+  if (a->mode == 1) {
+    // Annealed fixed-step projected gradient: T_k geometric from t0 to t1.
+    for (int k = 0; k < a->anneal_iters; ++k) {
+      C.T = a->t0 * pow(a->t1 / a->t0, (double)k / (double)max(a->anneal_iters - 1, 1));
+      fit_sides<T>(C, S, false, a->floor);
+      stats_and_direction<T>(C, S);
+      if (tid == 0 && k <= MI) hist[k] = S.packet[0];
+      if (S.packet[6] != 0.0) {
+        error = kCholeskyFailure;
+        break;
+      }
+      if (tid == 0) {
+        const double st = a->step;
+        const double r = sqrt(S.geometry[0] + 2.0 * st * S.geometry[1] + st * st * S.geometry[2]);
+        for (int j = 0; j < d; ++j) S.theta[j] = (S.theta[j] + st * S.pdir[j]) / r;
+        S.theta[d] += st * S.pdir[d];
+      }
+      __syncthreads();
+      ++it;
+    }
+    C.T = a->temperature[node];
+  } else {
+    fit_sides<T>(C, S, false, a->floor);
+    stats_and_direction<T>(C, S);
+    const double g0 = S.packet[1];
+    const double tol = a->grad_atol + a->grad_rtol * g0;
+    if (tid == 0) {
+      hist[0 * (MI + 1) + 0] = S.packet[0];
+      hist[1 * (MI + 1) + 0] = S.packet[1];
+      a->scalars[(size_t)prob * kSplitScalars + 0] = g0;
+      a->scalars[(size_t)prob * kSplitScalars + 1] = tol;
+    }
+    auto feasible_now = [&]() {
+      return S.packet[6] == 0.0 && S.packet[3] >= required && n - S.packet[3] >= required;
+    };
+    if (feasible_now()) {
+      if (tid <= d) S.last[tid] = S.theta[tid];
+      feasible_it = 0;
+    }
+    __syncthreads();
+    while (true) {
+      if (S.packet[6] != 0.0) {
+        error = kCholeskyFailure;
+        break;
+      }
+      if (it >= MI) break;
+      if (S.packet[1] <= tol) {
+        converged = 1;
+        reason = kGradientTolerance;
+        break;
+      }
+      if (S.packet[2] >= 0.0) {
+        reason = kNonDescent;
+        break;
+      }
+      // Line search with retries (art/splitters.py:423-460).
+      int failures = 0, accepted = 0, selected = -1;
+      double step = 0.0, value = INFINITY;
+      while (true) {
+        // Candidates alpha_k = a0 rho^k by repeated multiplication
+        // (art/_cuda_batched_optimizer.cu:47-57); a logical batch of H
+        // candidates is evaluated in passes of at most kSplitMaxH.
+        double acur = a->adaptive ? alpha_ctrl : a->alpha0;
+        selected = -1;
+        for (int offset = 0; offset < S_; offset += H) {
+          const int count = min(H, S_ - offset);
+          int sel = -1;
+          for (int sub = 0; sub < count; sub += kSplitMaxH) {
+            const int c2 = min(kSplitMaxH, count - sub);
+            __syncthreads();  // every thread has finished reading the last pass
+            if (tid == 0) {
+              double t = acur;
+              for (int k = 0; k < c2; ++k) {
+                alphas[k] = t;
+                t *= a->rho;
+              }
+            }
+            for (int k = 0; k < c2; ++k) acur *= a->rho;
+            __syncthreads();
+            trial_losses<T>(C, S, alphas, c2, losses);
+            for (int k = 0; k < c2 && sel < 0; ++k) {
+              value = losses[k];
+              step = alphas[k];
+              if (isfinite(value) && value <= S.packet[0] + a->armijo_c * step * S.packet[2])
+                sel = offset + sub + k;
+            }
+          }
+          physical += count;
+          logical += sel >= 0 ? sel - offset + 1 : count;
+          if (sel >= 0) {
+            selected = sel;
+            break;
+          }
+        }
+        const bool ok = selected >= 0;
+        if (a->adaptive) {
+          if (!ok) alpha_ctrl = clip_alpha(a, step * a->rho);
+          else if (selected == 0) alpha_ctrl = clip_alpha(a, step * a->grow);
+          else if (selected >= a->heavy) alpha_ctrl = clip_alpha(a, step * a->recovery);
+          else alpha_ctrl = clip_alpha(a, step / a->rho);
+        }
+        if (ok) {
+          accepted = 1;
+          break;
+        }
+        ++failures;
+        if (!a->adaptive) {
+          reason = kLineSearchFailed;
+          break;
+        }
+        if (failures >= a->max_failures) {
+          reason = kMaxLineSearchFailures;
+          break;
+        }
+        if (alpha_ctrl <= a->alpha_min) {
+          reason = kAlphaMinReached;
+          break;
+        }
+      }
+      if (!accepted) break;
+      ++it;
+      // Commit: w <- (w + alpha p_w) / r(alpha), z <- z + alpha p_z.
+      __syncthreads();
+      if (tid == 0) {
+        const double r = sqrt(S.geometry[0] + 2.0 * step * S.geometry[1] +
+                              step * step * S.geometry[2]);
+        for (int j = 0; j < d; ++j) S.theta[j] = (S.theta[j] + step * S.pdir[j]) / r;
+        S.theta[d] += step * S.pdir[d];
+      }
+      __syncthreads();
+      fit_sides<T>(C, S, false, a->floor);
+      stats_and_direction<T>(C, S);
+      if (tid == 0) {
+        hist[0 * (MI + 1) + it] = S.packet[0];
+        hist[1 * (MI + 1) + it] = S.packet[1];
+        hist[2 * (MI + 1) + it - 1] = step;
+        hist[3 * (MI + 1) + it - 1] = (double)selected;
+        hist[4 * (MI + 1) + it - 1] = value;
+      }
+      if (feasible_now()) {
+        if (tid <= d) S.last[tid] = S.theta[tid];
+        feasible_it = it;
+      }
+      __syncthreads();
+    }
+  }
+
+  // Final partition (art/_cuda_batched_optimizer.py:413-430).
+  double* hard = a->hard + (size_t)prob * 8;
+  int fallback = 0;
+  const int finished = !error;  // the final partition stage runs
+  if (!error) {
+    const double right = S.packet[3];
+    fallback = a->mode == 0 && a->fallback && feasible_it >= 0 &&
+               fmin(right, n - right) < required;
+    if (fallback && tid <= d) S.theta[tid] = S.last[tid];
+    __syncthreads();
+    if (tid == 0) {
+      double nw = 0.0;
+      for (int j = 0; j < d; ++j) nw += S.theta[j] * S.theta[j];
+      nw = sqrt(nw);
+      for (int j = 0; j < d; ++j) S.theta[j] /= nw;
+    }
+    __syncthreads();
+    if (a->hard_refit) fit_sides<T>(C, S, true, 0.0);
+    hard_stats<T>(C, S, a->hard_refit != 0, hard);
+    if (tid == 0) {
+      const double right2 = hard[3];
+      if (!a->hard_refit && hard[6] != 0.0) error = kNonfiniteSplit;
+      else if (fmin(right2, n - right2) < required) error = kMinSidePoints;
+      else if (hard[6] != 0.0) error = kCholeskyFailure;
+      sflag[0] = error;
+    }
+    __syncthreads();
+    error = sflag[0];
+  }
+  if (tid <= d) a->theta_out[(size_t)prob * (d + 1) + tid] = S.theta[tid];
+  if (tid < p) {
+    a->beta_out[((size_t)prob * 2 + 0) * p + tid] = S.beta[0][tid];
+    a->beta_out[((size_t)prob * 2 + 1) * p + tid] = S.beta[1][tid];
+  }
+  if (tid == 0) {
+    hard[7] = fallback;
+    int* info = a->info + (size_t)prob * 8;
+    info[0] = error ? error : reason;
+    info[1] = converged;
+    info[2] = it;
+    info[3] = fallback ? feasible_it : -1;
+    info[4] = (int)physical;
+    info[5] = (int)logical;
+    info[6] = error != 0;
+    info[7] = finished;
+    double* sc = a->scalars + (size_t)prob * kSplitScalars;
+    sc[2] = a->adaptive ? alpha_ctrl : a->alpha0;
+    sc[3] = C.T;
+    sc[4] = S.packet[4];  // ridge_effective of the last soft fit, left/right
+    sc[5] = S.packet[5];
+  }
+}
+
+}  // namespace art
+
+using namespace art;
+
+extern "C" int art_split_optimize(const art_split_args* args, int problems, hipStream_t stream) {
+  if (!args || problems < 1) return fail(ART_ERR_ARG, "art_split_optimize: bad arguments");
+  const int p = args->basis.p;
+  const int d = args->d;
+  if (p < 1 || p > kSplitMaxP || d < 1 || d > kSplitMaxD)
+    return fail(ART_ERR_ARG, "art_split_optimize: requires 1 <= p <= 32 and 1 <= d <= 31");
+  if (args->n < 1 || args->H < 1 || args->max_iters < 0 || args->max_backtracks < 0 ||
+      args->max_failures < 1 || args->mode < 0 || args->mode > 2 || args->anneal_iters < 0)
+    return fail(ART_ERR_ARG, "art_split_optimize: invalid sizes or controls");
+  if (!args->X || !args->y || !args->temperature || !args->theta0 || !args->node ||
+      !args->theta_out || !args->beta_out || !args->hard || !args->info || !args->history ||
+      !args->scalars)
+    return fail(ART_ERR_ARG, "art_split_optimize: null device pointer");
+  const int T = (p + 1 + 15) / 16;
+  const dim3 grid(problems), block(256);
+  switch (T) {
+    case 1: split_optimize_kernel<1><<<grid, block, 0, stream>>>(*args); break;
+    case 2: split_optimize_kernel<2><<<grid, block, 0, stream>>>(*args); break;
+    default: split_optimize_kernel<3><<<grid, block, 0, stream>>>(*args); break;
+  }
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? ART_OK : hip_fail(e, "split_optimize_kernel launch");
+}

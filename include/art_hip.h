@@ -34,6 +34,16 @@ extern "C" {
 #define ART_MIXED_REFINE_TOL 1e-4 /* accept |delta| <= tol*|beta| after one refinement */
 #define ART_SOLVE_LDS_BYTES 65536
 
+/* Feature basis of the leaf models (art/_cuda_features.py:13-29): degree 1
+ * or 2, optional bias, bias first or (degree 1 only) last; p columns. */
+typedef struct {
+  int d;
+  int p;
+  int degree;
+  int bias;
+  int bias_last;
+} art_basis;
+
 /* Opaque resident workspace: chunk tables and partial-Gram buffers. */
 typedef struct art_ridge_plan art_ridge_plan;
 
@@ -116,6 +126,55 @@ int art_plan_timing_enable(art_ridge_plan* plan, int slots);
 /* Waits for and returns up to `max` recorded (gram_ms, solve_ms) pairs;
  * returns how many were written (or a negative error). */
 int art_plan_timing_read(art_ridge_plan* plan, float* gram_ms, float* solve_ms, int max);
+
+/* ---------------------------------------------------------------------
+ * Soft-oblique split optimizer.  One problem = one node x one start theta0.
+ * Device arrays: X (K, n, d), y (K, n), temperature (K,), theta0 (P, d+1),
+ * node (P,) int32 -> outputs theta_out (P, d+1) (the final, unit-w split),
+ * beta_out (P, 2, p) (hard-refit left/right coefficients), hard (P, 8)
+ * [MSE, MSE_L, MSE_R, n_right, lambda_L, lambda_R, bad, min_side_fallback],
+ * info (P, 8) int32 [stop or error code, converged, n_iters,
+ * fallback_iteration, physical candidates, logical candidates, error,
+ * final partition stage ran],
+ * history (P, 5, max_iters+1) [soft loss, |projected grad|, accepted step,
+ * backtracks, line-search loss], scalars (P, 6) [initial |grad|, gradient
+ * tolerance, final alpha, temperature, lambda_L and lambda_R of the last
+ * soft fit].  H is the logical backtrack batch (candidates examined per
+ * batch before the earliest Armijo pass is taken).
+ * Codes: 0 max_iters, 1 gradient_tolerance, 2 non_descent_direction,
+ * 3 line_search_failed, 4 max_line_search_failures, 5 alpha_min_reached,
+ * 6 cholesky_failure, 7 min_side_points, 8 nonfinite_split.
+ * mode 0: Armijo + AdaptiveAlpha (the reference loop); mode 1: annealed
+ * fixed step (anneal_iters iterations, T geometric from t0 to t1, step);
+ * mode 2: parent fit only (unweighted fit of all rows; hard[0] = its MSE). */
+typedef struct {
+  const double* X;
+  const double* y;
+  const double* temperature;
+  const double* theta0;
+  const int* node;
+  int n;
+  int d;
+  art_basis basis;
+  double ridge, floor, grad_atol, grad_rtol, alpha0, rho, armijo_c, alpha_min, alpha_max, grow,
+      recovery;
+  int max_iters, max_backtracks, heavy, max_failures, adaptive, required, fallback, hard_refit, H,
+      mode, anneal_iters;
+  double t0, t1, step;
+  double* theta_out;
+  double* beta_out;
+  double* hard;
+  int* info;
+  double* history;
+  double* scalars;
+} art_split_args;
+
+/* Runs every problem of a cohort to completion in one launch (p <= 32,
+ * d <= 31, H >= 1).  Replaces the host-driven loop of
+ * BatchedOptimizerWorkspace._run/search/advance/_finish
+ * (art/_cuda_batched_optimizer.py:331-526) and, for one node, the CPU
+ * SoftObliqueSplitter._optimize_from_initialization (art/splitters.py:365-690). */
+int art_split_optimize(const art_split_args* args, int problems, hipStream_t stream);
 
 /* Frees a plan's device buffers (synchronizes the device).
  * Replaces BatchedCholeskyWorkspace.close (art/_cuda_batched_cholesky.py:339-352). */

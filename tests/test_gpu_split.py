@@ -1,0 +1,318 @@
+"""GPU parity of the device-controlled split optimizer vs the oracle.
+
+Cases and tolerances follow tests/test_cuda_batched_optimizer.py of the
+reference: theta rtol 1e-9 / atol 1e-10, soft-loss history rtol 1e-9,
+accepted steps rtol 1e-13, backtrack counts and stop reasons exact, hard-refit
+coefficients rtol 1e-10 / atol 1e-12 against an FP64 weighted ridge solve of
+the final partition.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import ridge as R
+from oracle import split as O
+
+pytestmark = pytest.mark.gpu
+
+
+def problems():
+    """tests/test_cuda_batched_optimizer.py:19-25 (seed 801)."""
+    rng = np.random.default_rng(801)
+    X = rng.uniform(-1, 1, (3, 43, 2))
+    y = np.where(X[:, :, 0] > .1, 1. + X[:, :, 1], -.5 + .3 * X[:, :, 1])
+    y[0] = 0.
+    theta = np.tile([1., 0., 0.], (3, 1))
+    return X, y, theta
+
+
+def settings_of(s, T):
+    """Oracle settings equal to a SoftObliqueSplitter's."""
+    from art import AffineRidgeModel
+
+    m = s.model_template
+    affine = isinstance(m, AffineRidgeModel)
+    return O.Settings(
+        temperature=T, ridge=m.ridge, weight_floor=s.weight_floor, max_iters=s.max_iters,
+        grad_atol=s.grad_atol, grad_rtol=s.grad_rtol, min_side_points=s.min_side_points,
+        min_side_fraction=s.min_side_fraction, min_side_fallback=s.min_side_fallback,
+        alpha0=s.alpha0, rho=s.rho, armijo_c=s.armijo_c, max_backtracks=s.max_backtracks,
+        adaptive_alpha=s.adaptive_alpha, alpha_min=s.alpha_min, alpha_max=s.alpha_max,
+        alpha_grow=s.alpha_grow, alpha_recovery=s.alpha_recovery,
+        heavy_backtrack_threshold=s.heavy_backtrack_threshold,
+        max_line_search_failures=s.max_line_search_failures,
+        degree=1 if affine else m.degree, bias_last=affine)
+
+
+def check_against_oracle(result, s, X, y, theta0, T):
+    o = O.optimize(X, y, theta0, settings_of(s, T))
+    np.testing.assert_allclose(np.r_[result.w, result.z], O.project_unit_w(o["theta"]),
+                               rtol=1e-9, atol=1e-10)
+    np.testing.assert_allclose(result.metadata["soft_loss_history"], o["soft_loss_history"],
+                               rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(result.metadata["step_size_history"], o["step_size_history"],
+                               rtol=1e-13)
+    assert result.metadata["backtrack_history"] == o["backtrack_history"]
+    assert result.stop_reason == o["stop_reason"]
+    assert result.n_iters == len(o["step_size_history"])
+    assert result.metadata["logical_line_search_candidates"] == o["logical_candidates"]
+    return o
+
+
+def check_hard_models(result, s, X, y):
+    if result.left_model is None:
+        return
+    so = settings_of(s, 1.0)
+    phi = O.design(X, so)
+    left = X @ result.w < result.z
+    for side, q in enumerate((left.astype(float), (~left).astype(float))):
+        ref, _ = R.solve_weighted_ridge(phi, y, q, s.model_template.ridge, 0.0)
+        fitted = result.left_model if side == 0 else result.right_model
+        np.testing.assert_allclose(fitted.coef_, ref, rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("degree,adaptive,H", [(1, False, 8), (1, True, 1), (2, True, 3),
+                                               (2, False, 11)])
+def test_nodes_match_independent_oracle_runs(torch_gpu, degree, adaptive, H):
+    from art import (AffineRidgeModel, BatchedOptimizerWorkspace, PolynomialRidgeModel,
+                     SoftObliqueSplitter, SplitNotFoundError)
+
+    X, y, theta = problems()
+    model = (AffineRidgeModel(1e-4, solver="normal") if degree == 1
+             else PolynomialRidgeModel(ridge=1e-4, solver="normal"))
+    s = SoftObliqueSplitter(model, .7, max_iters=4, alpha0=2., adaptive_alpha=adaptive,
+                            min_side_points=2, backtrack_batch_size=H)
+    temp = np.array([.3, .7, 1.2])
+    for reorder in (False, True):
+        with BatchedOptimizerWorkspace(s, X, y, temperature=temp, reorder=reorder) as ws:
+            results = ws.optimize(theta)
+            assert all(not isinstance(r, SplitNotFoundError) for r in results), results
+            for j, result in enumerate(results):
+                check_against_oracle(result, s, X[j], y[j], theta[j], temp[j])
+                check_hard_models(result, s, X[j], y[j])
+                md = result.metadata
+                assert md["physical_line_search_candidates"] >= md["logical_line_search_candidates"]
+                assert md["backtrack_batch_size"] == min(H, s.max_backtracks + 1)
+            assert results[0].n_iters == 0
+            assert results[0].stop_reason == "gradient_tolerance"
+            parent = [O.parent_loss(X[j], y[j], settings_of(s, 1.)) for j in range(3)]
+            np.testing.assert_allclose([r.parent_loss for r in results], parent,
+                                       rtol=1e-9, atol=1e-13)
+
+
+def test_restarts_and_numerical_failures_are_isolated(torch_gpu):
+    from art import (AffineRidgeModel, BatchedOptimizerWorkspace, SoftObliqueSplitter,
+                     SplitNotFoundError)
+
+    X, y, theta = problems()
+    X[1, :, 1] = 0.  # singular design without ridge
+    s = SoftObliqueSplitter(AffineRidgeModel(0., solver="normal"), .7, max_iters=2,
+                            n_restarts=2, min_side_points=2)
+    initial = np.stack((theta, np.tile([0., 1., 0.], (3, 1))))
+    with BatchedOptimizerWorkspace(s, X, y, reorder=True) as ws:
+        result = ws.optimize(initial)
+        assert isinstance(result[1], SplitNotFoundError)
+        assert result[1].reason == "cholesky_failure"
+        assert all(not isinstance(result[j], SplitNotFoundError) for j in (0, 2))
+        # The kept restart is the one with the lower hard loss.
+        for j in (0, 2):
+            losses = []
+            for r in range(2):
+                o = O.optimize(X[j], y[j], initial[r, j], settings_of(s, .7))
+                losses.append(O.hard_result(X[j], y[j], o["theta"], settings_of(s, .7))["loss"])
+            assert result[j].loss == pytest.approx(min(losses), rel=1e-9, abs=1e-13)
+        again = ws.optimize(initial)
+        np.testing.assert_allclose(result[2].predict(X[2]), again[2].predict(X[2]), atol=1e-12)
+    with pytest.raises(RuntimeError, match="closed"):
+        ws.optimize(initial)
+
+
+@pytest.mark.parametrize("adaptive", [False, True])
+def test_line_search_exhaustion_preserves_theta_and_counts_retries(torch_gpu, adaptive):
+    from art import AffineRidgeModel, BatchedOptimizerWorkspace, SoftObliqueSplitter
+
+    X, y, theta = problems()
+    s = SoftObliqueSplitter(AffineRidgeModel(1e-4, solver="normal"), .7, max_iters=2,
+                            alpha0=1e12, alpha_max=1e12, max_backtracks=0,
+                            max_line_search_failures=2, adaptive_alpha=adaptive)
+    with BatchedOptimizerWorkspace(s, X, y) as ws:
+        result = ws.optimize(theta)
+        for j in (1, 2):
+            check_against_oracle(result[j], s, X[j], y[j], theta[j], .7)
+            assert result[j].n_iters == 0
+            np.testing.assert_allclose(np.r_[result[j].w, result[j].z], theta[j], atol=1e-14)
+            assert result[j].metadata["physical_line_search_candidates"] == (2 if adaptive else 1)
+
+
+@pytest.mark.parametrize("hard_refit", [False, True])
+def test_last_feasible_parameters_are_restored_before_hard_refits(torch_gpu, hard_refit):
+    from art import AffineRidgeModel, BatchedOptimizerWorkspace, SoftObliqueSplitter
+
+    data = []
+    for seed in (2, 3):
+        rng = np.random.default_rng(seed)
+        x = rng.uniform(-1, 1, (60, 2))
+        data.append((x, (x[:, 0] > .55).astype(float)))
+    X, y = np.stack([v[0] for v in data]), np.stack([v[1] for v in data])
+    theta = np.tile([1., 0., 0.], (2, 1))
+    s = SoftObliqueSplitter(AffineRidgeModel(1e-4, solver="normal"), .25, max_iters=6,
+                            alpha0=5., min_side_points=28, min_side_fallback=False)
+    with BatchedOptimizerWorkspace(s, X, y) as ws:
+        results = ws.optimize(theta, min_side_fallback=True, hard_refit=hard_refit)
+        assert not s.min_side_fallback
+        so = settings_of(s, .25)
+        so.min_side_fallback = True
+        for j, result in enumerate(results):
+            o = O.optimize(X[j], y[j], theta[j], so)
+            assert o["min_side_fallback"]
+            np.testing.assert_allclose(np.r_[result.w, result.z], O.project_unit_w(o["theta"]),
+                                       rtol=1e-8, atol=1e-10)
+            assert result.metadata["fallback_iteration"] == o["fallback_iteration"]
+            assert result.metadata["min_side_fallback"]
+            assert min(result.n_left, result.n_right) >= 28
+            assert result.metadata["hard_refit"] is hard_refit
+            assert (result.left_model is None) == (not hard_refit)
+
+
+def test_failed_attempts_can_retry_and_then_accept_independently(torch_gpu):
+    from art import (AffineRidgeModel, BatchedOptimizerWorkspace, SoftObliqueSplitter,
+                     SplitNotFoundError)
+
+    X, y, theta = problems()
+    s = SoftObliqueSplitter(AffineRidgeModel(1e-4, solver="normal"), .7, max_iters=2,
+                            alpha0=1000., armijo_c=.8, max_backtracks=0,
+                            max_line_search_failures=20)
+    with BatchedOptimizerWorkspace(s, X, y) as ws:
+        results = ws.optimize(theta)
+        for j in (1, 2):
+            assert not isinstance(results[j], SplitNotFoundError), results[j]
+            check_against_oracle(results[j], s, X[j], y[j], theta[j], .7)
+            assert results[j].n_iters == 2
+            assert results[j].metadata["physical_line_search_candidates"] > 2
+
+
+def test_single_slot_and_cached_parent_loss(torch_gpu):
+    from art import AffineRidgeModel, BatchedOptimizerWorkspace, SoftObliqueSplitter
+
+    X, y, theta = problems()
+    s = SoftObliqueSplitter(AffineRidgeModel(1e-4, solver="normal"), 1., max_iters=0)
+    with BatchedOptimizerWorkspace(s, X[2:], y[2:]) as ws:
+        result = ws.optimize(theta[2:], parent_loss=np.array([.125]))[0]
+        assert ws.fit_sizes == [("soft", 1), ("hard", 1)]
+        assert result.parent_loss == .125
+        assert result.split_gain == pytest.approx(.125 - result.loss)
+        assert result.n_iters == 0 and result.stop_reason == "max_iters"
+        np.testing.assert_allclose(np.r_[result.w, result.z], theta[2])
+        h = O.hard_result(X[2], y[2], theta[2], settings_of(s, 1.))
+        assert result.loss == pytest.approx(h["loss"], rel=1e-10)
+        assert (result.n_left, result.n_right) == (h["n_left"], h["n_right"])
+
+
+@pytest.mark.parametrize("degree", [1, 2])
+def test_partition_export_skips_singular_hard_refit(torch_gpu, degree):
+    from art import (AffineRidgeModel, BatchedOptimizerWorkspace, PolynomialRidgeModel,
+                     SoftObliqueSplitter, SplitNotFoundError)
+
+    rng = np.random.default_rng(912)
+    X = rng.uniform(-.5, .5, (2, 20, 2))
+    X[:, -1] = [2., 3.]  # a one-point right partition has a singular hard Gram
+    y = np.sin(X[:, :, 0]) + X[:, :, 1] ** 2
+    model = AffineRidgeModel(ridge=0.) if degree == 1 else PolynomialRidgeModel(ridge=0.)
+    s = SoftObliqueSplitter(model, 1., max_iters=0)
+    theta = np.tile([1., 0., 1.], (2, 1))
+    parent = np.array([O.parent_loss(x, z, settings_of(s, 1.)) for x, z in zip(X, y)])
+    with BatchedOptimizerWorkspace(s, X, y) as ws:
+        fitted = ws.optimize(theta, parent_loss=parent)
+        assert all(isinstance(r, SplitNotFoundError) and r.reason == "cholesky_failure"
+                   for r in fitted)
+        results = ws.optimize(theta, parent_loss=parent, hard_refit=False)
+        assert ws.fit_sizes == [("soft", 2)]
+        for result in results:
+            assert not isinstance(result, SplitNotFoundError), result
+            assert result.n_left == 19 and result.n_right == 1
+            assert result.left_model is result.right_model is None
+            assert result.loss is result.split_gain is result.relative_split_gain is None
+            assert result.metadata["hard_refit"] is False
+            np.testing.assert_array_equal(np.r_[result.w, result.z], theta[0])
+
+
+@pytest.mark.parametrize("reorder", [False, True])
+def test_capacity_loads_match_independent_runs(torch_gpu, reorder):
+    from art import (BatchedOptimizerWorkspace, PolynomialRidgeModel, SoftObliqueSplitter,
+                     SplitNotFoundError)
+
+    X, y, theta = problems()
+    s = SoftObliqueSplitter(PolynomialRidgeModel(ridge=1e-4), .7, max_iters=2,
+                            min_side_points=2, alpha0=.5, max_backtracks=4)
+    with BatchedOptimizerWorkspace.allocate(s, 5, X.shape[1], X.shape[2], reorder=reorder) as ws:
+        with pytest.raises(RuntimeError, match="Load"):
+            ws.optimize()
+        previous = None
+        for batch, k in enumerate((3, 1, 2)):
+            take = [2, 1, 0][:k]
+            bX, by = X[take] + .07 * batch, y[take] + .2 * batch
+            ws.host_X[:k], ws.host_y[:k] = bX, by
+            temps = np.linspace(.5, 1., k)
+            ws.load(ws.host_X[:k], ws.host_y[:k], temperature=temps)
+            parent = None
+            if batch == 1:
+                parent = np.array([O.parent_loss(a, b, settings_of(s, 1.)) for a, b in zip(bX, by)])
+            results = ws.optimize(theta[take], parent_loss=parent)
+            assert len(results) == k and ws.K == 5 and ws.k == k
+            assert all(count <= k for _, count in ws.fit_sizes)
+            if not reorder:
+                assert all(count == k for _, count in ws.fit_sizes)
+            assert any(phase == "parent" for phase, _ in ws.fit_sizes) == (parent is None)
+            for j, result in enumerate(results):
+                assert not isinstance(result, SplitNotFoundError), result
+                check_against_oracle(result, s, bX[j], by[j], theta[take][j], temps[j])
+            if previous is not None:
+                old, predictions = previous
+                np.testing.assert_array_equal(old.predict(X[0]), predictions)
+            previous = results[0], results[0].predict(X[0]).copy()
+    with pytest.raises(RuntimeError, match="closed"):
+        ws.load(X, y)
+
+
+def test_selected_retry_and_random_starts(torch_gpu):
+    from art import (AffineRidgeModel, BatchedOptimizerWorkspace, SoftObliqueSplitter,
+                     SplitNotFoundError)
+
+    X, y, theta = problems()
+    s = SoftObliqueSplitter(AffineRidgeModel(1e-4), .7, max_iters=1, min_side_points=2,
+                            min_side_fallback=True, random_state=28)
+    with BatchedOptimizerWorkspace.allocate(s, 5, X.shape[1], X.shape[2]) as ws:
+        ws.load(X, y)
+        first = ws.optimize(theta)
+        retry = ws.optimize(theta[[2, 1]], nodes=[2, 1], min_side_fallback=False)
+        assert s.min_side_fallback
+        assert all(count <= 2 for _, count in ws.fit_sizes)
+        for result, j in zip(retry, (2, 1)):
+            assert not isinstance(result, SplitNotFoundError), result
+            np.testing.assert_allclose(np.r_[result.w, result.z], np.r_[first[j].w, first[j].z],
+                                       rtol=1e-9, atol=1e-10)
+        assert ws.optimize(nodes=[]) == []
+        for ids in ([3], [1, 1], [-1], [1.2]):
+            with pytest.raises(ValueError, match="nodes"):
+                ws.optimize(nodes=ids)
+        fresh = ws.optimize(nodes=[2], parent_loss=np.array([first[2].parent_loss]))[0]
+        assert fresh.metadata["initialization_seed"] is not None
+        # A random start still satisfies the oracle's loop from that start.
+        w = fresh.metadata["initial_theta"]
+        check_against_oracle(fresh, s, X[2], y[2], w, .7)
+
+
+def test_splitter_split_matches_oracle_for_one_node(torch_gpu):
+    from art import PolynomialRidgeModel, SoftObliqueSplitter
+
+    rng = np.random.default_rng(5)
+    X = rng.uniform(-1.0, 1.0, size=(100, 2))
+    y = np.where(X[:, 0] >= 0.0, 0.8 - 0.4 * X[:, 0] + 1.2 * X[:, 1],
+                 -0.5 + 1.1 * X[:, 0] - 0.3 * X[:, 1])
+    s = SoftObliqueSplitter(PolynomialRidgeModel(ridge=1e-6), .15, max_iters=30,
+                            min_side_points=2, optimizer="cupy_batched_cholesky")
+    theta0 = np.array([.9, .3, .05])
+    result = s.split(X, y, theta0=theta0)
+    check_against_oracle(result, s, X, y, theta0, .15)
+    check_hard_models(result, s, X, y)
+    assert result.split_gain > 0
