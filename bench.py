@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--leaves", type=int, default=4096)
     ap.add_argument("--chunk-rows", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after timing, run a plain 1 GiB device copy three times: the "
+                         "calibration kernel for FETCH_SIZE / WRITE_SIZE counter runs")
     return ap.parse_args()
 
 
@@ -98,6 +101,26 @@ def timed_fits(batch, lib, X, y, w, ridge, precision, steps, warmup):
             "first_kernel_ms": float(np.mean(first[:n])), "rest_ms": float(np.mean(rest[:n]))}
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (profiles/hbm_traffic.json: FETCH_SIZE + WRITE_SIZE per dispatch, each
+    scaled by the factor a plain copy of known size reads), or None."""
+    path = ROOT / "profiles" / "hbm_traffic.json"
+    if not path.exists():
+        return None
+    entry = json.loads(path.read_text()).get("kernels", {}).get(kernel)
+    return None if entry is None else float(entry["bytes_per_launch"])
+
+
+def copy_probe(torch, nbytes=1 << 30):
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"[bench] copy probe: 3 copies of {nbytes} bytes", file=sys.stderr, flush=True)
+
+
 def rel_err(a, b):
     return np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-300)
 
@@ -148,8 +171,9 @@ def main():
         traffic_bytes = N * (8 + 2) * 8
         achieved = traffic_bytes / (kernel_ms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                    "kernel": "gram_leaf_mixed_kernel<3>", "algorithmic_bytes_per_launch": traffic_bytes,
+                    "frac": achieved / HBM_PEAK_GBS,
+                    "traffic": measured_traffic("gram_leaf_mixed_kernel<3, 8>"),
+                    "kernel": "gram_leaf_mixed_kernel<3, 8>", "algorithmic_bytes_per_launch": traffic_bytes,
                     "kernel_ms": kernel_ms}
     else:
         # FP64 Gram on MFMA: symmetric entries p(p+1)/2 plus p right-hand-side
@@ -157,7 +181,8 @@ def main():
         flops = (p * (p + 1) + 2.0 * p) * N
         achieved = flops / (kernel_ms * 1e-3) / 1e12
         roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS["fp64"],
-                    "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["fp64"], "traffic": None,
+                    "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["fp64"],
+                    "traffic": measured_traffic("gram_leaf_f64_kernel<3>"),
                     "kernel": "gram_leaf_f64_kernel<3>", "algorithmic_flops_per_launch": flops,
                     "kernel_ms": kernel_ms}
     result = {
@@ -202,6 +227,8 @@ def main():
         cerr = rel_err(got[both], cref[both])
         result["rel_err_vs_cpu_fp64"] = {"max": float(cerr.max()),
                                          "p99": float(np.percentile(cerr, 99))}
+    if args.copy_probe:
+        copy_probe(torch)
     batch.close()
     print(json.dumps(result))
 
