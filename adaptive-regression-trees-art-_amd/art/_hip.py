@@ -58,6 +58,18 @@ class SplitArgs(ctypes.Structure):
     ]
 
 
+class HarArgs(ctypes.Structure):
+    """art_har_args of include/art_hip.h."""
+
+    _fields_ = [
+        *[(name, ctypes.c_void_p) for name in ("A", "b", "row_offsets", "x0", "shape",
+                                               "directions", "uniforms", "samples")],
+        ("seed", ctypes.c_uint64),
+        *[(name, ctypes.c_int) for name in ("P", "d", "chains", "n", "first", "stride",
+                                            "steps")],
+    ]
+
+
 _PROTOTYPES = {
     "art_last_error": (ctypes.c_char_p, []),
     "art_version": (_i, []),
@@ -75,6 +87,8 @@ _PROTOTYPES = {
     "art_plan_timing_read": (_i, [_p, _p, _p, _i]),
     "art_plan_destroy": (None, [_p]),
     "art_split_optimize": (_i, [ctypes.POINTER(SplitArgs), _i, _p]),
+    "art_har_sample": (_i, [ctypes.POINTER(HarArgs), _p]),
+    "art_philox4x32_10": (_i, [_p, _p, _i64, _p, _p]),
 }
 
 

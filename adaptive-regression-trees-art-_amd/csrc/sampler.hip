@@ -1,0 +1,225 @@
+// Hit-and-run sampling of many convex polytopes {x : A x <= b} (row a7).
+//
+// One lane per chain, one 64-lane workgroup per (polytope, block of 64
+// chains).  Per move a lane draws a direction, recomputes every slack
+// s_i = b_i - A_i.x exactly in FP64 together with A_i.d (the constraint rows
+// are uniform across the wave and arrive through scalar loads), takes the
+// chord [max_{A_i.d<0} s_i/A_i.d, min_{A_i.d>0} s_i/A_i.d] and moves to
+// t = t_lo + u (t_hi - t_lo) (art/sampling.py:877-962,
+// art/_numba_sampling.py:9-40).  Recomputing the slack instead of updating it
+// incrementally keeps every recorded sample within rounding of A x <= b for
+// any chain length and needs no per-chain slack storage.
+//
+// Schedule (one rule for the reference's batched and scalar samplers): chain
+// c records after moves first, first + stride, ..., K_c records in all, with
+// K_c = n / C + (c < n % C); record k of chain c is sample k*C + c
+// (art/sampling.py:745-803 batched: first = burn_in + stride;
+// art/sampling.py:480-494 scalar: one chain, first = burn_in + 1).
+//
+// Random stream (no reference counterpart; the reference draws host NumPy
+// arrays): Philox4x32-10 (Salmon et al., SC'11) with key (seed_lo, seed_hi)
+// and counter (move * W + w, chain, polytope, 0); FP32 Box-Muller directions
+// and a 53-bit FP64 chord uniform.  A fixed-stream mode reads directions and
+// uniforms from device arrays instead, for parity with the reference's
+// NumPy-driven samplers.
+
+#include "art_common.h"
+#include "art_hip.h"
+
+namespace art {
+
+__device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint32_t lo0 = 0xD2511F53u * c.x;
+    const uint32_t hi0 = __umulhi(0xD2511F53u, c.x);
+    const uint32_t lo1 = 0xCD9E8D57u * c.z;
+    const uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z);
+    c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+  }
+  return c;
+}
+
+__host__ __device__ constexpr int har_calls_per_step(int d) {
+  return (2 * ((d + 1) / 2) + 2 + 3) / 4;
+}
+
+__global__ __launch_bounds__(256) void philox_kernel(const uint4* __restrict__ ctr,
+                                                     const uint2* __restrict__ key, int64_t count,
+                                                     uint4* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) out[i] = philox4x32_10(ctr[i], key[i].x, key[i].y);
+}
+
+template <int DM, bool kFixed>
+__global__ __launch_bounds__(64) void har_kernel(art_har_args a) {
+  constexpr int WM = har_calls_per_step(DM);
+  const int d = a.d;
+  const int C = a.chains;
+  const int wpp = (C + 63) / 64;
+  const int poly = blockIdx.x / wpp;
+  const int c = (blockIdx.x - poly * wpp) * 64 + (int)threadIdx.x;
+  if (c >= C) return;
+  const int64_t r0 = a.row_offsets[poly];
+  const int m = (int)(a.row_offsets[poly + 1] - r0);
+  const double* __restrict__ A = a.A + r0 * d;
+  const double* __restrict__ b = a.b + r0;
+  const int full = a.n / C, rem = a.n % C;
+  const int records = full + (c < rem ? 1 : 0);
+  const int moves = a.first + (records - 1) * a.stride;
+  const int W = har_calls_per_step(d);
+  const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
+
+  double x[DM];
+  const double* x0 = a.x0 + ((size_t)poly * C + c) * d;
+#pragma unroll
+  for (int j = 0; j < DM; ++j) x[j] = j < d ? x0[j] : 0.0;
+  double* out = a.samples + (size_t)poly * a.n * d;
+
+  for (int s = 0; s < moves; ++s) {
+    double dir[DM];
+    double u;
+    if constexpr (kFixed) {
+      const size_t row = ((size_t)poly * a.steps + s) * C + c;
+      const double* dp = a.directions + row * d;
+#pragma unroll
+      for (int j = 0; j < DM; ++j) dir[j] = j < d ? dp[j] : 0.0;
+      u = a.uniforms[row];
+    } else {
+      uint32_t w[4 * WM];
+#pragma unroll
+      for (int q = 0; q < WM; ++q) {
+        uint4 r = make_uint4(0u, 0u, 0u, 0u);
+        if (q < W)
+          r = philox4x32_10(make_uint4((uint32_t)(s * W + q), (uint32_t)c, (uint32_t)poly, 0u), k0,
+                            k1);
+        w[4 * q] = r.x;
+        w[4 * q + 1] = r.y;
+        w[4 * q + 2] = r.z;
+        w[4 * q + 3] = r.w;
+      }
+      // Box-Muller pairs from words (2q, 2q+1); u1 in (0, 1] keeps log finite.
+#pragma unroll
+      for (int q = 0; q < (DM + 1) / 2; ++q) {
+        const float u1 = (float)((w[2 * q] >> 8) + 1u) * 0x1p-24f;
+        const float u2 = (float)(w[2 * q + 1] >> 8) * 0x1p-24f;
+        const float r = sqrtf(-2.0f * logf(u1));
+        float sn, cs;
+        sincospif(2.0f * u2, &sn, &cs);
+        dir[2 * q] = (double)(r * cs);
+        if (2 * q + 1 < DM) dir[2 * q + 1] = (double)(r * sn);
+      }
+      // The chord uniform follows the ceil(d/2) pairs.
+      const int P = (d + 1) / 2;
+      uint32_t wa = 0u, wb = 0u;
+#pragma unroll
+      for (int q = 0; q < (DM + 1) / 2; ++q)
+        if (q == P - 1) {
+          wa = w[2 * q + 2];
+          wb = w[2 * q + 3];
+        }
+      u = ((double)(wa >> 5) * 67108864.0 + (double)(wb >> 6)) * 0x1p-53;
+#pragma unroll
+      for (int j = 0; j < DM; ++j)
+        if (j >= d) dir[j] = 0.0;
+    }
+    if (a.shape) {
+      double t[DM];
+#pragma unroll
+      for (int i = 0; i < DM; ++i) {
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < DM; ++j)
+          if (i < d && j < d) acc = fma(a.shape[i * d + j], dir[j], acc);
+        t[i] = acc;
+      }
+#pragma unroll
+      for (int i = 0; i < DM; ++i) dir[i] = t[i];
+    }
+    double tlo = -INFINITY, thi = INFINITY;
+    for (int i = 0; i < m; ++i) {
+      const double* Ai = A + (size_t)i * d;
+      double ad = 0.0, ax = 0.0;
+#pragma unroll
+      for (int j = 0; j < DM; ++j)
+        if (j < d) {
+          const double aij = Ai[j];
+          ad = fma(aij, dir[j], ad);
+          ax = fma(aij, x[j], ax);
+        }
+      const double ratio = (b[i] - ax) / ad;
+      if (ad < 0.0) tlo = fmax(tlo, ratio);
+      else if (ad > 0.0) thi = fmin(thi, ratio);
+    }
+    const double t = (thi - tlo) * u + tlo;
+#pragma unroll
+    for (int j = 0; j < DM; ++j) x[j] += dir[j] * t;
+    const int done = s + 1;
+    if (done >= a.first && (done - a.first) % a.stride == 0) {
+      double* o = out + ((size_t)((done - a.first) / a.stride) * C + c) * d;
+#pragma unroll
+      for (int j = 0; j < DM; ++j)
+        if (j < d) o[j] = x[j];
+    }
+  }
+}
+
+template <int DM>
+void launch_har(const art_har_args& a, hipStream_t stream) {
+  const int blocks = a.P * ((a.chains + 63) / 64);
+  if (a.directions)
+    har_kernel<DM, true><<<blocks, 64, 0, stream>>>(a);
+  else
+    har_kernel<DM, false><<<blocks, 64, 0, stream>>>(a);
+}
+
+}  // namespace art
+
+using namespace art;
+
+extern "C" int art_philox4x32_10(const uint32_t* counters, const uint32_t* keys, int64_t count,
+                                 uint32_t* out, hipStream_t stream) {
+  if (count < 0 || (count > 0 && (!counters || !keys || !out)))
+    return fail(ART_ERR_ARG, "art_philox4x32_10: bad arguments");
+  if (count == 0) return ART_OK;
+  const int64_t blocks = (count + 255) / 256;
+  philox_kernel<<<(unsigned)blocks, 256, 0, stream>>>((const uint4*)counters, (const uint2*)keys,
+                                                      count, (uint4*)out);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? ART_OK : hip_fail(e, "philox_kernel launch");
+}
+
+extern "C" int art_har_sample(const art_har_args* a, hipStream_t stream) {
+  if (!a) return fail(ART_ERR_ARG, "art_har_sample: null arguments");
+  if (a->P < 1 || a->d < 1 || a->d > 32 || a->chains < 1 || a->n < a->chains || a->first < 1 ||
+      a->stride < 1)
+    return fail(ART_ERR_ARG,
+                "art_har_sample: need P >= 1, 1 <= d <= 32, 1 <= chains <= n, first >= 1, "
+                "stride >= 1");
+  if (!a->A || !a->b || !a->row_offsets || !a->x0 || !a->samples)
+    return fail(ART_ERR_ARG, "art_har_sample: null device pointer");
+  if ((a->directions == nullptr) != (a->uniforms == nullptr))
+    return fail(ART_ERR_ARG, "art_har_sample: directions and uniforms go together");
+  const int full = a->n / a->chains, rem = a->n % a->chains;
+  const int64_t moves = a->first + (int64_t)(full + (rem ? 1 : 0) - 1) * a->stride;
+  if (moves > (int64_t)0x7fffffff / har_calls_per_step(32))
+    return fail(ART_ERR_ARG, "art_har_sample: too many moves per chain");
+  if (a->directions && a->steps < moves)
+    return fail(ART_ERR_ARG, "art_har_sample: fixed stream shorter than the longest chain");
+  if ((int64_t)a->P * ((a->chains + 63) / 64) > 0x7fffffff)
+    return fail(ART_ERR_ARG, "art_har_sample: grid too large");
+  const int d = a->d;
+  if (d <= 2) launch_har<2>(*a, stream);
+  else if (d <= 4) launch_har<4>(*a, stream);
+  else if (d <= 8) launch_har<8>(*a, stream);
+  else if (d <= 12) launch_har<12>(*a, stream);
+  else if (d <= 16) launch_har<16>(*a, stream);
+  else if (d <= 24) launch_har<24>(*a, stream);
+  else launch_har<32>(*a, stream);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? ART_OK : hip_fail(e, "har_kernel launch");
+}

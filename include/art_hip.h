@@ -180,6 +180,58 @@ int art_split_optimize(const art_split_args* args, int problems, hipStream_t str
  * Replaces BatchedCholeskyWorkspace.close (art/_cuda_batched_cholesky.py:339-352). */
 void art_plan_destroy(art_ridge_plan* plan);
 
+/* ---------------------------------------------------------------------
+ * Hit-and-run sampling of P convex polytopes {x : A x <= b} in d <= 32
+ * dimensions, `chains` chains per polytope, lane per chain.
+ * Device arrays: A (rows, d) and b (rows,) hold every polytope's constraints
+ * back to back, polytope q owning rows [row_offsets[q], row_offsets[q+1])
+ * (row_offsets (P+1,) int64); x0 (P, chains, d) feasible starts; shape
+ * (d, d) row-major or NULL (direction d <- shape . d, the covariance shaping
+ * V diag(sqrt(lambda)) of art/sampling.py:496-537,865-875).
+ * Schedule: chain c records after moves first, first + stride, ...;
+ * K_c = n / chains + (c < n % chains) records; record k is sample
+ * k * chains + c of samples (P, n, d).  The batched sampler is
+ * first = burn_in + max(1, thinning), stride = max(1, thinning)
+ * (art/sampling.py:618-803); the scalar sampler is chains = 1,
+ * first = burn_in + 1, stride = thinning (art/sampling.py:445-494).
+ * Random stream: directions == NULL draws Philox4x32-10 (key = seed,
+ * counter (move * W + w, chain, polytope, 0), W = ceil((2 ceil(d/2) + 2) / 4))
+ * FP32 Box-Muller directions and a 53-bit chord uniform; otherwise
+ * directions (P, steps, chains, d) and uniforms (P, steps, chains) are read
+ * as given (steps >= the longest chain's moves).
+ * Replaces BatchedHitAndRunSampler.sample / _advance_batch
+ * (art/sampling.py:618-962) and the Numba kernels sample_precomputed_chains
+ * (art/_numba_sampling.py:43-167). */
+typedef struct {
+  const double* A;
+  const double* b;
+  const int64_t* row_offsets;
+  const double* x0;
+  const double* shape;
+  const double* directions;
+  const double* uniforms;
+  double* samples;
+  uint64_t seed;
+  int P;
+  int d;
+  int chains;
+  int n;
+  int first;
+  int stride;
+  int steps;
+} art_har_args;
+
+int art_har_sample(const art_har_args* args, hipStream_t stream);
+
+/* out[i] = Philox4x32-10(counters[i], keys[i]) for i < count: counters (count, 4),
+ * keys (count, 2), out (count, 4) uint32 device arrays.  The random stream
+ * of art_har_sample is defined in terms of this function.  It replaces the
+ * cuRAND Philox generator the reference uses for its random starts
+ * (art/_cuda_batched_optimizer.py:151-156,286-297). */
+int art_philox4x32_10(const uint32_t* counters, const uint32_t* keys, int64_t count,
+                      uint32_t* out, hipStream_t stream);
+
+
 #ifdef __cplusplus
 }
 #endif
