@@ -63,7 +63,8 @@ class HarArgs(ctypes.Structure):
 
     _fields_ = [
         *[(name, ctypes.c_void_p) for name in ("A", "b", "row_offsets", "x0", "shape",
-                                               "directions", "uniforms", "samples")],
+                                               "box_lo", "box_hi", "directions", "uniforms",
+                                               "samples")],
         ("seed", ctypes.c_uint64),
         *[(name, ctypes.c_int) for name in ("P", "d", "chains", "n", "first", "stride",
                                             "steps")],

@@ -120,14 +120,42 @@ def find_feasible_point(region, bounds=None, max_tries=20000, random_state=None,
     raise RuntimeError("Could not find a feasible point inside the region.")
 
 
+def split_axis_rows(A, b, offsets):
+    """Separate rows x_j <= c and -x_j <= c from the general rows.
+
+    Returns (A_general, b_general, offsets_general, lo (P, d), hi (P, d)); lo
+    and hi hold the tightest axis bound per side (-inf / +inf where none), or
+    are None when no polytope has an axis row.
+    """
+    P, d = offsets.size - 1, A.shape[1]
+    owner = np.repeat(np.arange(P), np.diff(offsets))
+    nz = A != 0.0
+    col = np.argmax(nz, axis=1)
+    val = A[np.arange(A.shape[0]), col]
+    axis = (nz.sum(axis=1) == 1) & ((val == 1.0) | (val == -1.0))
+    if not axis.any():
+        return A, b, offsets, None, None
+    lo = np.full((P, d), -np.inf)
+    hi = np.full((P, d), np.inf)
+    up, dn = axis & (val == 1.0), axis & (val == -1.0)
+    np.minimum.at(hi, (owner[up], col[up]), b[up])
+    np.maximum.at(lo, (owner[dn], col[dn]), -b[dn])
+    keep = ~axis
+    counts = np.bincount(owner[keep], minlength=P)
+    off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    return np.ascontiguousarray(A[keep]), np.ascontiguousarray(b[keep]), off, lo, hi
+
+
 class PolytopeBatch:
     """P polytopes of one dimension d, resident on the device.
 
     ``regions``: a sequence of PolytopeRegion, or a pair (A_list, b_list).
-    Rows are stored back to back (CSR by polytope) as float64.
+    Rows are stored back to back (CSR by polytope) as float64.  With
+    ``split_box`` (default) rows +-e_j are held as per-coordinate bounds,
+    which gives the same chords for less arithmetic.
     """
 
-    def __init__(self, regions, *, stream=None):
+    def __init__(self, regions, *, stream=None, split_box=True):
         if isinstance(regions, tuple) and len(regions) == 2:
             As, bs = regions
         else:
@@ -137,26 +165,51 @@ class PolytopeBatch:
         bs = [np.asarray(b, dtype=float).reshape(-1) for b in bs]
         if not As:
             raise ValueError("At least one polytope is required.")
-        d = As[0].shape[1]
+        d = As[0].shape[1] if As[0].ndim == 2 else -1
         for a, b in zip(As, bs):
             if a.ndim != 2 or a.shape[1] != d or a.shape[0] != b.shape[0] or a.shape[0] < 1:
                 raise ValueError("Every polytope needs A (m, d) and b (m,) with one d.")
-            if not (np.isfinite(a).all() and np.isfinite(b).all()):
-                raise ValueError("A and b must be finite.")
+        rows = np.array([a.shape[0] for a in As], dtype=np.int64)
+        offsets = np.concatenate([[0], np.cumsum(rows)]).astype(np.int64)
+        self._setup(np.concatenate(As), np.concatenate(bs), offsets, stream, split_box)
+
+    @classmethod
+    def from_csr(cls, A, b, offsets, *, stream=None, split_box=True):
+        """Polytope q owns rows [offsets[q], offsets[q+1]) of A (rows, d), b (rows,)."""
+        A = np.ascontiguousarray(A, dtype=float)
+        b = np.ascontiguousarray(b, dtype=float).reshape(-1)
+        offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        if (A.ndim != 2 or offsets.ndim != 1 or offsets.size < 2 or offsets[0] != 0
+                or offsets[-1] != A.shape[0] or b.shape[0] != A.shape[0]
+                or np.any(np.diff(offsets) < 1)):
+            raise ValueError("Need A (rows, d), b (rows,) and offsets (P+1,) from 0 to rows "
+                             "with at least one row per polytope.")
+        self = cls.__new__(cls)
+        self._setup(A, b, offsets, stream, split_box)
+        return self
+
+    def _setup(self, A, b, offsets, stream, split_box):
+        if not (np.isfinite(A).all() and np.isfinite(b).all()):
+            raise ValueError("A and b must be finite.")
+        d = A.shape[1]
         if not 1 <= d <= 32:
             raise ValueError("The device sampler supports 1 <= d <= 32.")
-        self.P, self.d = len(As), d
-        self.rows = np.array([a.shape[0] for a in As], dtype=np.int64)
-        offsets = np.concatenate([[0], np.cumsum(self.rows)]).astype(np.int64)
+        self.P, self.d = offsets.size - 1, d
+        self.rows = np.diff(offsets)
+        lo = hi = None
+        if split_box:
+            A, b, offsets, lo, hi = split_axis_rows(A, b, offsets)
         torch = _hip.torch_device()
         self.torch, self.lib = torch, _hip.load()
         self.device = torch.device("cuda", torch.cuda.current_device())
         self.stream = torch.cuda.current_stream() if stream is None else stream
         with torch.cuda.stream(self.stream):
-            self.A = torch.from_numpy(np.concatenate(As)).to(self.device)
-            self.b = torch.from_numpy(np.concatenate(bs)).to(self.device)
+            self.A = torch.from_numpy(np.ascontiguousarray(A)).to(self.device)
+            self.b = torch.from_numpy(np.ascontiguousarray(b)).to(self.device)
             self.offsets = torch.from_numpy(offsets).to(self.device)
-        self.host_A, self.host_b, self.host_offsets = As, bs, offsets
+            self.box_lo = None if lo is None else torch.from_numpy(lo).to(self.device)
+            self.box_hi = None if hi is None else torch.from_numpy(hi).to(self.device)
+        self.host_offsets = offsets
         self.last_kernel_ms = None
 
     def sample(self, x0, n, *, first, stride, seed=0, shape=None, directions=None,
@@ -182,6 +235,8 @@ class PolytopeBatch:
         a = _hip.HarArgs()
         a.A, a.b, a.row_offsets = _hip.ptr(self.A), _hip.ptr(self.b), _hip.ptr(self.offsets)
         a.x0 = _hip.ptr(x0)
+        if self.box_lo is not None:
+            a.box_lo, a.box_hi = _hip.ptr(self.box_lo), _hip.ptr(self.box_hi)
         keep = [x0]
         if shape is not None:
             sh = torch.as_tensor(np.asarray(shape, dtype=float), **dev).contiguous()

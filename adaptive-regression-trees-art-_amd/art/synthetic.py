@@ -106,3 +106,52 @@ def leaf_ridge_workload(n_leaves=4096, d=8, min_rows=64, max_rows=16384,
         tau = rng.uniform(0.05, 0.5, size=g)
         w[s:e] = np.prod(sigmoid((X[s:e] @ a.T - c) / tau), axis=1)
     return LeafRidgeWorkload(X, y, w, offsets, float(ridge), degree)
+
+
+@dataclass
+class PolytopeWorkload:
+    """P polytopes: the box [-1, 1]^d (2d rows) plus k_j random cuts a.x <= b
+    with unit Gaussian a and b = a.x*_j + 0.05 + U[0, 0.5], so the seeded
+    interior point x*_j keeps slack >= 0.05 on every cut (and >= 0.05 on the
+    box, x* ~ U[-0.95, 0.95]^d).  CSR rows: A (rows, d), b (rows,),
+    offsets (P+1,); starts (P, d) = x*."""
+
+    A: np.ndarray
+    b: np.ndarray
+    offsets: np.ndarray
+    starts: np.ndarray
+
+    @property
+    def n_polytopes(self):
+        return len(self.offsets) - 1
+
+    def polytope(self, j):
+        s, e = int(self.offsets[j]), int(self.offsets[j + 1])
+        return self.A[s:e], self.b[s:e]
+
+
+def polytope_workload(n_polytopes=65536, d=12, min_cuts=4, max_cuts=40, seed=4):
+    rng = np.random.default_rng(seed)
+    P = int(n_polytopes)
+    starts = rng.uniform(-0.95, 0.95, size=(P, d))
+    cuts = rng.integers(min_cuts, max_cuts + 1, size=P)
+    rows = 2 * d + cuts
+    offsets = np.concatenate([[0], np.cumsum(rows)]).astype(np.int64)
+    A = np.empty((int(offsets[-1]), d))
+    b = np.empty(int(offsets[-1]))
+    eye = np.eye(d)
+    a_all = rng.normal(size=(int(cuts.sum()), d))
+    a_all /= np.linalg.norm(a_all, axis=1, keepdims=True)
+    slack = 0.05 + rng.uniform(0.0, 0.5, size=int(cuts.sum()))
+    owner = np.repeat(np.arange(P), cuts)
+    b_cut = np.einsum("ij,ij->i", a_all, starts[owner]) + slack
+    cut_start = np.concatenate([[0], np.cumsum(cuts)])
+    for j in range(P):
+        s = int(offsets[j])
+        A[s:s + d] = eye
+        A[s + d:s + 2 * d] = -eye
+        b[s:s + 2 * d] = 1.0
+        c0, c1 = int(cut_start[j]), int(cut_start[j + 1])
+        A[s + 2 * d:s + 2 * d + (c1 - c0)] = a_all[c0:c1]
+        b[s + 2 * d:s + 2 * d + (c1 - c0)] = b_cut[c0:c1]
+    return PolytopeWorkload(A, b, offsets, starts)

@@ -26,6 +26,8 @@
 #include "art_common.h"
 #include "art_hip.h"
 
+#include <utility>
+
 namespace art {
 
 __device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1) {
@@ -55,46 +57,57 @@ __global__ __launch_bounds__(256) void philox_kernel(const uint4* __restrict__ c
   if (i < count) out[i] = philox4x32_10(ctr[i], key[i].x, key[i].y);
 }
 
-template <int DM, bool kFixed>
+// Read-only inputs go through the constant address space: uniform indices
+// then compile to scalar loads (one copy per wave, no vector-memory waits).
+typedef const __attribute__((address_space(4))) double cf64;
+typedef const __attribute__((address_space(4))) int64_t ci64;
+
+// DM = the compile-time dimension; EXACT: d == DM (no per-column predicates).
+template <int DM, bool EXACT>
 __global__ __launch_bounds__(64) void har_kernel(art_har_args a) {
   constexpr int WM = har_calls_per_step(DM);
-  const int d = a.d;
+  const int d = EXACT ? DM : a.d;
   const int C = a.chains;
   const int wpp = (C + 63) / 64;
   const int poly = blockIdx.x / wpp;
   const int c = (blockIdx.x - poly * wpp) * 64 + (int)threadIdx.x;
   if (c >= C) return;
-  const int64_t r0 = a.row_offsets[poly];
-  const int m = (int)(a.row_offsets[poly + 1] - r0);
-  const double* __restrict__ A = a.A + r0 * d;
-  const double* __restrict__ b = a.b + r0;
+  ci64* offs = (ci64*)a.row_offsets;
+  const int64_t r0 = offs[poly];
+  const int m = (int)(offs[poly + 1] - r0);
+  cf64* A = (cf64*)(a.A + r0 * d);
+  cf64* b = (cf64*)(a.b + r0);
+  cf64* shape = (cf64*)a.shape;
+  cf64* box_lo = (cf64*)(a.box_lo ? a.box_lo + (size_t)poly * d : nullptr);
+  cf64* box_hi = (cf64*)(a.box_hi ? a.box_hi + (size_t)poly * d : nullptr);
   const int full = a.n / C, rem = a.n % C;
   const int records = full + (c < rem ? 1 : 0);
   const int moves = a.first + (records - 1) * a.stride;
   const int W = har_calls_per_step(d);
   const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
+  const bool fixed = a.directions != nullptr;
 
   double x[DM];
   const double* x0 = a.x0 + ((size_t)poly * C + c) * d;
 #pragma unroll
-  for (int j = 0; j < DM; ++j) x[j] = j < d ? x0[j] : 0.0;
+  for (int j = 0; j < DM; ++j) x[j] = (EXACT || j < d) ? x0[j] : 0.0;
   double* out = a.samples + (size_t)poly * a.n * d;
 
   for (int s = 0; s < moves; ++s) {
     double dir[DM];
     double u;
-    if constexpr (kFixed) {
+    if (fixed) {
       const size_t row = ((size_t)poly * a.steps + s) * C + c;
       const double* dp = a.directions + row * d;
 #pragma unroll
-      for (int j = 0; j < DM; ++j) dir[j] = j < d ? dp[j] : 0.0;
+      for (int j = 0; j < DM; ++j) dir[j] = (EXACT || j < d) ? dp[j] : 0.0;
       u = a.uniforms[row];
     } else {
       uint32_t w[4 * WM];
 #pragma unroll
       for (int q = 0; q < WM; ++q) {
         uint4 r = make_uint4(0u, 0u, 0u, 0u);
-        if (q < W)
+        if (EXACT || q < W)
           r = philox4x32_10(make_uint4((uint32_t)(s * W + q), (uint32_t)c, (uint32_t)poly, 0u), k0,
                             k1);
         w[4 * q] = r.x;
@@ -102,58 +115,80 @@ __global__ __launch_bounds__(64) void har_kernel(art_har_args a) {
         w[4 * q + 2] = r.z;
         w[4 * q + 3] = r.w;
       }
-      // Box-Muller pairs from words (2q, 2q+1); u1 in (0, 1] keeps log finite.
+      // Box-Muller pairs from words (2q, 2q+1): u1 in (0, 1] keeps the log
+      // finite; the hardware sine and cosine take revolutions, so
+      // sin(2 pi u2) is v_sin_f32(u2).
 #pragma unroll
       for (int q = 0; q < (DM + 1) / 2; ++q) {
         const float u1 = (float)((w[2 * q] >> 8) + 1u) * 0x1p-24f;
         const float u2 = (float)(w[2 * q + 1] >> 8) * 0x1p-24f;
-        const float r = sqrtf(-2.0f * logf(u1));
-        float sn, cs;
-        sincospif(2.0f * u2, &sn, &cs);
-        dir[2 * q] = (double)(r * cs);
-        if (2 * q + 1 < DM) dir[2 * q + 1] = (double)(r * sn);
+        const float r = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));
+        dir[2 * q] = (double)(r * __builtin_amdgcn_cosf(u2));
+        if (2 * q + 1 < DM) dir[2 * q + 1] = (double)(r * __builtin_amdgcn_sinf(u2));
       }
       // The chord uniform follows the ceil(d/2) pairs.
-      const int P = (d + 1) / 2;
-      uint32_t wa = 0u, wb = 0u;
+      uint32_t wa, wb;
+      if (EXACT) {
+        wa = w[2 * ((DM + 1) / 2)];
+        wb = w[2 * ((DM + 1) / 2) + 1];
+      } else {
+        const int P = (d + 1) / 2;
+        wa = wb = 0u;
 #pragma unroll
-      for (int q = 0; q < (DM + 1) / 2; ++q)
-        if (q == P - 1) {
-          wa = w[2 * q + 2];
-          wb = w[2 * q + 3];
-        }
+        for (int q = 0; q < (DM + 1) / 2; ++q)
+          if (q == P - 1) {
+            wa = w[2 * q + 2];
+            wb = w[2 * q + 3];
+          }
+      }
       u = ((double)(wa >> 5) * 67108864.0 + (double)(wb >> 6)) * 0x1p-53;
+      if (!EXACT) {
 #pragma unroll
-      for (int j = 0; j < DM; ++j)
-        if (j >= d) dir[j] = 0.0;
+        for (int j = 0; j < DM; ++j)
+          if (j >= d) dir[j] = 0.0;
+      }
     }
-    if (a.shape) {
+    if (shape) {
       double t[DM];
 #pragma unroll
       for (int i = 0; i < DM; ++i) {
         double acc = 0.0;
 #pragma unroll
         for (int j = 0; j < DM; ++j)
-          if (i < d && j < d) acc = fma(a.shape[i * d + j], dir[j], acc);
+          if (EXACT || (i < d && j < d)) acc = fma(shape[i * d + j], dir[j], acc);
         t[i] = acc;
       }
 #pragma unroll
       for (int i = 0; i < DM; ++i) dir[i] = t[i];
     }
     double tlo = -INFINITY, thi = INFINITY;
+    if (box_lo) {
+      // Axis rows x_j <= hi_j and -x_j <= -lo_j: slack and direction product
+      // need no multiplication, and the tighter of repeated rows always wins
+      // the chord's min/max, so one bound per side gives the same chord.
+#pragma unroll
+      for (int j = 0; j < DM; ++j)
+        if (EXACT || j < d) {
+          const double rl = (box_lo[j] - x[j]) / dir[j];
+          const double rh = (box_hi[j] - x[j]) / dir[j];
+          const bool pos = dir[j] > 0.0, neg = dir[j] < 0.0;
+          tlo = pos ? fmax(tlo, rl) : (neg ? fmax(tlo, rh) : tlo);
+          thi = pos ? fmin(thi, rh) : (neg ? fmin(thi, rl) : thi);
+        }
+    }
+#pragma unroll 2
     for (int i = 0; i < m; ++i) {
-      const double* Ai = A + (size_t)i * d;
       double ad = 0.0, ax = 0.0;
 #pragma unroll
       for (int j = 0; j < DM; ++j)
-        if (j < d) {
-          const double aij = Ai[j];
+        if (EXACT || j < d) {
+          const double aij = A[i * d + j];
           ad = fma(aij, dir[j], ad);
           ax = fma(aij, x[j], ax);
         }
       const double ratio = (b[i] - ax) / ad;
-      if (ad < 0.0) tlo = fmax(tlo, ratio);
-      else if (ad > 0.0) thi = fmin(thi, ratio);
+      tlo = ad < 0.0 ? fmax(tlo, ratio) : tlo;
+      thi = ad > 0.0 ? fmin(thi, ratio) : thi;
     }
     const double t = (thi - tlo) * u + tlo;
 #pragma unroll
@@ -163,18 +198,23 @@ __global__ __launch_bounds__(64) void har_kernel(art_har_args a) {
       double* o = out + ((size_t)((done - a.first) / a.stride) * C + c) * d;
 #pragma unroll
       for (int j = 0; j < DM; ++j)
-        if (j < d) o[j] = x[j];
+        if (EXACT || j < d) o[j] = x[j];
     }
   }
 }
 
-template <int DM>
+template <int DM, bool EXACT>
 void launch_har(const art_har_args& a, hipStream_t stream) {
   const int blocks = a.P * ((a.chains + 63) / 64);
-  if (a.directions)
-    har_kernel<DM, true><<<blocks, 64, 0, stream>>>(a);
-  else
-    har_kernel<DM, false><<<blocks, 64, 0, stream>>>(a);
+  har_kernel<DM, EXACT><<<blocks, 64, 0, stream>>>(a);
+}
+
+using HarLaunch = void (*)(const art_har_args&, hipStream_t);
+
+template <int... Ds>
+constexpr HarLaunch exact_launch(int d, std::integer_sequence<int, Ds...>) {
+  HarLaunch table[] = {launch_har<Ds + 1, true>...};
+  return table[d - 1];
 }
 
 }  // namespace art
@@ -200,8 +240,10 @@ extern "C" int art_har_sample(const art_har_args* a, hipStream_t stream) {
     return fail(ART_ERR_ARG,
                 "art_har_sample: need P >= 1, 1 <= d <= 32, 1 <= chains <= n, first >= 1, "
                 "stride >= 1");
-  if (!a->A || !a->b || !a->row_offsets || !a->x0 || !a->samples)
+  if (!a->row_offsets || !a->x0 || !a->samples)
     return fail(ART_ERR_ARG, "art_har_sample: null device pointer");
+  if ((a->box_lo == nullptr) != (a->box_hi == nullptr))
+    return fail(ART_ERR_ARG, "art_har_sample: box_lo and box_hi go together");
   if ((a->directions == nullptr) != (a->uniforms == nullptr))
     return fail(ART_ERR_ARG, "art_har_sample: directions and uniforms go together");
   const int full = a->n / a->chains, rem = a->n % a->chains;
@@ -213,13 +255,9 @@ extern "C" int art_har_sample(const art_har_args* a, hipStream_t stream) {
   if ((int64_t)a->P * ((a->chains + 63) / 64) > 0x7fffffff)
     return fail(ART_ERR_ARG, "art_har_sample: grid too large");
   const int d = a->d;
-  if (d <= 2) launch_har<2>(*a, stream);
-  else if (d <= 4) launch_har<4>(*a, stream);
-  else if (d <= 8) launch_har<8>(*a, stream);
-  else if (d <= 12) launch_har<12>(*a, stream);
-  else if (d <= 16) launch_har<16>(*a, stream);
-  else if (d <= 24) launch_har<24>(*a, stream);
-  else launch_har<32>(*a, stream);
+  if (d <= 16) exact_launch(d, std::make_integer_sequence<int, 16>{})(*a, stream);
+  else if (d <= 24) launch_har<24, false>(*a, stream);
+  else launch_har<32, false>(*a, stream);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? ART_OK : hip_fail(e, "har_kernel launch");
 }

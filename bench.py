@@ -38,6 +38,10 @@ def parse():
     ap.add_argument("--leaves", type=int, default=4096)
     ap.add_argument("--chunk-rows", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--only", choices=("all", "ridge", "sampler"), default="all",
+                    help="run one leg (profiling); the JSON line keeps the ridge metric")
+    ap.add_argument("--sampler-steps", type=int, default=3)
+    ap.add_argument("--polytopes", type=int, default=65536)
     ap.add_argument("--copy-probe", action="store_true",
                     help="after timing, run a plain 1 GiB device copy three times: the "
                          "calibration kernel for FETCH_SIZE / WRITE_SIZE counter runs")
@@ -121,6 +125,84 @@ def copy_probe(torch, nbytes=1 << 30):
     print(f"[bench] copy probe: 3 copies of {nbytes} bytes", file=sys.stderr, flush=True)
 
 
+def bench_sampler(args):
+    """Hit-and-run leg: P polytopes in d=12 (box plus 4-40 random cuts), 64
+    chains x 1000 moves, thinning 10 (6,400 samples per polytope), Philox
+    directions; one step = one launch over every polytope, samples written
+    to HBM."""
+    import torch
+
+    from art.sampling import PolytopeBatch
+    from art.synthetic import polytope_workload
+
+    P, C, moves, thin, d = args.polytopes, 64, 1000, 10, 12
+    wl = polytope_workload(P, d=d, seed=4)
+    batch = PolytopeBatch.from_csr(wl.A, wl.b, wl.offsets)
+    n = C * (moves // thin)
+    x0 = torch.from_numpy(np.repeat(wl.starts[:, None, :], C, axis=1)).to("cuda")
+    out = torch.empty((P, n, d), dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream()
+    batch.sample(x0, n, first=thin, stride=thin, seed=1, out=out)
+    torch.cuda.synchronize()
+    steps = max(1, args.sampler_steps)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    ev[0].record(stream)
+    for i in range(steps):
+        batch.sample(x0, n, first=thin, stride=thin, seed=100 + i, out=out)
+        ev[i + 1].record(stream)
+    torch.cuda.synchronize()
+    step_ms = np.array([ev[i].elapsed_time(ev[i + 1]) for i in range(steps)])
+    ms = float(step_ms.mean())
+    m = np.diff(wl.offsets).astype(np.float64)
+    flops = C * moves * float(np.sum(2 * m * d + 3 * m + 2 * d))
+    check = min(P, 512)
+    sub = out[:check].cpu().numpy()
+    worst = max(float(np.max(sub[q] @ wl.polytope(q)[0].T - wl.polytope(q)[1]))
+                for q in range(check))
+    print(f"[bench] sampler {ms:.2f} ms/step, worst slack violation {worst:.3e}",
+          file=sys.stderr, flush=True)
+    result = {
+        "metric": "hit-and-run samples/s", "value": P * n / (ms * 1e-3), "unit": "samples/s",
+        "ms_per_step": ms, "steps": steps,
+        "polytopes_per_s": P / (ms * 1e-3),
+        "dtype": "f64 chord and slacks, f32 Philox directions",
+        "config": {"workload": (f"{P} polytopes, d=12, box plus 4-40 random cuts, 64 chains x "
+                                "1000 moves, thinning 10, seed 4"),
+                   "polytopes": P, "mean_rows": float(m.mean()), "samples_per_polytope": n},
+        "max_constraint_violation": worst, "violation_checked_polytopes": check,
+        "roofline": {"bound": "fp64", "achieved": flops / (ms * 1e-3) / 1e12,
+                     "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
+                     "frac": flops / (ms * 1e-3) / 1e12 / PEAK_TFLOPS["fp64"],
+                     "algorithmic_flops_per_launch": flops,
+                     "flops_per_chain_move": "2 m d + 3 m + 2 d",
+                     "samples_write_GBs": P * n * d * 8 / (ms * 1e-3) / 1e9,
+                     "kernel": "har_kernel<12, true>"},
+    }
+    if not args.no_cpu:
+        # The reference's NumPy batched transition (oracle restatement), one
+        # thread, on a bounded sample of the same polytopes.
+        from threadpoolctl import threadpool_limits
+
+        from oracle import sampling as S
+
+        rng = np.random.default_rng(0)
+        k = 16
+        with threadpool_limits(1):
+            t0 = time.perf_counter()
+            for q in range(k):
+                A, b = wl.polytope(q)
+                S.batched_sample(A, b, np.repeat(wl.starts[q][None], C, axis=0), n, 0, thin,
+                                 rng.normal(size=(moves, C, d)), rng.random((moves, C)))
+            dt = time.perf_counter() - t0
+        result["cpu_baseline"] = {"value": k * n / dt, "unit": "samples/s", "cores": 1,
+                                  "kind": "port",
+                                  "sample": (f"first {k} polytopes, NumPy batched transitions "
+                                             f"(64 chains, 1000 moves each), {dt:.2f} s")}
+    del out
+    torch.cuda.empty_cache()
+    return result
+
+
 def rel_err(a, b):
     return np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-300)
 
@@ -138,6 +220,10 @@ def main():
 
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a HIP device")
+    sys.path.insert(0, str(ROOT))
+    if args.only == "sampler":
+        print(json.dumps(bench_sampler(args)))
+        return
     wl = leaf_ridge_workload(n_leaves=args.leaves, seed=2)
     dev = torch.device("cuda", 0)
     X = torch.from_numpy(wl.X).to(dev)
@@ -230,6 +316,10 @@ def main():
     if args.copy_probe:
         copy_probe(torch)
     batch.close()
+    del X, y, w
+    torch.cuda.empty_cache()
+    if args.only == "all":
+        result["hit_and_run"] = bench_sampler(args)
     print(json.dumps(result))
 
 

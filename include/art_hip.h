@@ -187,7 +187,11 @@ void art_plan_destroy(art_ridge_plan* plan);
  * back to back, polytope q owning rows [row_offsets[q], row_offsets[q+1])
  * (row_offsets (P+1,) int64); x0 (P, chains, d) feasible starts; shape
  * (d, d) row-major or NULL (direction d <- shape . d, the covariance shaping
- * V diag(sqrt(lambda)) of art/sampling.py:496-537,865-875).
+ * V diag(sqrt(lambda)) of art/sampling.py:496-537,865-875).  Optional axis
+ * bounds box_lo, box_hi (P, d) (+-inf where absent) add the rows
+ * x_j <= box_hi[q, j] and -x_j <= -box_lo[q, j] to polytope q; they give the
+ * same chord as listing those rows in A, for less work.  A polytope may then
+ * have no rows in A.
  * Schedule: chain c records after moves first, first + stride, ...;
  * K_c = n / chains + (c < n % chains) records; record k is sample
  * k * chains + c of samples (P, n, d).  The batched sampler is
@@ -208,6 +212,8 @@ typedef struct {
   const int64_t* row_offsets;
   const double* x0;
   const double* shape;
+  const double* box_lo;
+  const double* box_hi;
   const double* directions;
   const double* uniforms;
   double* samples;

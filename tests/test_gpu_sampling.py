@@ -57,9 +57,10 @@ def test_philox_matches_the_oracle_bit_for_bit(torch_gpu):
     np.testing.assert_array_equal(got[0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8])
 
 
+@pytest.mark.parametrize("split_box", [True, False])
 @pytest.mark.parametrize("C,n,burn,thin", [(1, 12, 3, 1), (4, 11, 2, 3), (4, 12, 0, 1),
                                            (70, 213, 5, 2), (3, 3, 0, 4)])
-def test_fixed_stream_matches_the_batched_oracle(torch_gpu, C, n, burn, thin):
+def test_fixed_stream_matches_the_batched_oracle(torch_gpu, C, n, burn, thin, split_box):
     from art.sampling import PolytopeBatch
 
     rng = np.random.default_rng(37 + C)
@@ -72,7 +73,7 @@ def test_fixed_stream_matches_the_batched_oracle(torch_gpu, C, n, burn, thin):
     tu = rng.random((thin, rem)) if rem else None
     expected = S.batched_sample(A, b, starts, n, burn, thin, dirs, us, td, tu)
     D, U = _pack(dirs, us, td, tu, C)
-    got = PolytopeBatch(([A], [b])).sample(
+    got = PolytopeBatch(([A], [b]), split_box=split_box).sample(
         starts[None], n, first=burn + thin, stride=thin, directions=D[None], uniforms=U[None])
     np.testing.assert_allclose(got[0].cpu().numpy(), expected, rtol=1e-12, atol=1e-12)
 

@@ -8,7 +8,7 @@ from art.domain import BoxDomain, PolytopeRegion, split_region
 from art.sampling import (BatchedHitAndRunSampler, HitAndRunSampler,
                           ParallelNumbaBatchedHitAndRunSampler, find_feasible_point,
                           floor_covariance_eigenvalues, infer_box_bounds,
-                          sample_covariance_eigendecomposition)
+                          sample_covariance_eigendecomposition, split_axis_rows)
 
 
 def test_region_and_split_conventions():
@@ -70,3 +70,22 @@ def test_parallel_sampler_defaults_and_thread_check():
         HitAndRunSampler(thinning=0).sample(box, 3, x0=np.zeros(2))
     with pytest.raises(ValueError, match="feasible"):
         HitAndRunSampler().sample(box, 3, x0=np.array([2., 0.]))
+
+
+def test_axis_rows_become_the_tightest_bounds():
+    A = np.array([[1., 0.], [-1., 0.], [0., 1.], [0., -1.], [1., 0.], [2., 0.], [1., 1.],
+                  [0., -1.]])
+    b = np.array([1., 1., 2., 3., .5, 1., .3, 2.5])
+    offsets = np.array([0, 5, 8])
+    Ag, bg, og, lo, hi = split_axis_rows(A, b, offsets)
+    np.testing.assert_array_equal(hi, [[.5, 2.], [np.inf, np.inf]])
+    np.testing.assert_array_equal(lo, [[-1., -3.], [-np.inf, -2.5]])
+    np.testing.assert_array_equal(og, [0, 0, 2])
+    np.testing.assert_array_equal(Ag, [[2., 0.], [1., 1.]])
+    np.testing.assert_array_equal(bg, [1., .3])
+    # The chord from bounds equals the chord from the rows.
+    from oracle import sampling as S
+    x, dvec = np.array([.1, .2]), np.array([.3, -.7])
+    lo_r, hi_r = S.step_interval(A[:5], b[:5], x, dvec)
+    r = np.concatenate([(lo[0] - x) / dvec, (hi[0] - x) / dvec])
+    assert lo_r == max(r[r < 0]) and hi_r == min(r[r > 0])
