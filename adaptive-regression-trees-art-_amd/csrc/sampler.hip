@@ -8,7 +8,9 @@
 // t = t_lo + u (t_hi - t_lo) (art/sampling.py:877-962,
 // art/_numba_sampling.py:9-40).  Recomputing the slack instead of updating it
 // incrementally keeps every recorded sample within rounding of A x <= b for
-// any chain length and needs no per-chain slack storage.
+// any chain length and needs no per-chain slack storage.  Ratios use a
+// reciprocal (v_rcp_f64 and two Newton steps) times the slack, a few ulp from
+// the quotient.
 //
 // Schedule (one rule for the reference's batched and scalar samplers): chain
 // c records after moves first, first + stride, ..., K_c records in all, with
@@ -165,15 +167,24 @@ __global__ __launch_bounds__(64) void har_kernel(art_har_args a) {
     if (box_lo) {
       // Axis rows x_j <= hi_j and -x_j <= -lo_j: slack and direction product
       // need no multiplication, and the tighter of repeated rows always wins
-      // the chord's min/max, so one bound per side gives the same chord.
+      // the chord's min/max, so one bound per side gives the same chord.  The
+      // two ratios share one reciprocal (v_rcp_f64 plus two Newton steps, a
+      // few ulp from the quotient); a zero direction component is skipped.
 #pragma unroll
       for (int j = 0; j < DM; ++j)
         if (EXACT || j < d) {
-          const double rl = (box_lo[j] - x[j]) / dir[j];
-          const double rh = (box_hi[j] - x[j]) / dir[j];
-          const bool pos = dir[j] > 0.0, neg = dir[j] < 0.0;
-          tlo = pos ? fmax(tlo, rl) : (neg ? fmax(tlo, rh) : tlo);
-          thi = pos ? fmin(thi, rh) : (neg ? fmin(thi, rl) : thi);
+          const double dj = dir[j];
+          double r = __builtin_amdgcn_rcp(dj);
+          r = fma(r, fma(-dj, r, 1.0), r);
+          r = fma(r, fma(-dj, r, 1.0), r);
+          const double rl = (box_lo[j] - x[j]) * r;
+          const double rh = (box_hi[j] - x[j]) * r;
+          const bool order = rl < rh;
+          const double lo_c = order ? rl : rh;
+          const double hi_c = order ? rh : rl;
+          const bool nz = dj != 0.0;
+          tlo = (nz && lo_c > tlo) ? lo_c : tlo;
+          thi = (nz && hi_c < thi) ? hi_c : thi;
         }
     }
 #pragma unroll 2
@@ -186,9 +197,12 @@ __global__ __launch_bounds__(64) void har_kernel(art_har_args a) {
           ad = fma(aij, dir[j], ad);
           ax = fma(aij, x[j], ax);
         }
-      const double ratio = (b[i] - ax) / ad;
-      tlo = ad < 0.0 ? fmax(tlo, ratio) : tlo;
-      thi = ad > 0.0 ? fmin(thi, ratio) : thi;
+      double r = __builtin_amdgcn_rcp(ad);
+      r = fma(r, fma(-ad, r, 1.0), r);
+      r = fma(r, fma(-ad, r, 1.0), r);
+      const double ratio = (b[i] - ax) * r;
+      tlo = (ad < 0.0 && ratio > tlo) ? ratio : tlo;
+      thi = (ad > 0.0 && ratio < thi) ? ratio : thi;
     }
     const double t = (thi - tlo) * u + tlo;
 #pragma unroll
