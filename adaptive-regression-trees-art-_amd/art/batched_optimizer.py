@@ -175,10 +175,13 @@ class BatchedOptimizerWorkspace:
         w = w / nrm
         return np.concatenate([w, [float(np.median(self.host_X[node] @ w))]])
 
-    def _launch(self, nodes, theta0, mode, fallback, hard_refit):
-        """One device launch over problems (nodes[i], theta0[i]); host copies out."""
+    def _launch(self, nodes, theta0, mode, fallback, hard_refit, anneal=None, timing=None):
+        """One device launch over problems (nodes[i], theta0[i]); host copies out.
+
+        anneal = (iters, t0, t1, step) for mode 1; timing, a list, receives
+        the launch's device milliseconds."""
         torch, s, P = self.torch, self.s, len(nodes)
-        mi = int(s.max_iters)
+        mi = int(s.max_iters) if anneal is None else int(anneal[0])
         dev = dict(device=self.device)
         f64 = dict(dtype=torch.float64, **dev)
         node_t = torch.as_tensor(np.ascontiguousarray(nodes, dtype=np.int32), **dev)
@@ -204,14 +207,24 @@ class BatchedOptimizerWorkspace:
         a.heavy, a.max_failures = int(s.heavy_backtrack_threshold), int(s.max_line_search_failures)
         a.adaptive, a.required = int(bool(s.adaptive_alpha)), int(self.required)
         a.fallback, a.hard_refit, a.H, a.mode = int(bool(fallback)), int(bool(hard_refit)), self.H, mode
-        a.anneal_iters, a.t0, a.t1, a.step = 0, 0.0, 0.0, 0.0
+        if anneal is None:
+            a.anneal_iters, a.t0, a.t1, a.step = 0, 0.0, 0.0, 0.0
+        else:
+            a.anneal_iters, a.t0, a.t1, a.step = int(anneal[0]), *map(float, anneal[1:])
         a.theta_out, a.beta_out, a.hard = _hip.ptr(out_theta), _hip.ptr(out_beta), _hip.ptr(hard)
         a.info, a.history, a.scalars = _hip.ptr(info), _hip.ptr(hist), _hip.ptr(scal)
         with torch.cuda.stream(self.stream):
+            if timing is not None:
+                ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+                ev[0].record(self.stream)
             _hip.check(self.lib.art_split_optimize(ctypes.byref(a), P,
                                                    _hip.stream_handle(self.stream)))
+            if timing is not None:
+                ev[1].record(self.stream)
         self.last_launch_problems = P
         self.stream.synchronize()
+        if timing is not None:
+            timing.append(ev[0].elapsed_time(ev[1]))
         return (out_theta.cpu().numpy(), out_beta.cpu().numpy(), hard.cpu().numpy(),
                 info.cpu().numpy(), hist.cpu().numpy(), scal.cpu().numpy())
 
@@ -299,6 +312,49 @@ class BatchedOptimizerWorkspace:
                     q, _, _, meta = best[i]
                     results[i] = self._export(q, theta[q], beta[q], hard[q], meta, parent_loss[i])
         return results
+
+    def anneal(self, theta0=None, *, nodes=None, t0=0.2, t1=0.01, iters=200, step=0.05,
+               timing=None):
+        """Annealed fixed-step projected gradient from every start, then a hard refit.
+
+        theta0: (r, d+1) or (R, r, d+1) starts for the r selected nodes, or
+        None for random starts (Gaussian unit w, z = median(X w)).  Iteration
+        k uses T_k = t0 (t1/t0)^(k/(iters-1)) and
+        theta <- retract(theta - step * P(grad)).  No reference counterpart
+        (the reference optimizes at a fixed temperature).  Returns a dict of
+        host arrays per (restart, node): theta (R, r, d+1), beta (R, r, 2, p),
+        hard loss (R, r), n_right (R, r), ok (R, r), soft-loss history
+        (R, r, iters), and best (r,): the restart with the lowest hard loss.
+        """
+        self._check_open()
+        if not self.k:
+            raise RuntimeError("Load data before optimizing.")
+        ids = self.all_ids[:self.k] if nodes is None else _node_ids(nodes, self.k)
+        count, R = len(ids), self.R
+        if iters < 1 or not (t0 > 0 and t1 > 0 and np.isfinite(step)):
+            raise ValueError("Need iters >= 1, positive temperatures and a finite step.")
+        if theta0 is None:
+            starts = np.stack([[self._random_theta(j) for j in ids] for _ in range(R)])
+        else:
+            starts = np.asarray(theta0, dtype=float)
+            if starts.ndim == 2:
+                starts = starts[None]
+            if starts.shape[1:] != (count, self.d + 1) or not np.isfinite(starts).all():
+                raise ValueError("theta0 must be finite with shape (r,d+1) or (R,r,d+1).")
+            starts = np.stack([[project_unit_w(t) for t in rs] for rs in starts])
+        Rr = starts.shape[0]
+        nodes_all = np.tile(ids, Rr)
+        theta, beta, hard, info, hist, _ = self._launch(
+            nodes_all, starts.reshape(-1, self.d + 1), 1, False, True,
+            anneal=(iters, t0, t1, step), timing=timing)
+        shape = (Rr, count)
+        ok = info[:, 6].reshape(shape) == 0
+        loss = np.where(ok, hard[:, 0].reshape(shape), np.inf)
+        return {"theta": theta.reshape(shape + (self.d + 1,)),
+                "beta": beta.reshape(shape + (2, self.p)), "loss": loss,
+                "n_right": hard[:, 3].reshape(shape).astype(int), "ok": ok,
+                "soft_loss_history": hist[:, 0, :iters].reshape(shape + (iters,)),
+                "best": np.argmin(loss, axis=0), "starts": starts}
 
     def _log_fits(self, prob_slot, info, count, hard_refit):
         """Record the logical fit batches of the device loop in ``fit_sizes``.

@@ -155,3 +155,14 @@ def polytope_workload(n_polytopes=65536, d=12, min_cuts=4, max_cuts=40, seed=4):
         A[s + 2 * d:s + 2 * d + (c1 - c0)] = a_all[c0:c1]
         b[s + 2 * d:s + 2 * d + (c1 - c0)] = b_cut[c0:c1]
     return PolytopeWorkload(A, b, offsets, starts)
+
+
+def split_workload(n_nodes=2048, n=8192, d=16, n_bumps=8, seed=3):
+    """Equal-size split nodes: X (K, n, d) ~ U[-1, 1], y (K, n) a seeded
+    n_bumps-bump Gaussian mixture per node."""
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(-1.0, 1.0, size=(n_nodes, n, d))
+    y = np.empty((n_nodes, n))
+    for k in range(n_nodes):
+        y[k] = BumpMixture.random(rng, n_bumps, d)(X[k])
+    return X, y

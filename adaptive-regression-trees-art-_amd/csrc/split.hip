@@ -53,6 +53,9 @@ __device__ inline double gate(double score) {
   return 1.0 / (1.0 + exp(-fmin(50.0, fmax(-50.0, score))));
 }
 
+constexpr int kChunk = 128;             // rows staged in LDS per pass step
+constexpr int kXStride = kSplitMaxD + 2;  // odd stride in doubles: 33
+
 struct SplitShared {
   double theta[kSplitMaxD + 1];
   double pdir[kSplitMaxD + 1];
@@ -67,8 +70,15 @@ struct SplitShared {
   double piv[2];
   double bc[2];
   int ired[4];
-  double dred[4 * (kSplitMaxD + 4)];
-  double tot[kSplitMaxD + 4];
+  double dred[4 * 64];
+  double tot[64];
+  double xs[kChunk * kXStride];  // staged rows: x_0..x_{d-1}, y; stride kXStride
+  double qs[2][kChunk];          // per-row side weights (fit) or gradient terms (stats)
+  signed char fa[kSplitMaxP];    // feature f = x[fa] * x[fb]; -1 stands for 1
+  signed char fb[kSplitMaxP];
+  signed char core[kSplitMaxP];  // basis column of non-constant feature k
+  int nf;                        // number of non-constant features
+  int cbias;                     // basis column of the constant, or -1
 };
 
 struct SplitCtx {
@@ -80,109 +90,201 @@ struct SplitCtx {
   double ridge;
 };
 
-// Feature f of the augmented row [phi(x) | y | 0...].
-__device__ inline double aug_feature(const Basis& B, const double* x, double yv, int fa, int fb,
-                                     int kind) {
-  if (kind == 1) return yv;
-  if (kind == 2) return 0.0;
-  double v = fa >= 0 ? x[fa] : 1.0;
-  if (fb >= 0) v *= x[fb];
+// Feature tables of the basis (once per kernel, before any pass).
+__device__ inline void init_basis_tables(const Basis& B, SplitShared& S) {
+  if (threadIdx.x == 0) {
+    int nf = 0, cb = -1;
+    for (int f = 0; f < B.p; ++f) {
+      int a, b;
+      basis_factors(B, f, a, b);
+      S.fa[f] = (signed char)a;
+      S.fb[f] = (signed char)b;
+      if (a < 0) cb = f;
+      else S.core[nf++] = (signed char)f;
+    }
+    S.nf = nf;
+    S.cbias = cb;
+  }
+  __syncthreads();
+}
+
+// Row staging: a 32-lane group per row, 256 threads -> 8 rows per sweep and
+// kStage sweeps per chunk.  All kStage global loads of a thread issue before
+// its LDS stores, so a chunk costs one memory round trip; the values are live
+// only while staging, and the second resident workgroup of the CU computes
+// meanwhile.  S.xs holds x_0..x_{d-1}, y per row with stride d + 1.
+constexpr int kStage = kChunk / 8;
+
+__device__ inline void stage_rows(const SplitCtx& C, SplitShared& S, int base, int rows) {
+  const int g = threadIdx.x >> 5, j = threadIdx.x & 31;
+  const bool isx = j < C.d, isy = j == C.d;
+  const double* src = isx ? C.X + ((size_t)base + g) * C.d + j : C.y + base + g;
+  const size_t stride = isx ? 8 * (size_t)C.d : 8;
+  const int lim = (isx || isy) ? rows : 0;
+  double v[kStage];
+#pragma unroll
+  for (int i = 0; i < kStage; ++i) {
+    v[i] = (g + 8 * i < lim) ? *src : 0.0;
+    src += stride;
+  }
+  double* dst = S.xs + g * kXStride + j;
+#pragma unroll
+  for (int i = 0; i < kStage; ++i)
+    if (g + 8 * i < lim) dst[8 * kXStride * i] = v[i];
+}
+
+// Iterates the node's rows in staged chunks: body(rows) runs with chunk rows
+// [base, base + rows) in S.xs.
+template <class Body>
+__device__ inline void for_each_chunk(const SplitCtx& C, SplitShared& S, Body body) {
+  for (int base = 0; base < C.n; base += kChunk) {
+    const int rows = min(kChunk, C.n - base);
+    __syncthreads();  // the previous chunk is consumed
+    stage_rows(C, S, base, rows);
+    __syncthreads();
+    body(rows);
+  }
+}
+
+__device__ inline double staged_feature(const SplitShared& S, const double* xr, int f) {
+  const int a = S.fa[f], b = S.fb[f];
+  double v = a >= 0 ? xr[a] : 1.0;
+  if (b >= 0) v *= xr[b];
   return v;
 }
 
-__device__ inline void column_kind(const Basis& B, int f, int& fa, int& fb, int& kind) {
-  fa = fb = -1;
-  kind = 2;
-  if (f < B.p) {
-    basis_factors(B, f, fa, fb);
-    kind = 0;
-  } else if (f == B.p) {
-    kind = 1;
-  }
-}
-
-// Row feature vector into registers (p <= PM).
-template <int PM>
-__device__ inline void row_features(const Basis& B, const double* x, double (&phi)[PM]) {
+// Sums of K values over the block; result in S.tot[0..K).  All threads call.
+template <int K>
+__device__ inline void block_sums(const double (&v)[K], SplitShared& S, int count = K) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
 #pragma unroll
-  for (int f = 0; f < PM; ++f) {
-    if (f < B.p) {
-      int a, b;
-      basis_factors(B, f, a, b);
-      double v = a >= 0 ? x[a] : 1.0;
-      if (b >= 0) v *= x[b];
-      phi[f] = v;
-    } else {
-      phi[f] = 0.0;
+  for (int k = 0; k < K; ++k)
+    if (k < count) {
+      const double s = wave_sum(v[k]);
+      if (lane == 0) S.dred[wave * 64 + k] = s;
     }
-  }
-}
-
-__device__ inline double dotw(const double* x, const double* w, int d) {
-  double u = 0.0;
-  for (int j = 0; j < d; ++j) u += x[j] * w[j];
-  return u;
+  __syncthreads();
+  if ((int)threadIdx.x < count)
+    S.tot[threadIdx.x] = ((S.dred[threadIdx.x] + S.dred[64 + threadIdx.x]) +
+                          S.dred[128 + threadIdx.x]) + S.dred[192 + threadIdx.x];
+  __syncthreads();
 }
 
 // Soft (hard = false) or hard (hard = true) weighted fits of both sides at
-// the current theta.  Writes beta, lam and status (0 / -1 bad gate /
-// pivot / -2 non-finite / -3 solve) for both sides into shared state.
+// the current theta; half = true fits with pi = 1/2 (the parent fit).
+// Writes beta, lam and status (0 / -1 bad gate / pivot / -2 non-finite /
+// -3 solve) for both sides into shared state.
+//
+// Per staged chunk of rows: one thread per row computes the gate and both
+// side weights q_L = max(1 - pi, floor), q_R = max(pi, floor) and the
+// weighted sums of 1, y and y^2; then v_mfma_f64_16x16x4 accumulates the
+// Gram of the non-constant features (lane (col, kq) feeds feature 16t + col
+// of row 4s + kq), and the same lanes accumulate the feature sums against
+// q and q y (the constant and y columns of the augmented system).
 template <int T>
 __device__ void fit_sides(const SplitCtx& C, SplitShared& S, bool hard, double floor,
                           bool half = false) {
   constexpr int NP = T * (T + 1) / 2;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int col = lane & 15, kq = lane >> 4;
-  const int d = C.d, p = C.p, n = C.n;
-  int fa[T], fb[T], kind[T];
+  const int d = C.d, p = C.p, n = C.n, sd = kXStride, nf = S.nf;
+  int fa[T], fb[T];
+  bool live[T];
 #pragma unroll
-  for (int t = 0; t < T; ++t) column_kind(C.B, 16 * t + col, fa[t], fb[t], kind[t]);
+  for (int t = 0; t < T; ++t) {
+    const int k = 16 * t + col;
+    live[t] = k < nf;
+    const int f = live[t] ? S.core[k] : 0;
+    fa[t] = live[t] ? S.fa[f] : 0;
+    fb[t] = live[t] ? S.fb[f] : -1;
+  }
   f64x4 acc0[NP], acc1[NP];
 #pragma unroll
   for (int q = 0; q < NP; ++q) {
     acc0[q] = f64x4{0.0, 0.0, 0.0, 0.0};
     acc1[q] = f64x4{0.0, 0.0, 0.0, 0.0};
   }
-  const double z = S.theta[d];
+  double e[4 * T];  // per tile: sum q0 v, q0 y v, q1 v, q1 y v
+#pragma unroll
+  for (int k = 0; k < 4 * T; ++k) e[k] = 0.0;
+  double sc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // sum q0, q0 y, q0 y^2, q1, ...
   int bad = 0;
-  for (int s = 0; 16 * s < n; ++s) {
-    const int r = 16 * s + 4 * wave + kq;
-    const bool valid = r < n;
-    const double* x = C.X + (size_t)(valid ? r : 0) * d;
-    const double yv = valid ? C.y[r] : 0.0;
-    const double u = dotw(x, S.theta, d);
-    double pi;
-    if (half) {
-      pi = 0.5;
-    } else if (hard) {
-      pi = (isfinite(u) && isfinite(z)) ? (double)(u >= z) : __builtin_nan("");
-    } else {
-      const double score = (u - z) / C.T;
-      pi = (C.T > 0.0 && isfinite(C.T) && isfinite(score)) ? gate(score) : __builtin_nan("");
-    }
-    const bool okg = isfinite(pi) && pi >= 0.0 && pi <= 1.0;
-    bad |= valid && !okg;
-    const double q0 = (valid && okg) ? fmax(1.0 - pi, floor) : 0.0;
-    const double q1 = (valid && okg) ? fmax(pi, floor) : 0.0;
-    double ph[T];
-#pragma unroll
-    for (int t = 0; t < T; ++t) ph[t] = valid ? aug_feature(C.B, x, yv, fa[t], fb[t], kind[t]) : 0.0;
-    int q = 0;
-#pragma unroll
-    for (int t1 = 0; t1 < T; ++t1)
-#pragma unroll
-      for (int t2 = t1; t2 < T; ++t2, ++q) {
-        acc0[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(q0 * ph[t1], ph[t2], acc0[q], 0, 0, 0);
-        acc1[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(q1 * ph[t1], ph[t2], acc1[q], 0, 0, 0);
+  const double z = S.theta[d];
+  for_each_chunk(C, S, [&](int rows) {
+    if (tid < kChunk) {
+      double q0 = 0.0, q1 = 0.0;
+      if (tid < rows) {
+        const double* xr = S.xs + tid * sd;
+        double u = 0.0;
+        for (int j = 0; j < d; ++j) u = fma(xr[j], S.theta[j], u);
+        double pi;
+        if (half) {
+          pi = 0.5;
+        } else if (hard) {
+          pi = (isfinite(u) && isfinite(z)) ? (double)(u >= z) : __builtin_nan("");
+        } else {
+          const double score = (u - z) / C.T;
+          pi = (C.T > 0.0 && isfinite(C.T) && isfinite(score)) ? gate(score) : __builtin_nan("");
+        }
+        const bool okg = isfinite(pi) && pi >= 0.0 && pi <= 1.0;
+        bad |= !okg;
+        q0 = okg ? fmax(1.0 - pi, floor) : 0.0;
+        q1 = okg ? fmax(pi, floor) : 0.0;
+        const double yv = xr[d];
+        sc[0] += q0;
+        sc[1] += q0 * yv;
+        sc[2] += q0 * yv * yv;
+        sc[3] += q1;
+        sc[4] += q1 * yv;
+        sc[5] += q1 * yv * yv;
       }
-  }
-  // Tiles into packed lower augmented systems: element (a, b) of tile (t1, t2)
-  // is G[16 t1 + a][16 t2 + b] and lands in slot (16 t2 + b, 16 t1 + a).
+      S.qs[0][tid] = q0;
+      S.qs[1][tid] = q1;
+    }
+    __syncthreads();
+    for (int st = wave; st < kChunk / 4; st += 4) {
+      const int r = 4 * st + kq;
+      const bool valid = r < rows;
+      const double* xr = S.xs + r * sd;
+      const double q0 = S.qs[0][r], q1 = S.qs[1][r];
+      const double yv = valid ? xr[d] : 0.0;
+      double v[T];
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        double x = 0.0;
+        if (valid && live[t]) {
+          x = xr[fa[t]];
+          if (fb[t] >= 0) x *= xr[fb[t]];
+        }
+        v[t] = x;
+      }
+      int q = 0;
+#pragma unroll
+      for (int t1 = 0; t1 < T; ++t1)
+#pragma unroll
+        for (int t2 = t1; t2 < T; ++t2, ++q) {
+          acc0[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(q0 * v[t1], v[t2], acc0[q], 0, 0, 0);
+          acc1[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(q1 * v[t1], v[t2], acc1[q], 0, 0, 0);
+        }
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        e[4 * t + 0] += q0 * v[t];
+        e[4 * t + 1] += q0 * yv * v[t];
+        e[4 * t + 2] += q1 * v[t];
+        e[4 * t + 3] += q1 * yv * v[t];
+      }
+    }
+  });
+  // Assemble the packed augmented systems [phi | y]^T diag(q) [phi | y].
   const int nel = (p + 1) * (p + 2) / 2;
-  for (int e = tid; e < nel; e += blockDim.x) {
-    S.M[0][e] = 0.0;
-    S.M[1][e] = 0.0;
+  __syncthreads();
+  for (int k = tid; k < nel; k += blockDim.x) {
+    S.M[0][k] = 0.0;
+    S.M[1][k] = 0.0;
   }
+  // Gram of the non-constant features: element (a, b) of tile pair (t1, t2)
+  // is G[16 t1 + a][16 t2 + b]; keep a <= b on diagonal tiles.
   for (int side = 0; side < 2; ++side) {
     int q = 0;
 #pragma unroll
@@ -195,9 +297,49 @@ __device__ void fit_sides(const SplitCtx& C, SplitShared& S, bool hard, double f
           S.red[wave * 256 + (kq + 4 * r) * 16 + col] = side == 0 ? acc0[q][r] : acc1[q][r];
         __syncthreads();
         const double v = ((S.red[tid] + S.red[256 + tid]) + S.red[512 + tid]) + S.red[768 + tid];
-        const int rr = 16 * t1 + (tid >> 4), cc = 16 * t2 + (tid & 15);
-        if (rr <= cc && cc <= p) S.M[side][tri(cc, rr)] = v;
+        const int ka = 16 * t1 + (tid >> 4), kb = 16 * t2 + (tid & 15);
+        if (ka < nf && kb < nf && (t1 != t2 || ka <= kb)) {
+          const int ca = S.core[ka], cb = S.core[kb];
+          S.M[side][ca >= cb ? tri(ca, cb) : tri(cb, ca)] = v;
+        }
       }
+  }
+  // Feature sums against q and q y: reduce over the four row lanes (kq) of
+  // each wave, then over the waves.
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 4 * T; ++k) {
+    double s = e[k];
+    s += __shfl_xor(s, 16, kWave);
+    s += __shfl_xor(s, 32, kWave);
+    if (kq == 0) S.red[wave * 256 + k * 16 + col] = s;
+  }
+  __syncthreads();
+  if (tid < 64 * T) {
+    const int k = tid >> 4, c = tid & 15;
+    const int t = k >> 2, kind = k & 3;
+    const int feat = 16 * t + c;
+    const double s = ((S.red[tid] + S.red[256 + tid]) + S.red[512 + tid]) + S.red[768 + tid];
+    if (feat < nf) {
+      const int cf = S.core[feat];
+      const int side = kind >> 1;
+      if (kind & 1) {
+        S.M[side][tri(p, cf)] = s;
+      } else if (S.cbias >= 0) {
+        const int cb = S.cbias;
+        S.M[side][cf >= cb ? tri(cf, cb) : tri(cb, cf)] = s;
+      }
+    }
+  }
+  block_sums<6>(sc, S);
+  if (tid == 0) {
+    for (int side = 0; side < 2; ++side) {
+      if (S.cbias >= 0) {
+        S.M[side][tri(S.cbias, S.cbias)] = S.tot[3 * side];
+        S.M[side][tri(p, S.cbias)] = S.tot[3 * side + 1];
+      }
+      S.M[side][tri(p, p)] = S.tot[3 * side + 2];
+    }
   }
   __syncthreads();
   bad = block_or(bad, S.ired);
@@ -213,12 +355,12 @@ __device__ void fit_sides(const SplitCtx& C, SplitShared& S, bool hard, double f
       __syncthreads();
       st = rows_cholesky(M, &S.piv[side], p, tid);
       if (st == 0) {
-        int nf = 0;
+        int nfin = 0;
         if (tid < p) {
-          for (int j = 0; j <= tid; ++j) nf |= !isfinite(M[tri(tid, j)]);
-          nf |= !isfinite(M[tri(p, tid)]);
+          for (int j = 0; j <= tid; ++j) nfin |= !isfinite(M[tri(tid, j)]);
+          nfin |= !isfinite(M[tri(p, tid)]);
         }
-        if (block_or(nf, S.ired)) st = -2;
+        if (block_or(nfin, S.ired)) st = -2;
       }
       if (st == 0) {
         x = rows_solve(M, S.bc, p, tid, tid < p ? M[tri(p, tid)] : 0.0);
@@ -234,100 +376,110 @@ __device__ void fit_sides(const SplitCtx& C, SplitShared& S, bool hard, double f
   }
 }
 
+// Predictions of both sides for one staged row.
+__device__ inline void side_predictions(const SplitShared& S, const double* xr, int p, double& pl,
+                                        double& pr) {
+  pl = 0.0;
+  pr = 0.0;
+  for (int f = 0; f < p; ++f) {
+    const double v = staged_feature(S, xr, f);
+    pl = fma(v, S.beta[0][f], pl);
+    pr = fma(v, S.beta[1][f], pr);
+  }
+}
+
 // Residual statistics at the current fit; then the tangent direction,
 // geometry and control packet (art/_cuda_optimizer_kernels.cu:108-195):
 // packet = [loss, |projected grad|, slope, n_right, lambda_L, lambda_R, bad].
-template <int T>
+// The gradient X^T c is formed per chunk from the per-row terms c in LDS,
+// a (column, row group) thread grid over the staged rows.
 __device__ void stats_and_direction(const SplitCtx& C, SplitShared& S) {
-  constexpr int PM = 16 * T;  // p + 1 <= 16 T, and d <= p
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int d = C.d, p = C.p, n = C.n;
+  const int tid = threadIdx.x;
+  const int d = C.d, p = C.p, n = C.n, sd = kXStride;
   const double z = S.theta[d];
   const double inv = 1.0 / (n * C.T);
-  double acc[PM + 3];
-#pragma unroll
-  for (int k = 0; k < PM + 3; ++k) acc[k] = 0.0;  // loss, gz, count, gw[0..d)
-  for (int r = tid; r < n; r += blockDim.x) {
-    const double* x = C.X + (size_t)r * d;
-    double phi[PM];
-    row_features<PM>(C.B, x, phi);
-    double pl = 0.0, pr = 0.0;
-#pragma unroll
-    for (int f = 0; f < PM; ++f)
-      if (f < p) {
-        pl += phi[f] * S.beta[0][f];
-        pr += phi[f] * S.beta[1][f];
+  const int gj = tid & 31, gg = tid >> 5, groups = (int)(blockDim.x >> 5);
+  double sums[3] = {0.0, 0.0, 0.0};  // loss, -sum c, count(pi >= 1/2)
+  double g = 0.0;                   // partial of (X^T c)[gj]
+  for_each_chunk(C, S, [&](int rows) {
+    if (tid < kChunk) {
+      double c = 0.0;
+      if (tid < rows) {
+        const double* xr = S.xs + tid * sd;
+        double pl, pr;
+        side_predictions(S, xr, p, pl, pr);
+        double u = 0.0;
+        for (int j = 0; j < d; ++j) u = fma(xr[j], S.theta[j], u);
+        const double pi = gate((u - z) / C.T);
+        const double yv = xr[d];
+        double l = yv - pl, rr = yv - pr;
+        l *= l;
+        rr *= rr;
+        c = (rr - l) * pi * (1.0 - pi) * inv;
+        sums[0] += (1.0 - pi) * l + pi * rr;
+        sums[1] -= c;
+        sums[2] += pi >= 0.5 ? 1.0 : 0.0;
       }
-    const double u = dotw(x, S.theta, d);
-    const double pi = gate((u - z) / C.T);
-    double l = C.y[r] - pl, rr = C.y[r] - pr;
-    l *= l;
-    rr *= rr;
-    const double c = (rr - l) * pi * (1.0 - pi) * inv;
-    acc[0] += ((1.0 - pi) * l + pi * rr) / n;
-    acc[1] -= c;
-    acc[2] += pi >= 0.5 ? 1.0 : 0.0;
-#pragma unroll
-    for (int j = 0; j < PM; ++j)
-      if (j < d) acc[3 + j] += c * x[j];
-  }
-  constexpr int W = kSplitMaxD + 4;
-#pragma unroll
-  for (int k = 0; k < PM + 3; ++k) {
-    if (k < 3 + d) {
-      const double v = wave_sum(acc[k]);
-      if (lane == 0) S.dred[wave * W + k] = v;
+      S.qs[0][tid] = c;
     }
-  }
+    __syncthreads();
+    if (gj < d)
+      for (int r = gg; r < rows; r += groups) g = fma(S.qs[0][r], S.xs[r * sd + gj], g);
+  });
+  block_sums<3>(sums, S);
+  const double loss = S.tot[0] / n, gz = S.tot[1], cnt = S.tot[2];
   __syncthreads();
-  if (tid < 3 + d)
-    S.tot[tid] = ((S.dred[tid] + S.dred[W + tid]) + S.dred[2 * W + tid]) + S.dred[3 * W + tid];
+  S.red[tid] = g;
+  __syncthreads();
+  if (tid < d) {
+    double s = 0.0;
+    for (int q = 0; q < groups; ++q) s += S.red[q * 32 + tid];
+    S.tot[3 + tid] = s;
+  }
   __syncthreads();
   if (tid == 0) {
     const double* w = S.theta;
-    const double* g = S.tot + 3;
+    const double* gw = S.tot + 3;
     double ww = 0.0, wg = 0.0;
     for (int j = 0; j < d; ++j) {
       ww += w[j] * w[j];
-      wg += w[j] * g[j];
+      wg += w[j] * gw[j];
     }
     const double proj = wg / ww;
     double pp = 0.0, wp = 0.0, gp = 0.0;
     for (int j = 0; j < d; ++j) {
-      const double v = -g[j] + proj * w[j];
+      const double v = -gw[j] + proj * w[j];
       S.pdir[j] = v;
       pp += v * v;
       wp += w[j] * v;
-      gp += g[j] * v;
+      gp += gw[j] * v;
     }
-    const double gzt = S.tot[1], norm = sqrt(pp + gzt * gzt);
-    S.pdir[d] = -gzt;
+    const double norm = sqrt(pp + gz * gz);
+    S.pdir[d] = -gz;
     S.geometry[0] = ww;
     S.geometry[1] = wp;
     S.geometry[2] = pp;
-    S.geometry[3] = -gzt;
-    S.packet[0] = S.tot[0];
+    S.geometry[3] = -gz;
+    S.packet[0] = loss;
     S.packet[1] = norm;
-    S.packet[2] = gp - gzt * gzt;
-    S.packet[3] = S.tot[2];
+    S.packet[2] = gp - gz * gz;
+    S.packet[3] = cnt;
     S.packet[4] = S.lam[0];
     S.packet[5] = S.lam[1];
     const bool failed = S.status[0] != 0 || S.status[1] != 0;
     S.packet[6] = (failed || !(ww > 0.0) || !isfinite(ww) || !isfinite(norm) ||
-                   !isfinite(S.packet[2]) || !isfinite(S.tot[0]))
+                   !isfinite(S.packet[2]) || !isfinite(loss))
                       ? 1.0
                       : 0.0;
   }
   __syncthreads();
 }
 
-// Frozen-model trial losses of `count` candidates alphas[0..count).
-template <int T>
+// Frozen-model trial losses of `count` (<= kSplitMaxH) candidates alphas[0..count).
 __device__ void trial_losses(const SplitCtx& C, SplitShared& S, const double* alphas, int count,
                              double* out) {
-  constexpr int PM = 16 * T;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int d = C.d, p = C.p, n = C.n;
+  const int tid = threadIdx.x;
+  const int d = C.d, p = C.p, n = C.n, sd = kXStride;
   const double z = S.theta[d], temp = C.T;
   const double g0 = S.geometry[0], g1 = S.geometry[1], g2 = S.geometry[2], g3 = S.geometry[3];
   double f[kSplitMaxH];
@@ -341,99 +493,74 @@ __device__ void trial_losses(const SplitCtx& C, SplitShared& S, const double* al
     okc[k] = (r2 > 0.0) && isfinite(r2) && (temp > 0.0) && isfinite(temp);
     rinv[k] = okc[k] ? 1.0 / sqrt(r2) : 0.0;
   }
-  for (int r = tid; r < n; r += blockDim.x) {
-    const double* x = C.X + (size_t)r * d;
-    double phi[PM];
-    row_features<PM>(C.B, x, phi);
-    double pl = 0.0, pr = 0.0;
-#pragma unroll
-    for (int ff = 0; ff < PM; ++ff)
-      if (ff < p) {
-        pl += phi[ff] * S.beta[0][ff];
-        pr += phi[ff] * S.beta[1][ff];
+  for_each_chunk(C, S, [&](int rows) {
+    if (tid < rows) {
+      const double* xr = S.xs + tid * sd;
+      double pl, pr;
+      side_predictions(S, xr, p, pl, pr);
+      const double yv = xr[d];
+      double l = yv - pl, rr = yv - pr;
+      l *= l;
+      rr *= rr;
+      double u = 0.0, v = 0.0;
+      for (int j = 0; j < d; ++j) {
+        u = fma(xr[j], S.theta[j], u);
+        v = fma(xr[j], S.pdir[j], v);
       }
-    double l = C.y[r] - pl, rr = C.y[r] - pr;
-    l *= l;
-    rr *= rr;
-    const double u = dotw(x, S.theta, d);
-    const double v = dotw(x, S.pdir, d);
 #pragma unroll
-    for (int k = 0; k < kSplitMaxH; ++k) {
-      if (k < count) {
-        const double al = alphas[k];
-        const double score = ((u + al * v) * rinv[k] - z - al * g3) / temp;
-        const double pi = gate(score);
-        const double fk = ((1.0 - pi) * l + pi * rr) / n;
-        f[k] += (okc[k] && isfinite(score) && isfinite(fk)) ? fk : INFINITY;
+      for (int k = 0; k < kSplitMaxH; ++k) {
+        if (k < count) {
+          const double al = alphas[k];
+          const double score = ((u + al * v) * rinv[k] - z - al * g3) / temp;
+          const double pi = gate(score);
+          const double fk = (1.0 - pi) * l + pi * rr;
+          f[k] += (okc[k] && isfinite(score) && isfinite(fk)) ? fk : INFINITY;
+        }
       }
     }
-  }
-#pragma unroll
-  for (int k = 0; k < kSplitMaxH; ++k) {
-    if (k < count) {
-      const double v = wave_sum(f[k]);
-      if (lane == 0) S.dred[wave * (kSplitMaxD + 4) + k] = v;
-    }
-  }
-  __syncthreads();
-  if (tid < count)
-    out[tid] = ((S.dred[tid] + S.dred[(kSplitMaxD + 4) + tid]) +
-                S.dred[2 * (kSplitMaxD + 4) + tid]) +
-               S.dred[3 * (kSplitMaxD + 4) + tid];
+  });
+  block_sums<kSplitMaxH>(f, S, count);
+  if (tid < count) out[tid] = S.tot[tid] / n;
   __syncthreads();
 }
 
 // Hard partition statistics after a hard fit (art/_cuda_batched_optimizer.cu:92-120):
 // [MSE, MSE_L, MSE_R, n_right, lambda_L, lambda_R, bad].
-template <int T>
 __device__ void hard_stats(const SplitCtx& C, SplitShared& S, bool refit, double* out,
                            bool all_right = false) {
-  constexpr int PM = 16 * T;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int d = C.d, p = C.p, n = C.n;
+  const int tid = threadIdx.x;
+  const int d = C.d, p = C.p, n = C.n, sd = kXStride;
   const double z = S.theta[d];
-  double sl = 0.0, sr = 0.0, cnt = 0.0;
+  double sums[3] = {0.0, 0.0, 0.0};  // SSE left, SSE right, count right
   int bad = 0;
-  for (int r = tid; r < n; r += blockDim.x) {
-    const double* x = C.X + (size_t)r * d;
-    const double u = dotw(x, S.theta, d);
-    const double pi = all_right ? 1.0
-                      : ((isfinite(u) && isfinite(z)) ? (double)(u >= z) : __builtin_nan(""));
-    bad |= !isfinite(pi);
-    const bool right = pi >= 0.5;
-    if (refit) {
-      double phi[PM];
-      row_features<PM>(C.B, x, phi);
-      double pred = 0.0;
-#pragma unroll
-      for (int f = 0; f < PM; ++f)
-        if (f < p) pred += phi[f] * S.beta[right ? 1 : 0][f];
-      const double e = C.y[r] - pred;
-      if (right) sr += e * e;
-      else sl += e * e;
+  for_each_chunk(C, S, [&](int rows) {
+    if (tid < rows) {
+      const double* xr = S.xs + tid * sd;
+      double u = 0.0;
+      for (int j = 0; j < d; ++j) u = fma(xr[j], S.theta[j], u);
+      const double pi = all_right ? 1.0
+                        : ((isfinite(u) && isfinite(z)) ? (double)(u >= z) : __builtin_nan(""));
+      bad |= !isfinite(pi);
+      const bool right = pi >= 0.5;
+      if (refit) {
+        const double* beta = S.beta[right ? 1 : 0];
+        double pred = 0.0;
+        for (int f = 0; f < p; ++f) pred = fma(staged_feature(S, xr, f), beta[f], pred);
+        const double e = xr[d] - pred;
+        sums[right ? 1 : 0] += e * e;
+      }
+      sums[2] += right ? 1.0 : 0.0;
     }
-    cnt += right ? 1.0 : 0.0;
-  }
-  {
-    const double v0 = wave_sum(sl), v1 = wave_sum(sr), v2 = wave_sum(cnt);
-    if (lane == 0) {
-      S.dred[wave * (kSplitMaxD + 4) + 0] = v0;
-      S.dred[wave * (kSplitMaxD + 4) + 1] = v1;
-      S.dred[wave * (kSplitMaxD + 4) + 2] = v2;
-    }
-  }
+  });
+  block_sums<3>(sums, S);
   bad = block_or(bad, S.ired);
-  __syncthreads();
   if (tid == 0) {
-    constexpr int W = kSplitMaxD + 4;
-    const double t[3] = {((S.dred[0] + S.dred[W]) + S.dred[2 * W]) + S.dred[3 * W],
-                         ((S.dred[1] + S.dred[W + 1]) + S.dred[2 * W + 1]) + S.dred[3 * W + 1],
-                         ((S.dred[2] + S.dred[W + 2]) + S.dred[2 * W + 2]) + S.dred[3 * W + 2]};
-    out[3] = t[2];
+    const double sl = S.tot[0], sr = S.tot[1], cnt = S.tot[2];
+    out[3] = cnt;
     if (refit) {
-      out[0] = (t[0] + t[1]) / n;
-      out[1] = t[2] < n ? t[0] / (n - t[2]) : 0.0;
-      out[2] = t[2] > 0 ? t[1] / t[2] : 0.0;
+      out[0] = (sl + sr) / n;
+      out[1] = cnt < n ? sl / (n - cnt) : 0.0;
+      out[2] = cnt > 0 ? sr / cnt : 0.0;
       out[4] = S.lam[0];
       out[5] = S.lam[1];
       out[6] = (S.status[0] != 0 || S.status[1] != 0 || !isfinite(out[0])) ? 1.0 : 0.0;
@@ -474,7 +601,7 @@ __global__ __launch_bounds__(256) void split_optimize_kernel(art_split_args args
   const int MI = a->max_iters;
   double* hist = a->history + (size_t)prob * 5 * (MI + 1);
   if (tid <= d) S.theta[tid] = a->theta0[(size_t)prob * (d + 1) + tid];
-  __syncthreads();
+  init_basis_tables(C.B, S);
   // Unit-normalize the start (art/splitters.py:71-83 is applied by the caller too).
   if (tid == 0) {
     double nw = 0.0;
@@ -490,7 +617,7 @@ __global__ __launch_bounds__(256) void split_optimize_kernel(art_split_args args
     // trace-scaled ridge); its MSE over all rows is the parent loss.
     fit_sides<T>(C, S, false, 0.0, true);
     double* hard = a->hard + (size_t)prob * 8;
-    hard_stats<T>(C, S, true, hard, true);
+    hard_stats(C, S, true, hard, true);
     if (tid < p) {
       a->beta_out[((size_t)prob * 2 + 0) * p + tid] = S.beta[1][tid];
       a->beta_out[((size_t)prob * 2 + 1) * p + tid] = S.beta[1][tid];
@@ -517,7 +644,7 @@ __global__ __launch_bounds__(256) void split_optimize_kernel(art_split_args args
     for (int k = 0; k < a->anneal_iters; ++k) {
       C.T = a->t0 * pow(a->t1 / a->t0, (double)k / (double)max(a->anneal_iters - 1, 1));
       fit_sides<T>(C, S, false, a->floor);
-      stats_and_direction<T>(C, S);
+      stats_and_direction(C, S);
       if (tid == 0 && k <= MI) hist[k] = S.packet[0];
       if (S.packet[6] != 0.0) {
         error = kCholeskyFailure;
@@ -535,7 +662,7 @@ __global__ __launch_bounds__(256) void split_optimize_kernel(art_split_args args
     C.T = a->temperature[node];
   } else {
     fit_sides<T>(C, S, false, a->floor);
-    stats_and_direction<T>(C, S);
+    stats_and_direction(C, S);
     const double g0 = S.packet[1];
     const double tol = a->grad_atol + a->grad_rtol * g0;
     if (tid == 0) {
@@ -591,7 +718,7 @@ __global__ __launch_bounds__(256) void split_optimize_kernel(art_split_args args
             }
             for (int k = 0; k < c2; ++k) acur *= a->rho;
             __syncthreads();
-            trial_losses<T>(C, S, alphas, c2, losses);
+            trial_losses(C, S, alphas, c2, losses);
             for (int k = 0; k < c2 && sel < 0; ++k) {
               value = losses[k];
               step = alphas[k];
@@ -643,7 +770,7 @@ __global__ __launch_bounds__(256) void split_optimize_kernel(art_split_args args
       }
       __syncthreads();
       fit_sides<T>(C, S, false, a->floor);
-      stats_and_direction<T>(C, S);
+      stats_and_direction(C, S);
       if (tid == 0) {
         hist[0 * (MI + 1) + it] = S.packet[0];
         hist[1 * (MI + 1) + it] = S.packet[1];
@@ -677,7 +804,7 @@ __global__ __launch_bounds__(256) void split_optimize_kernel(art_split_args args
     }
     __syncthreads();
     if (a->hard_refit) fit_sides<T>(C, S, true, 0.0);
-    hard_stats<T>(C, S, a->hard_refit != 0, hard);
+    hard_stats(C, S, a->hard_refit != 0, hard);
     if (tid == 0) {
       const double right2 = hard[3];
       if (!a->hard_refit && hard[6] != 0.0) error = kNonfiniteSplit;
@@ -729,13 +856,11 @@ extern "C" int art_split_optimize(const art_split_args* args, int problems, hipS
       !args->theta_out || !args->beta_out || !args->hard || !args->info || !args->history ||
       !args->scalars)
     return fail(ART_ERR_ARG, "art_split_optimize: null device pointer");
-  const int T = (p + 1 + 15) / 16;
+  // Gram tiles cover the non-constant features only.
+  const int nf = p - (args->basis.bias ? 1 : 0);
   const dim3 grid(problems), block(256);
-  switch (T) {
-    case 1: split_optimize_kernel<1><<<grid, block, 0, stream>>>(*args); break;
-    case 2: split_optimize_kernel<2><<<grid, block, 0, stream>>>(*args); break;
-    default: split_optimize_kernel<3><<<grid, block, 0, stream>>>(*args); break;
-  }
+  if (nf <= 16) split_optimize_kernel<1><<<grid, block, 0, stream>>>(*args);
+  else split_optimize_kernel<2><<<grid, block, 0, stream>>>(*args);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? ART_OK : hip_fail(e, "split_optimize_kernel launch");
 }

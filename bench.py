@@ -38,10 +38,12 @@ def parse():
     ap.add_argument("--leaves", type=int, default=4096)
     ap.add_argument("--chunk-rows", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--only", choices=("all", "ridge", "sampler"), default="all",
+    ap.add_argument("--only", choices=("all", "ridge", "sampler", "split"), default="all",
                     help="run one leg (profiling); the JSON line keeps the ridge metric")
     ap.add_argument("--sampler-steps", type=int, default=3)
     ap.add_argument("--polytopes", type=int, default=65536)
+    ap.add_argument("--split-nodes", type=int, default=64,
+                    help="nodes of the annealed split leg (64 restarts each)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after timing, run a plain 1 GiB device copy three times: the "
                          "calibration kernel for FETCH_SIZE / WRITE_SIZE counter runs")
@@ -203,6 +205,66 @@ def bench_sampler(args):
     return result
 
 
+def bench_split(args):
+    """Annealed split leg: nodes of n=8192 points in d=16 (8-bump target), 64
+    random restarts each, affine children (p=17, ridge 1e-4), T from 0.2 to
+    0.01 over 200 iterations, step 0.05; one launch runs every (node, restart)
+    problem and its hard refit."""
+    import torch
+
+    from art import AffineRidgeModel, BatchedOptimizerWorkspace, SoftObliqueSplitter
+    from art.synthetic import split_workload
+
+    K, R, iters = args.split_nodes, 64, 200
+    X, y = split_workload(K, seed=3)
+    s = SoftObliqueSplitter(AffineRidgeModel(1e-4), 0.2, n_restarts=R, random_state=3)
+    ws = BatchedOptimizerWorkspace(s, X, y)
+    starts = np.stack([[ws._random_theta(j) for j in range(K)] for _ in range(R)])
+    times = []
+    res = ws.anneal(starts, iters=iters, timing=times)
+    ms = times[0]
+    n, d, p = X.shape[1], X.shape[2], ws.p
+    # Algorithmic FP64 flops per point, problem and iteration: two weighted
+    # Grams and right-hand sides (2 x 2 (p+1)^2 / 2 symmetric) plus residuals,
+    # gradient and gates (2 x 2p + 2d + 20).
+    per_point = 2 * (p + 1) * (p + 1) + 4 * p + 2 * d + 20
+    flops = float(K) * R * iters * n * per_point
+    print(f"[bench] split {ms:.1f} ms for {K} nodes x {R} restarts", file=sys.stderr, flush=True)
+    result = {
+        "metric": "annealed node splits/s", "value": K / (ms * 1e-3), "unit": "node splits/s",
+        "ms_per_launch": ms, "problems": K * R,
+        "dtype": "f64 (MFMA child Grams, FP64 Cholesky)",
+        "config": {"workload": (f"{K} nodes x 64 restarts, n=8192, d=16, affine p=17, "
+                                "ridge 1e-4, T 0.2->0.01 over 200 iterations, step 0.05, seed 3"),
+                   "nodes": K, "restarts": R, "iterations": iters},
+        "failed_problems": int((~res["ok"]).sum()),
+        "roofline": {"bound": "mfma", "achieved": flops / (ms * 1e-3) / 1e12,
+                     "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
+                     "frac": flops / (ms * 1e-3) / 1e12 / PEAK_TFLOPS["fp64"],
+                     "algorithmic_flops_per_launch": flops,
+                     "flops_per_point_iteration": per_point,
+                     "kernel": "split_optimize_kernel<2>"},
+    }
+    if not args.no_cpu:
+        from threadpoolctl import threadpool_limits
+
+        from oracle import split as O
+
+        so = O.Settings(ridge=1e-4, degree=1, bias_last=True)
+        k = 2
+        with threadpool_limits(1):
+            t0 = time.perf_counter()
+            for q in range(k):
+                O.annealed(X[q], y[q], starts[q, 0], so, iters=iters, t0=0.2, t1=0.01, step=0.05)
+            dt = time.perf_counter() - t0
+        result["cpu_baseline"] = {"value": k / dt / R, "unit": "node splits/s", "cores": 1,
+                                  "kind": "port",
+                                  "sample": (f"{k} (node, restart) problems, NumPy FP64 annealed "
+                                             f"loop, {dt:.2f} s; node splits/s = problems/s / 64")}
+    ws.close()
+    return result
+
+
 def rel_err(a, b):
     return np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-300)
 
@@ -223,6 +285,9 @@ def main():
     sys.path.insert(0, str(ROOT))
     if args.only == "sampler":
         print(json.dumps(bench_sampler(args)))
+        return
+    if args.only == "split":
+        print(json.dumps(bench_split(args)))
         return
     wl = leaf_ridge_workload(n_leaves=args.leaves, seed=2)
     dev = torch.device("cuda", 0)

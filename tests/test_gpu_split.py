@@ -316,3 +316,31 @@ def test_splitter_split_matches_oracle_for_one_node(torch_gpu):
     check_against_oracle(result, s, X, y, theta0, .15)
     check_hard_models(result, s, X, y)
     assert result.split_gain > 0
+
+
+def test_annealed_mode_matches_the_oracle(torch_gpu):
+    from art import AffineRidgeModel, BatchedOptimizerWorkspace, SoftObliqueSplitter
+
+    X, y, theta = problems()
+    s = SoftObliqueSplitter(AffineRidgeModel(1e-4), .7, n_restarts=2, min_side_points=2)
+    starts = np.stack([theta, np.tile([.6, .8, .1], (3, 1))])
+    with BatchedOptimizerWorkspace(s, X, y) as ws:
+        res = ws.anneal(starts, t0=.5, t1=.05, iters=25, step=.3)
+    so = settings_of(s, .7)
+    for r in range(2):
+        for j in range(3):
+            th, losses = O.annealed(X[j], y[j], starts[r, j], so, iters=25, t0=.5, t1=.05,
+                                    step=.3)
+            np.testing.assert_allclose(res["theta"][r, j], O.project_unit_w(th), rtol=1e-9,
+                                       atol=1e-10)
+            np.testing.assert_allclose(res["soft_loss_history"][r, j], losses, rtol=1e-9,
+                                       atol=1e-12)
+            try:
+                h = O.hard_result(X[j], y[j], th, so)
+            except ValueError:
+                assert not res["ok"][r, j]
+                continue
+            assert res["ok"][r, j]
+            assert res["loss"][r, j] == pytest.approx(h["loss"], rel=1e-9, abs=1e-14)
+            np.testing.assert_allclose(res["beta"][r, j], h["beta"], rtol=1e-9, atol=1e-12)
+    assert res["best"].shape == (3,)
