@@ -88,6 +88,7 @@ _PROTOTYPES = {
     "art_plan_timing_read": (_i, [_p, _p, _p, _i]),
     "art_plan_destroy": (None, [_p]),
     "art_split_optimize": (_i, [ctypes.POINTER(SplitArgs), _i, _p]),
+    "art_split_anneal_batched": (_i, [ctypes.POINTER(SplitArgs), _i, _p]),
     "art_har_sample": (_i, [ctypes.POINTER(HarArgs), _p]),
     "art_philox4x32_10": (_i, [_p, _p, _i64, _p, _p]),
 }

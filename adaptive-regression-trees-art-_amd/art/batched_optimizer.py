@@ -175,7 +175,8 @@ class BatchedOptimizerWorkspace:
         w = w / nrm
         return np.concatenate([w, [float(np.median(self.host_X[node] @ w))]])
 
-    def _launch(self, nodes, theta0, mode, fallback, hard_refit, anneal=None, timing=None):
+    def _launch(self, nodes, theta0, mode, fallback, hard_refit, anneal=None, timing=None,
+                batched=False):
         """One device launch over problems (nodes[i], theta0[i]); host copies out.
 
         anneal = (iters, t0, t1, step) for mode 1; timing, a list, receives
@@ -217,8 +218,9 @@ class BatchedOptimizerWorkspace:
             if timing is not None:
                 ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
                 ev[0].record(self.stream)
-            _hip.check(self.lib.art_split_optimize(ctypes.byref(a), P,
-                                                   _hip.stream_handle(self.stream)))
+            entry = (self.lib.art_split_anneal_batched if batched
+                     else self.lib.art_split_optimize)
+            _hip.check(entry(ctypes.byref(a), P, _hip.stream_handle(self.stream)))
             if timing is not None:
                 ev[1].record(self.stream)
         self.last_launch_problems = P
@@ -314,7 +316,7 @@ class BatchedOptimizerWorkspace:
         return results
 
     def anneal(self, theta0=None, *, nodes=None, t0=0.2, t1=0.01, iters=200, step=0.05,
-               timing=None):
+               timing=None, batched=None):
         """Annealed fixed-step projected gradient from every start, then a hard refit.
 
         theta0: (r, d+1) or (R, r, d+1) starts for the r selected nodes, or
@@ -325,6 +327,10 @@ class BatchedOptimizerWorkspace:
         host arrays per (restart, node): theta (R, r, d+1), beta (R, r, 2, p),
         hard loss (R, r), n_right (R, r), ok (R, r), soft-loss history
         (R, r, iters), and best (r,): the restart with the lowest hard loss.
+
+        batched (default: when d <= 16 and at most 16 non-constant features)
+        runs 16 starts of a node per workgroup with complementary Grams
+        (art_split_anneal_batched); otherwise one workgroup per start.
         """
         self._check_open()
         if not self.k:
@@ -343,16 +349,35 @@ class BatchedOptimizerWorkspace:
                 raise ValueError("theta0 must be finite with shape (r,d+1) or (R,r,d+1).")
             starts = np.stack([[project_unit_w(t) for t in rs] for rs in starts])
         Rr = starts.shape[0]
-        nodes_all = np.tile(ids, Rr)
-        theta, beta, hard, info, hist, _ = self._launch(
-            nodes_all, starts.reshape(-1, self.d + 1), 1, False, True,
-            anneal=(iters, t0, t1, step), timing=timing)
+        nf = self.p - (1 if self.bias else 0)
+        eligible = self.d <= 16 and nf <= 16
+        if batched is None:
+            batched = eligible
+        if batched and not eligible:
+            raise ValueError("The batched annealer needs d <= 16 and at most 16 features.")
+        if batched:
+            # Node-major blocks of 16 starts; a node's last block repeats its
+            # last start (results of the copies are dropped).
+            B = -(-Rr // 16) * 16
+            order = np.minimum(np.arange(B), Rr - 1)
+            nodes_all = np.repeat(ids, B)
+            thetas = starts.transpose(1, 0, 2)[:, order].reshape(-1, self.d + 1)
+            outs = self._launch(nodes_all, thetas, 1, False, True,
+                                anneal=(iters, t0, t1, step), timing=timing, batched=True)
+            keep = (np.arange(count)[:, None] * B + np.arange(Rr)[None, :]).T.reshape(-1)
+            theta, beta, hard, info, hist, _ = (o[keep] for o in outs)
+        else:
+            nodes_all = np.tile(ids, Rr)
+            theta, beta, hard, info, hist, _ = self._launch(
+                nodes_all, starts.reshape(-1, self.d + 1), 1, False, True,
+                anneal=(iters, t0, t1, step), timing=timing)
         shape = (Rr, count)
         ok = info[:, 6].reshape(shape) == 0
         loss = np.where(ok, hard[:, 0].reshape(shape), np.inf)
         return {"theta": theta.reshape(shape + (self.d + 1,)),
                 "beta": beta.reshape(shape + (2, self.p)), "loss": loss,
                 "n_right": hard[:, 3].reshape(shape).astype(int), "ok": ok,
+                "info": info.reshape(shape + (8,)),
                 "soft_loss_history": hist[:, 0, :iters].reshape(shape + (iters,)),
                 "best": np.argmin(loss, axis=0), "starts": starts}
 

@@ -42,7 +42,7 @@ def parse():
                     help="run one leg (profiling); the JSON line keeps the ridge metric")
     ap.add_argument("--sampler-steps", type=int, default=3)
     ap.add_argument("--polytopes", type=int, default=65536)
-    ap.add_argument("--split-nodes", type=int, default=64,
+    ap.add_argument("--split-nodes", type=int, default=256,
                     help="nodes of the annealed split leg (64 restarts each)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after timing, run a plain 1 GiB device copy three times: the "
@@ -234,6 +234,8 @@ def bench_split(args):
         "metric": "annealed node splits/s", "value": K / (ms * 1e-3), "unit": "node splits/s",
         "ms_per_launch": ms, "problems": K * R,
         "dtype": "f64 (MFMA child Grams, FP64 Cholesky)",
+        "error_codes": {str(k): int(v) for k, v in
+                        zip(*np.unique(res["info"][..., 0], return_counts=True))},
         "config": {"workload": (f"{K} nodes x 64 restarts, n=8192, d=16, affine p=17, "
                                 "ridge 1e-4, T 0.2->0.01 over 200 iterations, step 0.05, seed 3"),
                    "nodes": K, "restarts": R, "iterations": iters},
@@ -243,7 +245,7 @@ def bench_split(args):
                      "frac": flops / (ms * 1e-3) / 1e12 / PEAK_TFLOPS["fp64"],
                      "algorithmic_flops_per_launch": flops,
                      "flops_per_point_iteration": per_point,
-                     "kernel": "split_optimize_kernel<2>"},
+                     "kernel": "anneal_kernel (16 starts per workgroup)"},
     }
     if not args.no_cpu:
         from threadpoolctl import threadpool_limits

@@ -176,6 +176,17 @@ typedef struct {
  * SoftObliqueSplitter._optimize_from_initialization (art/splitters.py:365-690). */
 int art_split_optimize(const art_split_args* args, int problems, hipStream_t stream);
 
+/* Annealed mode only (mode and Armijo fields ignored), restart-batched:
+ * problems come in blocks of 16 consecutive problems that share one node
+ * (node[16b] is the block's node; pad a node's starts with duplicates), one
+ * workgroup per block streams the node's rows once per pass for all 16
+ * starts.  The left-side systems are the node's unweighted system minus the
+ * right-side ones (complementary Grams, as in art/_cuda_cholesky.py:56-89),
+ * so sides differ from two independently floored fits by at most the weight
+ * floor.  Needs d <= 16 and at most 16 non-constant features.  Same outputs
+ * as art_split_optimize mode 1 (history row 0 = soft loss per iteration). */
+int art_split_anneal_batched(const art_split_args* args, int problems, hipStream_t stream);
+
 /* Frees a plan's device buffers (synchronizes the device).
  * Replaces BatchedCholeskyWorkspace.close (art/_cuda_batched_cholesky.py:339-352). */
 void art_plan_destroy(art_ridge_plan* plan);

@@ -318,14 +318,15 @@ def test_splitter_split_matches_oracle_for_one_node(torch_gpu):
     assert result.split_gain > 0
 
 
-def test_annealed_mode_matches_the_oracle(torch_gpu):
+@pytest.mark.parametrize("batched", [False, True])
+def test_annealed_mode_matches_the_oracle(torch_gpu, batched):
     from art import AffineRidgeModel, BatchedOptimizerWorkspace, SoftObliqueSplitter
 
     X, y, theta = problems()
     s = SoftObliqueSplitter(AffineRidgeModel(1e-4), .7, n_restarts=2, min_side_points=2)
     starts = np.stack([theta, np.tile([.6, .8, .1], (3, 1))])
     with BatchedOptimizerWorkspace(s, X, y) as ws:
-        res = ws.anneal(starts, t0=.5, t1=.05, iters=25, step=.3)
+        res = ws.anneal(starts, t0=.5, t1=.05, iters=25, step=.3, batched=batched)
     so = settings_of(s, .7)
     for r in range(2):
         for j in range(3):
@@ -344,3 +345,33 @@ def test_annealed_mode_matches_the_oracle(torch_gpu):
             assert res["loss"][r, j] == pytest.approx(h["loss"], rel=1e-9, abs=1e-14)
             np.testing.assert_allclose(res["beta"][r, j], h["beta"], rtol=1e-9, atol=1e-12)
     assert res["best"].shape == (3,)
+
+
+@pytest.mark.parametrize("degree", [1, 2])
+def test_batched_annealer_matches_the_per_start_kernel(torch_gpu, degree):
+    # 20 starts per node (two blocks of 16, the second padded), d = 5
+    # affine or d = 3 quadratic, uneven row count (300).
+    from art import (AffineRidgeModel, BatchedOptimizerWorkspace, PolynomialRidgeModel,
+                     SoftObliqueSplitter)
+
+    rng = np.random.default_rng(71)
+    d = 5 if degree == 1 else 3
+    X = rng.uniform(-1, 1, (3, 300, d))
+    y = np.sin(2 * X[..., 0]) + X[..., 1] * X[..., -1]
+    model = AffineRidgeModel(1e-4) if degree == 1 else PolynomialRidgeModel(ridge=1e-4)
+    s = SoftObliqueSplitter(model, .3, n_restarts=20, min_side_points=3, random_state=5)
+    with BatchedOptimizerWorkspace(s, X, y) as ws:
+        starts = np.stack([[ws._random_theta(j) for j in range(3)] for _ in range(20)])
+        one = ws.anneal(starts, t0=.3, t1=.03, iters=30, step=.2, batched=False)
+        two = ws.anneal(starts, t0=.3, t1=.03, iters=30, step=.2, batched=True)
+    np.testing.assert_array_equal(one["ok"], two["ok"])
+    np.testing.assert_allclose(two["theta"], one["theta"], rtol=1e-9, atol=1e-10)
+    np.testing.assert_allclose(two["soft_loss_history"], one["soft_loss_history"], rtol=1e-9,
+                               atol=1e-12)
+    ok = one["ok"]
+    np.testing.assert_allclose(two["loss"][ok], one["loss"][ok], rtol=1e-9, atol=1e-13)
+    np.testing.assert_allclose(two["beta"][ok], one["beta"][ok], rtol=1e-8, atol=1e-10)
+    np.testing.assert_array_equal(two["n_right"], one["n_right"])
+    so = settings_of(s, .3)
+    th, losses = O.annealed(X[1], y[1], starts[7, 1], so, iters=30, t0=.3, t1=.03, step=.2)
+    np.testing.assert_allclose(two["theta"][7, 1], O.project_unit_w(th), rtol=1e-9, atol=1e-10)
