@@ -464,6 +464,8 @@ __device__ void anneal_hard_stats(const AnnealCtx& C, AnnealShared& S) {
   __syncthreads();
 }
 
+}  // namespace
+
 __global__ __launch_bounds__(kAThreads) void anneal_kernel(art_split_args args) {
   extern __shared__ __align__(16) unsigned char anneal_smem[];
   AnnealShared& S = *reinterpret_cast<AnnealShared*>(anneal_smem);
@@ -589,8 +591,6 @@ __global__ __launch_bounds__(kAThreads) void anneal_kernel(art_split_args args) 
     }
   }
 }
-
-}  // namespace
 
 }  // namespace art
 
