@@ -84,6 +84,7 @@ _PROTOTYPES = {
     "art_plan_info": (_i, [_p, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "art_leaf_fallback_count": (_i, [_p, ctypes.POINTER(_i)]),
     "art_leaf_set_refine_tol": (_i, [_p, _d]),
+    "art_leaf_set_refine_steps": (_i, [_p, _i]),
     "art_plan_timing_enable": (_i, [_p, _i]),
     "art_plan_timing_read": (_i, [_p, _p, _p, _i]),
     "art_plan_destroy": (None, [_p]),

@@ -105,6 +105,10 @@ class LeafRidgeBatch:
         _hip.check(self.lib.art_leaf_fallback_count(self.plan, ctypes.byref(count)))
         return count.value
 
+    def set_refine_steps(self, steps):
+        """Refinement steps after the first, on leaves still flagged (default 1)."""
+        _hip.check(self.lib.art_leaf_set_refine_steps(self.plan, int(steps)))
+
     def set_refine_tol(self, tol):
         """Acceptance tolerance |delta| <= tol |beta| of the mixed refinement."""
         _hip.check(self.lib.art_leaf_set_refine_tol(self.plan, float(tol)))

@@ -835,10 +835,10 @@ template <int T>
 __global__ __launch_bounds__(256) void residual_leaf_kernel(
     const double* __restrict__ X, const double* __restrict__ y, const double* __restrict__ w,
     Basis B, double floor, const LeafChunk* __restrict__ chunks, const double* __restrict__ beta,
-    const int* __restrict__ status, double* __restrict__ rpart) {
+    const int* __restrict__ status, double* __restrict__ rpart, const int* __restrict__ mask) {
   const int c = blockIdx.x;
   const int leaf = chunks[c].leaf;
-  if (status[leaf] != 0) return;
+  if (status[leaf] != 0 || (mask && !mask[leaf])) return;
   extern __shared__ double smem[];
   __shared__ double rred[4][16 * T];
   const int d = B.d, p = B.p, rb = stage_rows_per_block(d), bsz = stage_buffer_doubles(d);
@@ -939,13 +939,13 @@ __global__ __launch_bounds__(256) void residual_quad_kernel(
     const double* __restrict__ X, const double* __restrict__ y, const double* __restrict__ w,
     Basis B, double floor, const LeafChunk* __restrict__ rchunks,
     const double* __restrict__ beta, const int* __restrict__ status,
-    double* __restrict__ rpart) {
+    double* __restrict__ rpart, const int* __restrict__ mask) {
   constexpr int NQ = D * (D + 1) / 2;
   constexpr int NF = 1 + D + NQ;  // [bias, linear, quadratic]
   static_assert(NF <= 64, "reduce-scatter holds one value per lane");
   const int c = blockIdx.x;
   const int leaf = rchunks[c].leaf;
-  if (status[leaf] != 0) return;
+  if (status[leaf] != 0 || (mask && !mask[leaf])) return;
   __shared__ double bs[NF];
   __shared__ double red[4][64];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -995,18 +995,24 @@ __global__ __launch_bounds__(256) void residual_quad_kernel(
 }
 
 // One refinement step per leaf with the stored factor; flags leaves that
-// need the exact FP64 path.
+// need more.  The first step (mask == nullptr) accepts |delta| <= tau |beta|:
+// with the low-precision Gram's contraction rho ~ |delta| / |beta| the error
+// left is ~ tau^2 |beta|.  Later steps run on the flagged leaves (mask) and
+// accept when the step contracts (rho = |delta_k| / |delta_{k-1}| < 1/2) and
+// the error left, ~ rho |delta_k| / (1 - rho), is at most tau^2 |beta|.
+// dnorm holds |delta|^2 of the last step per leaf.
 __global__ __launch_bounds__(128) void correct_rows_kernel(
     const SystemDesc* __restrict__ systems, const int* __restrict__ roff, int p, int rstride,
     const double* __restrict__ rpart,
     const double* __restrict__ factor, const double* __restrict__ lambda,
     double* __restrict__ beta, const int* __restrict__ status, int* __restrict__ need,
-    double tau) {
+    double tau, const int* __restrict__ mask, double* __restrict__ dnorm) {
   extern __shared__ double M[];
   __shared__ double bc[2];
   __shared__ double red[2];
   const SystemDesc S = systems[blockIdx.x];
   const int leaf = S.out, i = threadIdx.x, nt = blockDim.x;
+  if (mask && !mask[leaf]) return;
   if (status[leaf] != 0) {
     if (i == 0) need[leaf] = 1;
     return;
@@ -1026,7 +1032,17 @@ __global__ __launch_bounds__(128) void correct_rows_kernel(
   const double nd = block_sum(i < p ? delta * delta : 0.0, red);
   const double nb = block_sum(i < p ? b1 * b1 : 0.0, red);
   if (i < p) beta[(size_t)leaf * p + i] = b1;
-  if (i == 0) need[leaf] = !(nd <= tau * tau * nb);
+  if (i == 0) {
+    if (!mask) {
+      need[leaf] = !(nd <= tau * tau * nb);
+    } else {
+      const double prev = dnorm[leaf];
+      const double rho = prev > 0.0 ? sqrt(nd / prev) : 1.0;
+      const double left = rho < 1.0 ? rho * sqrt(nd) / (1.0 - rho) : INFINITY;
+      need[leaf] = !(rho < 0.5 && left <= tau * tau * sqrt(nb));
+    }
+    dnorm[leaf] = nd;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1125,6 +1141,8 @@ struct art_ridge_plan {
   LeafChunk* d_rchunks = nullptr;
   int nrchunks = 0;
   double refine_tol = ART_MIXED_REFINE_TOL;
+  int refine_steps = ART_MIXED_REFINE_STEPS;  // extra steps on flagged leaves
+  double* d_dnorm = nullptr;
   // optional per-fit kernel timing: 3 events per slot (before Gram, after
   // Gram, after solve), recorded on the fit's stream
   int timing_slots = 0;
@@ -1198,6 +1216,7 @@ static void plan_free(art_ridge_plan* P) {
   (void)hipFree(P->d_rpart);
   (void)hipFree(P->d_tpart);
   (void)hipFree(P->d_need);
+  (void)hipFree(P->d_dnorm);
   (void)hipFree(P->d_goff);
   (void)hipFree(P->d_roff);
   (void)hipFree(P->d_rchunks);
@@ -1393,6 +1412,7 @@ static int leaf_fit_mixed(art_ridge_plan* P, const double* X, const double* y, c
     ART_HIP(hipMalloc(&P->d_rpart, rp * sizeof(double)));
     ART_HIP(hipMalloc(&P->d_tpart, (size_t)P->nchunks * sizeof(double)));
     ART_HIP(hipMalloc(&P->d_need, (size_t)P->K * sizeof(int)));
+    ART_HIP(hipMalloc(&P->d_dnorm, (size_t)P->K * sizeof(double)));
   }
   const size_t smem = leaf_stage_smem_bytes(P->basis.d);
   const dim3 cgrid(P->nchunks), sgrid(P->nsys), block(256);
@@ -1419,41 +1439,46 @@ static int leaf_fit_mixed(art_ridge_plan* P, const double* X, const double* y, c
   int rc = launch_solve(P, P->nsys, P->nchunks, ridge, beta, status, lambda, stream, nullptr,
                         P->d_factor, P->d_tpart);
   if (rc != ART_OK) return rc;
-  // Residual: the register kernel for quadratic bases with bias-first or no
-  // bias (2 <= d <= 9), the feature-layout kernel otherwise.
+  // Refinement: one FP64 residual pass and correction for every leaf, then up
+  // to refine_steps more on the leaves still flagged.  The residual uses the
+  // register kernel for quadratic bases with bias-first or no bias
+  // (2 <= d <= 9), the feature-layout kernel otherwise.
   const bool quad = P->basis.degree == 2 && !P->basis.bias_last && P->basis.d >= 2 &&
                     P->basis.d <= 9;
-  if (quad) {
-    const dim3 rgrid(P->nrchunks);
-    switch (P->basis.d) {
+  for (int step = 0; step <= P->refine_steps; ++step) {
+    const int* mask = step == 0 ? nullptr : P->d_need;
+    if (quad) {
+      const dim3 rgrid(P->nrchunks);
+      switch (P->basis.d) {
 #define ART_CASE(DD)                                                                     \
   case DD:                                                                               \
     residual_quad_kernel<DD><<<rgrid, block, 0, stream>>>(X, y, w, P->basis, floor,       \
                                                           P->d_rchunks, beta, status,    \
-                                                          P->d_rpart);                   \
+                                                          P->d_rpart, mask);             \
     break;
-      ART_CASE(2) ART_CASE(3) ART_CASE(4) ART_CASE(5) ART_CASE(6) ART_CASE(7) ART_CASE(8)
-      ART_CASE(9)
+        ART_CASE(2) ART_CASE(3) ART_CASE(4) ART_CASE(5) ART_CASE(6) ART_CASE(7) ART_CASE(8)
+        ART_CASE(9)
 #undef ART_CASE
-    }
-  } else {
-    switch (T) {
+      }
+    } else {
+      switch (T) {
 #define ART_CASE(TT)                                                                         \
   case TT:                                                                                   \
     residual_leaf_kernel<TT><<<cgrid, block, smem, stream>>>(X, y, w, P->basis, floor,       \
                                                              P->d_chunks, beta, status,      \
-                                                             P->d_rpart);                    \
+                                                             P->d_rpart, mask);              \
     break;
-      ART_CASE(1) ART_CASE(2) ART_CASE(3) ART_CASE(4) ART_CASE(5)
+        ART_CASE(1) ART_CASE(2) ART_CASE(3) ART_CASE(4) ART_CASE(5)
 #undef ART_CASE
+      }
     }
+    ART_HIP(hipGetLastError());
+    correct_rows_kernel<<<sgrid, dim3(64 * ((p + 63) / 64)),
+                          (size_t)p * (p + 1) / 2 * sizeof(double), stream>>>(
+        P->d_sys, quad ? P->d_roff : P->d_goff, p, quad ? p : 16 * T, P->d_rpart, P->d_factor,
+        lambda, beta, status, P->d_need, P->refine_tol, mask, P->d_dnorm);
+    ART_HIP(hipGetLastError());
   }
-  ART_HIP(hipGetLastError());
-  correct_rows_kernel<<<sgrid, dim3(64 * ((p + 63) / 64)), (size_t)p * (p + 1) / 2 * sizeof(double),
-                        stream>>>(P->d_sys, quad ? P->d_roff : P->d_goff, p, quad ? p : 16 * T,
-                                  P->d_rpart, P->d_factor, lambda, beta, status, P->d_need,
-                                  P->refine_tol);
-  ART_HIP(hipGetLastError());
   // Exact FP64 path for the flagged leaves only.
   launch_gram_leaf_f64(P, X, y, w, floor, P->d_need, stream);
   ART_HIP(hipGetLastError());
@@ -1555,6 +1580,14 @@ int art_leaf_fallback_count(art_ridge_plan* P, int* count) {
   std::vector<int> need(P->K);
   ART_HIP(hipMemcpy(need.data(), P->d_need, P->K * sizeof(int), hipMemcpyDeviceToHost));
   for (int v : need) *count += v != 0;
+  return ART_OK;
+}
+
+int art_leaf_set_refine_steps(art_ridge_plan* P, int steps) {
+  if (!P || P->kind != 0) return fail(ART_ERR_ARG, "art_leaf_set_refine_steps: not a leaf plan");
+  if (steps < 0 || steps > 16)
+    return fail(ART_ERR_ARG, "art_leaf_set_refine_steps: steps must be in [0, 16]");
+  P->refine_steps = steps;
   return ART_OK;
 }
 

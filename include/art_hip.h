@@ -32,6 +32,7 @@ extern "C" {
 
 #define ART_DEFAULT_CHUNK_ROWS 2048
 #define ART_MIXED_REFINE_TOL 1e-4 /* accept |delta| <= tol*|beta| after one refinement */
+#define ART_MIXED_REFINE_STEPS 1  /* further refinement steps on still-flagged leaves */
 #define ART_SOLVE_LDS_BYTES 65536
 
 /* Feature basis of the leaf models (art/_cuda_features.py:13-29): degree 1
@@ -97,6 +98,13 @@ int art_leaf_fallback_count(art_ridge_plan* plan, int* count);
 
 /* Acceptance tolerance of the mixed refinement (default ART_MIXED_REFINE_TOL). */
 int art_leaf_set_refine_tol(art_ridge_plan* plan, double tol);
+
+/* Refinement steps after the first, run only on leaves the previous step
+ * flagged (default ART_MIXED_REFINE_STEPS, 0..16).  A later step accepts a
+ * leaf when its correction contracts (rho < 1/2) and the estimated error
+ * left, rho |delta| / (1 - rho), is at most tol^2 |beta|; leaves still
+ * flagged take the exact FP64 path.  No reference counterpart. */
+int art_leaf_set_refine_steps(art_ridge_plan* plan, int steps);
 
 /* Equal-size gated nodes (the BatchedCholeskyWorkspace contract): capacity K
  * nodes of n rows and p features.  Allocates device buffers (synchronous).
