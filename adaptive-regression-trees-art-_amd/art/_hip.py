@@ -90,6 +90,7 @@ _PROTOTYPES = {
     "art_plan_destroy": (None, [_p]),
     "art_split_optimize": (_i, [ctypes.POINTER(SplitArgs), _i, _p]),
     "art_split_anneal_batched": (_i, [ctypes.POINTER(SplitArgs), _i, _p]),
+    "art_split_init": (_i, [_p, _i, _i, _p, _i, ctypes.c_uint64, ctypes.c_uint64, _p, _p]),
     "art_har_sample": (_i, [ctypes.POINTER(HarArgs), _p]),
     "art_philox4x32_10": (_i, [_p, _p, _i64, _p, _p]),
 }

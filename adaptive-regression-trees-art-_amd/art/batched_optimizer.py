@@ -8,8 +8,9 @@ and metadata keys.  Where the reference runs a host loop with one
 device->host transfer per iteration and per backtrack batch, this workspace
 makes one launch of ``art_split_optimize`` for every (node, restart) problem
 of the call and downloads the results once.  Random starts (Gaussian unit w,
-z = median(X w), art/splitters.py:692-700) are drawn on the host from the
-splitter's RNG.
+z = median(X w), art/splitters.py:692-700) are drawn on the device
+(``art_split_init``: Philox4x32-10 normals seeded from the splitter's RNG,
+radix-select median).
 """
 
 from __future__ import annotations
@@ -111,7 +112,7 @@ class BatchedOptimizerWorkspace:
         self.s, self.capacity, self.k = splitter, self.K, 0
         self.reorder = bool(reorder)
         self.seed = int(splitter._rng().integers(0, np.iinfo(np.int64).max))
-        self._init_rng = np.random.default_rng(self.seed)
+        self._init_calls = 0
         self.required = splitter._required_side_points(self.n)
         self.degree, self.bias, self.bias_last = model_basis(splitter.model_template)
         self.p = feature_count(self.d, self.degree, self.bias)
@@ -166,14 +167,22 @@ class BatchedOptimizerWorkspace:
         self.fit_sizes.clear()
         return self
 
-    def _random_theta(self, node):
-        """Gaussian unit w and z = median(X w) (art/splitters.py:692-700)."""
-        w = self._init_rng.normal(size=self.d)
-        nrm = np.linalg.norm(w)
-        if nrm <= 1e-12:
-            w[0], nrm = 1.0, 1.0
-        w = w / nrm
-        return np.concatenate([w, [float(np.median(self.host_X[node] @ w))]])
+    def _device_starts(self, nodes):
+        """Random starts (Gaussian unit w, z = median(X w)) for node slots, drawn
+        on the device (art_split_init); each call advances the counter."""
+        torch, P = self.torch, len(nodes)
+        if P == 0:
+            return np.empty((0, self.d + 1))
+        node_t = torch.as_tensor(np.ascontiguousarray(nodes, dtype=np.int32), device=self.device)
+        out = torch.empty((P, self.d + 1), dtype=torch.float64, device=self.device)
+        with torch.cuda.stream(self.stream):
+            _hip.check(self.lib.art_split_init(
+                _hip.ptr(self.x), self.n, self.d, _hip.ptr(node_t), P,
+                self.seed & 0xFFFFFFFFFFFFFFFF, self._init_calls, _hip.ptr(out),
+                _hip.stream_handle(self.stream)))
+        self._init_calls += 1
+        self.stream.synchronize()
+        return out.cpu().numpy()
 
     def _launch(self, nodes, theta0, mode, fallback, hard_refit, anneal=None, timing=None,
                 batched=False):
@@ -271,14 +280,16 @@ class BatchedOptimizerWorkspace:
             for i, j in enumerate(ids):
                 if not parent_ok[i]:
                     continue
-                if initial is not None and (r == 0 or initial.ndim == 3):
-                    start = project_unit_w(initial[i] if initial.ndim == 2 else initial[r, i])
-                else:
-                    start = self._random_theta(j)
-                prob_random.append(initial is None or (r > 0 and initial.ndim == 2))
+                random = initial is None or (r > 0 and initial.ndim == 2)
+                start = None if random else project_unit_w(
+                    initial[i] if initial.ndim == 2 else initial[r, i])
+                prob_random.append(random)
                 prob_node.append(j)
                 prob_theta.append(start)
                 prob_slot.append((r, i))
+        draw = [q for q, rnd in enumerate(prob_random) if rnd]
+        for q, start in zip(draw, self._device_starts([prob_node[q] for q in draw])):
+            prob_theta[q] = start
         results = [None] * count
         for i in range(count):
             if not parent_ok[i]:
@@ -320,7 +331,8 @@ class BatchedOptimizerWorkspace:
         """Annealed fixed-step projected gradient from every start, then a hard refit.
 
         theta0: (r, d+1) or (R, r, d+1) starts for the r selected nodes, or
-        None for random starts (Gaussian unit w, z = median(X w)).  Iteration
+        None for random starts drawn on the device (Gaussian unit w,
+        z = median(X w)).  Iteration
         k uses T_k = t0 (t1/t0)^(k/(iters-1)) and
         theta <- retract(theta - step * P(grad)).  No reference counterpart
         (the reference optimizes at a fixed temperature).  Returns a dict of
@@ -340,7 +352,7 @@ class BatchedOptimizerWorkspace:
         if iters < 1 or not (t0 > 0 and t1 > 0 and np.isfinite(step)):
             raise ValueError("Need iters >= 1, positive temperatures and a finite step.")
         if theta0 is None:
-            starts = np.stack([[self._random_theta(j) for j in ids] for _ in range(R)])
+            starts = self._device_starts(np.tile(ids, R)).reshape(R, count, self.d + 1)
         else:
             starts = np.asarray(theta0, dtype=float)
             if starts.ndim == 2:
@@ -413,7 +425,7 @@ class BatchedOptimizerWorkspace:
             backtrack_history=[int(v) for v in hist[q, 3, :it]],
             line_search_loss_history=[float(v) for v in hist[q, 4, :it]],
             optimizer="cupy_batched_cholesky", device_optimizer="hip_persistent",
-            reorder=self.reorder, initialization="host" if random_init else "provided",
+            reorder=self.reorder, initialization="gpu" if random_init else "provided",
             initialization_seed=self.seed if random_init else None, gpu_solver="cholesky",
             solver_fallback=False, condition_estimation=False, complementary_grams=False,
             backtrack_batch_size=self.H, Xw_refresh=s.Xw_refresh,

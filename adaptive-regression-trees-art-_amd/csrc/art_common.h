@@ -13,6 +13,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kWave = 64;
+constexpr int kSplitInitMaxD = 32;  // widest node input of the random-start kernel
 
 // Host-side error reporting shared by every entry point: the message is
 // returned by art_last_error() on the calling thread.

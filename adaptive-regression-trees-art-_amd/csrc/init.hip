@@ -1,0 +1,149 @@
+// Random split starts on the device: w = a unit Gaussian direction, z = the
+// median of X w over the node's rows (art/splitters.py:692-700 on the CPU;
+// art/_cuda_batched_optimizer.cu:25-45 with cuRAND normals and cp.median on
+// the reference's GPU path).
+//
+// Directions: FP64 Box-Muller pairs from Philox4x32-10 (Salmon et al., SC'11),
+// key (seed_lo, seed_hi), counter (pair i, problem, counter_lo, counter_hi);
+// the call's four words give u1 = ((w0 >> 5) 2^26 + (w1 >> 6) + 1) 2^-53 in
+// (0, 1] and u2 = ((w2 >> 5) 2^26 + (w3 >> 6)) 2^-53 in [0, 1), and
+// (r cos 2 pi u2, r sin 2 pi u2) with r = sqrt(-2 ln u1) are normals 2i, 2i+1.
+// A direction of norm <= 1e-12 keeps its components with w_0 = 1, as the
+// reference's unit_vector does.
+// Median: radix select on order-preserving 64-bit keys of u = x.w (8 passes
+// of 8 bits, LDS histograms); an even row count averages the two middle
+// values, as numpy.median does.
+
+#include "art_common.h"
+#include "art_hip.h"
+
+namespace art {
+
+__device__ __forceinline__ uint4 init_philox(uint4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint32_t lo0 = 0xD2511F53u * c.x;
+    const uint32_t hi0 = __umulhi(0xD2511F53u, c.x);
+    const uint32_t lo1 = 0xCD9E8D57u * c.z;
+    const uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z);
+    c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+  }
+  return c;
+}
+
+__device__ inline uint64_t order_key(double v) {
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+__device__ inline double key_value(uint64_t k) {
+  const uint64_t b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+  return __longlong_as_double((long long)b);
+}
+
+struct InitShared {
+  double w[kSplitInitMaxD];
+  unsigned hist[256];
+  unsigned long long prefix, mask;
+  unsigned rank;
+};
+
+__device__ inline double row_score(const double* x, const double* w, int d) {
+  double u = 0.0;
+  for (int j = 0; j < d; ++j) u = fma(x[j], w[j], u);
+  return u;
+}
+
+// The rank-th smallest (0-based) of x.w over the node's rows.
+__device__ double select_score(const double* X, int n, int d, InitShared& S, unsigned rank) {
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    S.prefix = 0ull;
+    S.mask = 0ull;
+    S.rank = rank;
+  }
+  for (int shift = 56; shift >= 0; shift -= 8) {
+    __syncthreads();
+    for (int b = tid; b < 256; b += blockDim.x) S.hist[b] = 0u;
+    __syncthreads();
+    const unsigned long long prefix = S.prefix, mask = S.mask;
+    for (int r = tid; r < n; r += blockDim.x) {
+      const uint64_t key = order_key(row_score(X + (size_t)r * d, S.w, d));
+      if ((key & mask) == prefix) atomicAdd(&S.hist[(key >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      unsigned k = S.rank, b = 0;
+      for (; b < 255u; ++b) {
+        if (k < S.hist[b]) break;
+        k -= S.hist[b];
+      }
+      S.rank = k;
+      S.prefix = prefix | ((unsigned long long)b << shift);
+      S.mask = mask | (255ull << shift);
+    }
+  }
+  __syncthreads();
+  return key_value(S.prefix);
+}
+
+__global__ __launch_bounds__(256) void split_init_kernel(const double* __restrict__ X, int n, int d,
+                                                         const int* __restrict__ node,
+                                                         uint64_t seed, uint64_t counter,
+                                                         double* __restrict__ theta) {
+  __shared__ InitShared S;
+  const int q = blockIdx.x, tid = threadIdx.x;
+  const double* Xn = X + (size_t)node[q] * n * d;
+  const int pairs = (d + 1) / 2;
+  if (tid < pairs) {
+    const uint4 r = init_philox(make_uint4((uint32_t)tid, (uint32_t)q, (uint32_t)counter,
+                                           (uint32_t)(counter >> 32)),
+                                (uint32_t)seed, (uint32_t)(seed >> 32));
+    const double u1 = ((double)(r.x >> 5) * 67108864.0 + (double)(r.y >> 6) + 1.0) * 0x1p-53;
+    const double u2 = ((double)(r.z >> 5) * 67108864.0 + (double)(r.w >> 6)) * 0x1p-53;
+    const double rad = sqrt(-2.0 * log(u1));
+    double s, c;
+    sincospi(2.0 * u2, &s, &c);
+    S.w[2 * tid] = rad * c;
+    if (2 * tid + 1 < d) S.w[2 * tid + 1] = rad * s;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double nn = 0.0;
+    for (int j = 0; j < d; ++j) nn += S.w[j] * S.w[j];
+    const double r = sqrt(nn);
+    if (r <= 1e-12) S.w[0] = 1.0;
+    else
+      for (int j = 0; j < d; ++j) S.w[j] /= r;
+  }
+  __syncthreads();
+  double z;
+  if (n & 1) {
+    z = select_score(Xn, n, d, S, (unsigned)(n / 2));
+  } else {
+    const double lo = select_score(Xn, n, d, S, (unsigned)(n / 2 - 1));
+    const double hi = select_score(Xn, n, d, S, (unsigned)(n / 2));
+    z = (lo + hi) * 0.5;
+  }
+  if (tid < d) theta[(size_t)q * (d + 1) + tid] = S.w[tid];
+  if (tid == 0) theta[(size_t)q * (d + 1) + d] = z;
+}
+
+}  // namespace art
+
+using namespace art;
+
+extern "C" int art_split_init(const double* X, int n, int d, const int* node, int problems,
+                              uint64_t seed, uint64_t counter, double* theta,
+                              hipStream_t stream) {
+  if (!X || !node || !theta || n < 1 || d < 1 || d > kSplitInitMaxD || problems < 0)
+    return fail(ART_ERR_ARG, "art_split_init: bad arguments (1 <= d <= 32, n >= 1)");
+  if (problems == 0) return ART_OK;
+  split_init_kernel<<<problems, 256, 0, stream>>>(X, n, d, node, seed, counter, theta);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? ART_OK : hip_fail(e, "split_init_kernel launch");
+}

@@ -219,9 +219,9 @@ def bench_split(args):
     X, y = split_workload(K, seed=3)
     s = SoftObliqueSplitter(AffineRidgeModel(1e-4), 0.2, n_restarts=R, random_state=3)
     ws = BatchedOptimizerWorkspace(s, X, y)
-    starts = np.stack([[ws._random_theta(j) for j in range(K)] for _ in range(R)])
     times = []
-    res = ws.anneal(starts, iters=iters, timing=times)
+    res = ws.anneal(None, iters=iters, timing=times)
+    starts = res["starts"]
     ms = times[0]
     n, d, p = X.shape[1], X.shape[2], ws.p
     # Algorithmic FP64 flops per point, problem and iteration: two weighted

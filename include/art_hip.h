@@ -195,6 +195,16 @@ int art_split_optimize(const art_split_args* args, int problems, hipStream_t str
  * as art_split_optimize mode 1 (history row 0 = soft loss per iteration). */
 int art_split_anneal_batched(const art_split_args* args, int problems, hipStream_t stream);
 
+/* Random starts theta (problems, d+1) for nodes node[q] of X (K, n, d):
+ * w = unit Gaussian direction (FP64 Box-Muller from Philox4x32-10, key seed,
+ * counter (pair, q, counter_lo, counter_hi)), z = median over the node's rows
+ * of x.w (mean of the two middle values for even n).  Replaces the
+ * reference's cuRAND initialization kernels initial_directions /
+ * initial_scores / initial_offsets with cp.median
+ * (art/_cuda_batched_optimizer.cu:25-45, art/_cuda_batched_optimizer.py:286-297). */
+int art_split_init(const double* X, int n, int d, const int* node, int problems, uint64_t seed,
+                   uint64_t counter, double* theta, hipStream_t stream);
+
 /* Frees a plan's device buffers (synchronizes the device).
  * Replaces BatchedCholeskyWorkspace.close (art/_cuda_batched_cholesky.py:339-352). */
 void art_plan_destroy(art_ridge_plan* plan);

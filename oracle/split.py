@@ -11,7 +11,12 @@ Restates, for one node and one start theta0:
   art/splitters.py:400-690;
 * the annealed fixed-step mode (temperature T_k = T0 (T1/T0)^(k/(K-1)), step
   eta along the negative projected gradient, unit-norm retraction), which has
-  no reference counterpart (parity unpinned beyond its own closed forms).
+  no reference counterpart (parity unpinned beyond its own closed forms);
+* the device random start (art/splitters.py:692-700 with the device's own
+  stream): FP64 Box-Muller from Philox4x32-10 words, w = normal / |normal|
+  (w_0 = 1 when the norm is <= 1e-12, art/_cuda_batched_optimizer.cu:7-23),
+  z = numpy.median(X w).  Pinned by the Philox known answers
+  (tests/test_oracle_sampling.py), numpy's median and the Gaussian moments.
 
 Pinned by (tests/test_oracle_split.py): central finite differences of the
 soft loss (gradient), the tangent-projection identities (w . p_w = 0 on the
@@ -337,4 +342,39 @@ def initial_theta(X, w):
     if nrm <= 1e-12:
         w[0], nrm = 1.0, 1.0
     w = w / nrm
+    return np.concatenate([w, [float(np.median(X @ w))]])
+
+
+def device_start(X, seed, q, counter):
+    """Random start of problem q of an art_split_init call (see module doc).
+
+    Pair i uses Philox4x32-10 counter (i, q, counter_lo, counter_hi), key
+    (seed_lo, seed_hi): u1 = ((w0 >> 5) 2^26 + (w1 >> 6) + 1) 2^-53,
+    u2 = ((w2 >> 5) 2^26 + (w3 >> 6)) 2^-53, normals r cos(2 pi u2),
+    r sin(2 pi u2), r = sqrt(-2 ln u1).
+    """
+    from .sampling import philox4x32_10
+
+    X = np.asarray(X, dtype=float)
+    d = X.shape[1]
+    pairs = (d + 1) // 2
+    ctr = np.array([[i, q, counter & 0xFFFFFFFF, (counter >> 32) & 0xFFFFFFFF]
+                    for i in range(pairs)], dtype=np.uint64).astype(np.uint32)
+    key = np.tile(np.array([seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF],
+                           dtype=np.uint64).astype(np.uint32), (pairs, 1))
+    w4 = philox4x32_10(ctr, key).astype(np.uint64)
+    u1 = ((w4[:, 0] >> np.uint64(5)).astype(float) * 2.0 ** 26
+          + (w4[:, 1] >> np.uint64(6)).astype(float) + 1.0) * 2.0 ** -53
+    u2 = ((w4[:, 2] >> np.uint64(5)).astype(float) * 2.0 ** 26
+          + (w4[:, 3] >> np.uint64(6)).astype(float)) * 2.0 ** -53
+    r = np.sqrt(-2.0 * np.log(u1))
+    normals = np.empty(2 * pairs)
+    normals[0::2] = r * np.cos(2.0 * np.pi * u2)
+    normals[1::2] = r * np.sin(2.0 * np.pi * u2)
+    w = normals[:d].copy()
+    nrm = np.linalg.norm(w)
+    if nrm <= 1e-12:
+        w[0] = 1.0
+    else:
+        w /= nrm
     return np.concatenate([w, [float(np.median(X @ w))]])

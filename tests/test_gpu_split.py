@@ -361,7 +361,7 @@ def test_batched_annealer_matches_the_per_start_kernel(torch_gpu, degree):
     model = AffineRidgeModel(1e-4) if degree == 1 else PolynomialRidgeModel(ridge=1e-4)
     s = SoftObliqueSplitter(model, .3, n_restarts=20, min_side_points=3, random_state=5)
     with BatchedOptimizerWorkspace(s, X, y) as ws:
-        starts = np.stack([[ws._random_theta(j) for j in range(3)] for _ in range(20)])
+        starts = ws._device_starts(np.tile(np.arange(3), 20)).reshape(20, 3, d + 1)
         one = ws.anneal(starts, t0=.3, t1=.03, iters=30, step=.2, batched=False)
         two = ws.anneal(starts, t0=.3, t1=.03, iters=30, step=.2, batched=True)
     np.testing.assert_array_equal(one["ok"], two["ok"])
@@ -375,3 +375,47 @@ def test_batched_annealer_matches_the_per_start_kernel(torch_gpu, degree):
     so = settings_of(s, .3)
     th, losses = O.annealed(X[1], y[1], starts[7, 1], so, iters=30, t0=.3, t1=.03, step=.2)
     np.testing.assert_allclose(two["theta"][7, 1], O.project_unit_w(th), rtol=1e-9, atol=1e-10)
+
+
+@pytest.mark.parametrize("n,d", [(8192, 16), (301, 3), (1, 2), (64, 31)])
+def test_device_random_starts_match_the_oracle(torch_gpu, n, d):
+    from art import AffineRidgeModel, BatchedOptimizerWorkspace, SoftObliqueSplitter
+
+    rng = np.random.default_rng(n + d)
+    X = rng.uniform(-1, 1, (2, n, d))
+    y = X[..., 0] ** 2
+    s = SoftObliqueSplitter(AffineRidgeModel(1e-4), .5, random_state=11)
+    with BatchedOptimizerWorkspace(s, X, y) as ws:
+        nodes = np.array([0, 1, 1, 0, 1])
+        first = ws._device_starts(nodes)
+        second = ws._device_starts(nodes)
+        for q, j in enumerate(nodes):
+            ref = O.device_start(X[j], ws.seed, q, 0)
+            np.testing.assert_allclose(first[q], ref, rtol=1e-12, atol=1e-13)
+            np.testing.assert_allclose(second[q], O.device_start(X[j], ws.seed, q, 1),
+                                       rtol=1e-12, atol=1e-13)
+        assert not np.allclose(first, second)
+        np.testing.assert_allclose(np.linalg.norm(first[:, :d], axis=1), 1.0, rtol=1e-14)
+        many = ws._device_starts(np.zeros(4000, dtype=int))
+    # Isotropy: unit directions have E[w_j^2] = 1/d.
+    np.testing.assert_allclose((many[:, :d] ** 2).mean(axis=0), 1.0 / d, atol=6 / np.sqrt(4000) / d * 3)
+    # The offset is a median: at most half the rows on either side.
+    u = X[0] @ many[0, :d]
+    assert np.count_nonzero(u < many[0, d]) <= n // 2 and np.count_nonzero(u > many[0, d]) <= n // 2
+
+
+def test_random_start_metadata_reports_device_initialization(torch_gpu):
+    from art import AffineRidgeModel, BatchedOptimizerWorkspace, SoftObliqueSplitter
+
+    X, y, _ = problems()
+    s = SoftObliqueSplitter(AffineRidgeModel(1e-4), .7, max_iters=3, min_side_points=2,
+                            random_state=28)
+    with BatchedOptimizerWorkspace(s, X, y) as ws:
+        out = ws.optimize()
+    for j, r in enumerate(out):
+        assert r.metadata["initialization"] == "gpu"
+        assert r.metadata["initialization_seed"] == ws.seed
+        start = r.metadata["initial_theta"]
+        np.testing.assert_allclose(start, O.device_start(X[j], ws.seed, j, 0), rtol=1e-12,
+                                   atol=1e-13)
+        check_against_oracle(r, s, X[j], y[j], start, .7)

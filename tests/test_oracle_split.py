@@ -249,3 +249,16 @@ def test_annealed_mode_decreases_the_loss_on_a_clean_split():
     theta, losses = O.annealed(X, y, np.array([.6, .8, .3]), s, iters=60, t0=.3, t1=.05, step=.2)
     assert losses[-1] < losses[0]
     assert abs(np.linalg.norm(theta[:-1]) - 1) < 1e-14
+
+
+def test_device_start_is_a_unit_gaussian_direction_and_a_median():
+    rng = np.random.default_rng(5)
+    X = rng.uniform(-1, 1, (101, 4))
+    starts = np.array([O.device_start(X, 77, q, 3) for q in range(3000)])
+    np.testing.assert_allclose(np.linalg.norm(starts[:, :4], axis=1), 1.0, rtol=1e-14)
+    np.testing.assert_allclose((starts[:, :4] ** 2).mean(axis=0), .25, atol=.02)
+    np.testing.assert_allclose(starts[:, :4].mean(axis=0), 0.0, atol=.05)
+    for t in starts[:20]:
+        u = X @ t[:4]
+        assert np.count_nonzero(u < t[4]) == 50 and np.count_nonzero(u > t[4]) == 50
+    assert not np.array_equal(O.device_start(X, 77, 0, 3), O.device_start(X, 77, 0, 4))
