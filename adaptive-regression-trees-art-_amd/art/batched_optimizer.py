@@ -341,8 +341,9 @@ class BatchedOptimizerWorkspace:
         (R, r, iters), and best (r,): the restart with the lowest hard loss.
 
         batched (default: when d <= 16 and at most 16 non-constant features)
-        runs 16 starts of a node per workgroup with complementary Grams
-        (art_split_anneal_batched); otherwise one workgroup per start.
+        runs 16 starts of a node per workgroup, the node's rows staged once
+        per pass for all of them (art_split_anneal_batched); otherwise one
+        workgroup per start.
         """
         self._check_open()
         if not self.k:

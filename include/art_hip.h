@@ -188,11 +188,12 @@ int art_split_optimize(const art_split_args* args, int problems, hipStream_t str
  * problems come in blocks of 16 consecutive problems that share one node
  * (node[16b] is the block's node; pad a node's starts with duplicates), one
  * workgroup per block streams the node's rows once per pass for all 16
- * starts.  The left-side systems are the node's unweighted system minus the
- * right-side ones (complementary Grams, as in art/_cuda_cholesky.py:56-89),
- * so sides differ from two independently floored fits by at most the weight
- * floor.  Needs d <= 16 and at most 16 non-constant features.  Same outputs
- * as art_split_optimize mode 1 (history row 0 = soft loss per iteration). */
+ * starts.  Each side's system is its own floored weighted Gram
+ * (max(1 - pi, floor) and max(pi, floor)), as in the two weighted fits of
+ * art/objectives.py:237-303, so the results equal art_split_optimize mode 1
+ * per start.  Needs d <= 16 and at most 16 non-constant features.  Same
+ * outputs as art_split_optimize mode 1 (history row 0 = soft loss per
+ * iteration). */
 int art_split_anneal_batched(const art_split_args* args, int problems, hipStream_t stream);
 
 /* Random starts theta (problems, d+1) for nodes node[q] of X (K, n, d):
@@ -201,7 +202,7 @@ int art_split_anneal_batched(const art_split_args* args, int problems, hipStream
  * of x.w (mean of the two middle values for even n).  Replaces the
  * reference's cuRAND initialization kernels initial_directions /
  * initial_scores / initial_offsets with cp.median
- * (art/_cuda_batched_optimizer.cu:25-45, art/_cuda_batched_optimizer.py:286-297). */
+ * (art/_cuda_batched_optimizer.cu:25-45, art/_cuda_batched_optimizer.py:291-305). */
 int art_split_init(const double* X, int n, int d, const int* node, int problems, uint64_t seed,
                    uint64_t counter, double* theta, hipStream_t stream);
 
@@ -262,7 +263,7 @@ int art_har_sample(const art_har_args* args, hipStream_t stream);
  * keys (count, 2), out (count, 4) uint32 device arrays.  The random stream
  * of art_har_sample is defined in terms of this function.  It replaces the
  * cuRAND Philox generator the reference uses for its random starts
- * (art/_cuda_batched_optimizer.py:151-156,286-297). */
+ * (art/_cuda_batched_optimizer.py:158-160,291-305). */
 int art_philox4x32_10(const uint32_t* counters, const uint32_t* keys, int64_t count,
                       uint32_t* out, hipStream_t stream);
 
