@@ -167,21 +167,30 @@ class BatchedOptimizerWorkspace:
         self.fit_sizes.clear()
         return self
 
-    def _device_starts(self, nodes):
+    def _device_starts(self, nodes, timing=None):
         """Random starts (Gaussian unit w, z = median(X w)) for node slots, drawn
-        on the device (art_split_init); each call advances the counter."""
+        on the device (art_split_init); each call advances the counter.
+        timing, a list, receives the kernel's milliseconds (HIP events on the
+        workspace stream)."""
         torch, P = self.torch, len(nodes)
         if P == 0:
             return np.empty((0, self.d + 1))
         node_t = torch.as_tensor(np.ascontiguousarray(nodes, dtype=np.int32), device=self.device)
         out = torch.empty((P, self.d + 1), dtype=torch.float64, device=self.device)
         with torch.cuda.stream(self.stream):
+            if timing is not None:
+                ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+                ev[0].record(self.stream)
             _hip.check(self.lib.art_split_init(
                 _hip.ptr(self.x), self.n, self.d, _hip.ptr(node_t), P,
                 self.seed & 0xFFFFFFFFFFFFFFFF, self._init_calls, _hip.ptr(out),
                 _hip.stream_handle(self.stream)))
+            if timing is not None:
+                ev[1].record(self.stream)
         self._init_calls += 1
         self.stream.synchronize()
+        if timing is not None:
+            timing.append(ev[0].elapsed_time(ev[1]))
         return out.cpu().numpy()
 
     def _launch(self, nodes, theta0, mode, fallback, hard_refit, anneal=None, timing=None,
@@ -339,6 +348,8 @@ class BatchedOptimizerWorkspace:
         host arrays per (restart, node): theta (R, r, d+1), beta (R, r, 2, p),
         hard loss (R, r), n_right (R, r), ok (R, r), soft-loss history
         (R, r, iters), and best (r,): the restart with the lowest hard loss.
+        timing, a list, receives the milliseconds of each device launch: the
+        random starts (when theta0 is None), then the annealer.
 
         batched (default: when d <= 16 and at most 16 non-constant features)
         runs 16 starts of a node per workgroup, the node's rows staged once
@@ -353,7 +364,7 @@ class BatchedOptimizerWorkspace:
         if iters < 1 or not (t0 > 0 and t1 > 0 and np.isfinite(step)):
             raise ValueError("Need iters >= 1, positive temperatures and a finite step.")
         if theta0 is None:
-            starts = self._device_starts(np.tile(ids, R)).reshape(R, count, self.d + 1)
+            starts = self._device_starts(np.tile(ids, R), timing).reshape(R, count, self.d + 1)
         else:
             starts = np.asarray(theta0, dtype=float)
             if starts.ndim == 2:

@@ -11,8 +11,11 @@
 // A direction of norm <= 1e-12 keeps its components with w_0 = 1, as the
 // reference's unit_vector does.
 // Median: radix select on order-preserving 64-bit keys of u = x.w (8 passes
-// of 8 bits, LDS histograms); an even row count averages the two middle
-// values, as numpy.median does.
+// of 8 bits, LDS histograms, the bin found by a wave scan).  A node of at
+// most kInitLdsRows rows scores each row once into LDS; a larger node
+// recomputes x.w on every pass.  An even row count averages the two middle
+// values, as numpy.median does: the upper one is the lower one again when
+// its key repeats, else the smallest key above it (one more sweep).
 
 #include "art_common.h"
 #include "art_hip.h"
@@ -45,11 +48,14 @@ __device__ inline double key_value(uint64_t k) {
   return __longlong_as_double((long long)b);
 }
 
+constexpr int kInitThreads = 512;
+constexpr int kInitLdsRows = 8192;  // 64 KB of keys
+
 struct InitShared {
   double w[kSplitInitMaxD];
   unsigned hist[256];
-  unsigned long long prefix, mask;
-  unsigned rank;
+  unsigned long long prefix, mask, above;
+  unsigned rank, bin_count;
 };
 
 __device__ inline double row_score(const double* x, const double* w, int d) {
@@ -58,8 +64,21 @@ __device__ inline double row_score(const double* x, const double* w, int d) {
   return u;
 }
 
-// The rank-th smallest (0-based) of x.w over the node's rows.
-__device__ double select_score(const double* X, int n, int d, InitShared& S, unsigned rank) {
+// Key of row r: from LDS when the node's keys are staged there, else x_r.w.
+struct RowKeys {
+  const double* X;
+  const unsigned long long* lds;
+  const double* w;
+  int d;
+  __device__ unsigned long long operator()(int r) const {
+    return lds ? lds[r] : order_key(row_score(X + (size_t)r * d, w, d));
+  }
+};
+
+// Radix select of the rank-th smallest key (0-based).  On return S.prefix is
+// that key, S.rank its rank among the equal keys and S.bin_count how many
+// keys equal it.
+__device__ void select_key(const RowKeys& keys, int n, InitShared& S, unsigned rank) {
   const int tid = threadIdx.x;
   if (tid == 0) {
     S.prefix = 0ull;
@@ -72,30 +91,64 @@ __device__ double select_score(const double* X, int n, int d, InitShared& S, uns
     __syncthreads();
     const unsigned long long prefix = S.prefix, mask = S.mask;
     for (int r = tid; r < n; r += blockDim.x) {
-      const uint64_t key = order_key(row_score(X + (size_t)r * d, S.w, d));
+      const unsigned long long key = keys(r);
       if ((key & mask) == prefix) atomicAdd(&S.hist[(key >> shift) & 255u], 1u);
     }
     __syncthreads();
-    if (tid == 0) {
-      unsigned k = S.rank, b = 0;
-      for (; b < 255u; ++b) {
-        if (k < S.hist[b]) break;
-        k -= S.hist[b];
+    if (tid < 64) {
+      // Lane l owns bins 4l..4l+3; an inclusive scan of the lane sums finds
+      // the one lane whose bins hold the rank.
+      const unsigned k = S.rank;
+      const unsigned h0 = S.hist[4 * tid], h1 = S.hist[4 * tid + 1], h2 = S.hist[4 * tid + 2],
+                     h3 = S.hist[4 * tid + 3];
+      const unsigned own = h0 + h1 + h2 + h3;
+      unsigned inc = own;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const unsigned v = __shfl_up(inc, off, 64);
+        if (tid >= off) inc += v;
       }
-      S.rank = k;
-      S.prefix = prefix | ((unsigned long long)b << shift);
-      S.mask = mask | (255ull << shift);
+      const unsigned excl = inc - own;
+      if (k >= excl && k < inc) {
+        unsigned kk = k - excl, b = 4u * tid, c = h0;
+        if (kk >= h0) {
+          kk -= h0; ++b; c = h1;
+          if (kk >= h1) {
+            kk -= h1; ++b; c = h2;
+            if (kk >= h2) { kk -= h2; ++b; c = h3; }
+          }
+        }
+        S.rank = kk;
+        S.bin_count = c;
+        S.prefix = prefix | ((unsigned long long)b << shift);
+        S.mask = mask | (255ull << shift);
+      }
     }
   }
   __syncthreads();
-  return key_value(S.prefix);
 }
 
-__global__ __launch_bounds__(256) void split_init_kernel(const double* __restrict__ X, int n, int d,
-                                                         const int* __restrict__ node,
-                                                         uint64_t seed, uint64_t counter,
-                                                         double* __restrict__ theta) {
+// Smallest key strictly above `key` (all rows are scanned).
+__device__ unsigned long long key_above(const RowKeys& keys, int n, InitShared& S,
+                                        unsigned long long key) {
+  const int tid = threadIdx.x;
+  if (tid == 0) S.above = ~0ull;
+  __syncthreads();
+  unsigned long long best = ~0ull;
+  for (int r = tid; r < n; r += blockDim.x) {
+    const unsigned long long k = keys(r);
+    if (k > key && k < best) best = k;
+  }
+  if (best != ~0ull) atomicMin(&S.above, best);
+  __syncthreads();
+  return S.above;
+}
+
+__global__ __launch_bounds__(kInitThreads) void split_init_kernel(
+    const double* __restrict__ X, int n, int d, const int* __restrict__ node, uint64_t seed,
+    uint64_t counter, double* __restrict__ theta) {
   __shared__ InitShared S;
+  extern __shared__ unsigned long long init_keys[];
   const int q = blockIdx.x, tid = threadIdx.x;
   const double* Xn = X + (size_t)node[q] * n * d;
   const int pairs = (d + 1) / 2;
@@ -121,13 +174,22 @@ __global__ __launch_bounds__(256) void split_init_kernel(const double* __restric
       for (int j = 0; j < d; ++j) S.w[j] /= r;
   }
   __syncthreads();
-  double z;
-  if (n & 1) {
-    z = select_score(Xn, n, d, S, (unsigned)(n / 2));
-  } else {
-    const double lo = select_score(Xn, n, d, S, (unsigned)(n / 2 - 1));
-    const double hi = select_score(Xn, n, d, S, (unsigned)(n / 2));
-    z = (lo + hi) * 0.5;
+  const bool staged = n <= kInitLdsRows;
+  if (staged) {
+    for (int r = tid; r < n; r += blockDim.x)
+      init_keys[r] = order_key(row_score(Xn + (size_t)r * d, S.w, d));
+  }
+  const RowKeys keys{Xn, staged ? init_keys : nullptr, S.w, d};
+  // select_key starts with a barrier, which also publishes the staged keys.
+  select_key(keys, n, S, (unsigned)((n - 1) / 2));
+  const unsigned long long lo = S.prefix;
+  double z = key_value(lo);
+  if (!(n & 1)) {
+    const bool repeats = S.rank + 1 < S.bin_count;
+    // Every thread reads S before key_above's first barrier and S.above is
+    // only written after it.
+    const unsigned long long hi = repeats ? lo : key_above(keys, n, S, lo);
+    z = (z + key_value(hi)) * 0.5;
   }
   if (tid < d) theta[(size_t)q * (d + 1) + tid] = S.w[tid];
   if (tid == 0) theta[(size_t)q * (d + 1) + d] = z;
@@ -143,7 +205,12 @@ extern "C" int art_split_init(const double* X, int n, int d, const int* node, in
   if (!X || !node || !theta || n < 1 || d < 1 || d > kSplitInitMaxD || problems < 0)
     return fail(ART_ERR_ARG, "art_split_init: bad arguments (1 <= d <= 32, n >= 1)");
   if (problems == 0) return ART_OK;
-  split_init_kernel<<<problems, 256, 0, stream>>>(X, n, d, node, seed, counter, theta);
-  const hipError_t e = hipGetLastError();
+  const int smem = (int)(sizeof(unsigned long long) * (n <= kInitLdsRows ? n : 0));
+  hipError_t e = hipFuncSetAttribute((const void*)split_init_kernel,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)(sizeof(unsigned long long) * kInitLdsRows));
+  if (e != hipSuccess) return hip_fail(e, "split_init_kernel shared memory attribute");
+  split_init_kernel<<<problems, kInitThreads, smem, stream>>>(X, n, d, node, seed, counter, theta);
+  e = hipGetLastError();
   return e == hipSuccess ? ART_OK : hip_fail(e, "split_init_kernel launch");
 }

@@ -377,12 +377,17 @@ def test_batched_annealer_matches_the_per_start_kernel(torch_gpu, degree):
     np.testing.assert_allclose(two["theta"][7, 1], O.project_unit_w(th), rtol=1e-9, atol=1e-10)
 
 
-@pytest.mark.parametrize("n,d", [(8192, 16), (301, 3), (1, 2), (64, 31)])
-def test_device_random_starts_match_the_oracle(torch_gpu, n, d):
+@pytest.mark.parametrize("n,d,ties", [(8192, 16, False), (301, 3, False), (1, 2, False),
+                                       (64, 31, False), (8193, 3, False), (10000, 2, False),
+                                       (2, 1, False), (400, 3, True), (9000, 2, True)])
+def test_device_random_starts_match_the_oracle(torch_gpu, n, d, ties):
+    # Nodes of more than 8192 rows take the kernel's recompute path; tied
+    # scores (rows from a 3-point grid) exercise the repeated middle key.
     from art import AffineRidgeModel, BatchedOptimizerWorkspace, SoftObliqueSplitter
 
     rng = np.random.default_rng(n + d)
-    X = rng.uniform(-1, 1, (2, n, d))
+    X = (rng.integers(0, 3, (2, n, d)).astype(float) if ties
+         else rng.uniform(-1, 1, (2, n, d)))
     y = X[..., 0] ** 2
     s = SoftObliqueSplitter(AffineRidgeModel(1e-4), .5, random_state=11)
     with BatchedOptimizerWorkspace(s, X, y) as ws:
