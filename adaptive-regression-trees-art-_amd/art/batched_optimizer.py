@@ -351,7 +351,7 @@ class BatchedOptimizerWorkspace:
         timing, a list, receives the milliseconds of each device launch: the
         random starts (when theta0 is None), then the annealer.
 
-        batched (default: when d <= 16 and at most 16 non-constant features)
+        batched (default: when d <= 16 and p <= 17)
         runs 16 starts of a node per workgroup, the node's rows staged once
         per pass for all of them (art_split_anneal_batched); otherwise one
         workgroup per start.
@@ -373,12 +373,11 @@ class BatchedOptimizerWorkspace:
                 raise ValueError("theta0 must be finite with shape (r,d+1) or (R,r,d+1).")
             starts = np.stack([[project_unit_w(t) for t in rs] for rs in starts])
         Rr = starts.shape[0]
-        nf = self.p - (1 if self.bias else 0)
-        eligible = self.d <= 16 and nf <= 16
+        eligible = self.d <= 16 and self.p <= 17
         if batched is None:
             batched = eligible
         if batched and not eligible:
-            raise ValueError("The batched annealer needs d <= 16 and at most 16 features.")
+            raise ValueError("The batched annealer needs d <= 16 and at most 17 basis features.")
         if batched:
             # Node-major blocks of 16 starts; a node's last block repeats its
             # last start (results of the copies are dropped).

@@ -6,20 +6,22 @@
 // takes a fixed projected step theta <- retract(theta - step P(grad))
 // (art/objectives.py:132-340, art/splitters.py:71-98; the annealing schedule
 // and fixed step have no reference counterpart).  The node's rows stream
-// through LDS once per pass for all 16 starts:
-//   fit     per row chunk, a row-thread phase forms every start's gates pi;
-//           then each wave runs v_mfma_f64_16x16x4 Grams of the non-constant
-//           features for its 2 starts and both sides (weights
-//           max(1 - pi, floor) and max(pi, floor), as the reference's two
-//           weighted fits), with the bias and y columns on the VALU;
+// through LDS in 64-row chunks, the next chunk's loads in flight while the
+// current one is processed; every contraction is an FP64 MFMA
+// (v_mfma_f64_16x16x4):
+//   fit     U = X W (rows x starts) gives the gates pi and the side weights
+//           q = max(1 - pi, floor), max(pi, floor), as the reference's two
+//           weighted fits; then all 32 augmented systems at once as one
+//           product G (32 start-sides x 171) = Q^T (32 x rows) Psi (rows x 171),
+//           Psi_r = the packed outer product of [phi(x_r), y_r];
 //   solve   trace-scaled ridge, Crout Cholesky and substitutions for 16 of
 //           the 32 systems at a time, one 32-lane group per system;
-//   stats   a row-thread phase forms residuals, gates and gradient terms c
-//           for the wave's 2 starts; a (start, column, half) thread grid
-//           forms X^T c from LDS.
+//   stats   U = X W again, predictions P = Phi B (rows x 32 start-sides),
+//           residual terms c per (row, start) in registers and the gradient
+//           C^T X (starts x dims).
 // A final hard fit and hard statistics give each start's partition.
 //
-// Limits: d <= 16 and at most 16 non-constant features (one MFMA tile);
+// Limits: d <= 16 and p <= 17 (so [phi, y] has at most 171 packed products);
 // problems come in blocks of 16 that share a node.
 
 #include "art_common.h"
@@ -30,16 +32,30 @@ namespace art {
 namespace {
 
 constexpr int kAR = 16;                            // starts per workgroup
-constexpr int kAThreads = 512;                     // 8 waves, 2 starts each
+constexpr int kAThreads = 512;                     // 8 waves
 constexpr int kARows = 64;                         // rows per staged chunk
 constexpr int kADim = 16;                          // max input dimension
 constexpr int kAStr = kADim + 1;                   // LDS row stride: x_0..x_{d-1}, y
 constexpr int kAP = kADim + 1;                     // max basis size
 constexpr int kATri = (kAP + 1) * (kAP + 2) / 2;   // packed augmented system
 constexpr int kAGroups = kAThreads / 32;           // systems factored at once
+constexpr int kFStr = kAP + 2;                     // F row stride (odd)
+constexpr int kQStr = 2 * kAR + 1;                 // Q / P row stride (odd)
+constexpr int kSlots = (kARows * (kADim + 1) + kAThreads - 1) / kAThreads;
 
-__device__ inline double agate(double score) {
-  return 1.0 / (1.0 + exp(-fmin(50.0, fmax(-50.0, score))));
+// Gate of a score s = (u - z) * (1/T): 1 / (1 + e^-s) with s clipped to
+// [-50, 50], the quotient from the hardware reciprocal and two Newton steps
+// (within an ulp or two of the division).
+__device__ inline double fast_gate(double s) {
+  const double a = 1.0 + exp(-fmin(50.0, fmax(-50.0, s)));
+  double r = __builtin_amdgcn_rcp(a);
+  r = fma(r, fma(-a, r, 1.0), r);
+  r = fma(r, fma(-a, r, 1.0), r);
+  return r;
+}
+
+__device__ inline f64x4 mfma4(double a, double b, f64x4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
 struct AnnealShared {
@@ -49,171 +65,258 @@ struct AnnealShared {
   double lam[kAR][2];
   int status[kAR][2];
   int bad[kAR];
-  double G[2 * kAR][kATri];     // augmented systems, index 2 r + side
-  double xs[kARows * kAStr];
-  double qs[kAR][kARows];       // gates (fit), gradient terms (stats)
-  double gw[2][kAR][kADim];     // X^T c halves
+  int badw[4][kAR];
   double acc[kAR][4];           // loss, -sum c, count(pi >= 1/2), bad
   double hacc[kAR][4];          // SSE_L, SSE_R, n_right, bad
   double geo[kAR][4];
+  double gsum[kAR][kADim];
   double piv[kAGroups];
   double bc[kAGroups][2];
-  signed char fa[kAP], fb[kAP], core[kAP];
-  int nf, cbias;
+  double xs[2][kARows * kAStr];  // double-buffered row chunks
+  signed char fa[kAP], fb[kAP];
+  union {
+    double G[2 * kAR][kATri];    // augmented systems, index 2 r + side
+    struct {
+      double F[kARows][kFStr];   // [phi, y] per row
+      double Q[kARows][kQStr];   // side weights, column 2 r + side
+    } fit;
+    struct {
+      double pi[kARows][kAR + 1];
+      double P[kARows][kQStr];   // predictions, column 2 r + side
+    } st;
+    double red[8][kAR][kADim + 3];  // per-wave gradient tiles and sums
+  } u;
 };
 
+typedef const __attribute__((address_space(1))) double gf64;
+
 struct AnnealCtx {
-  const double* X;
-  const double* y;
+  gf64* X;
+  gf64* y;
   int n, d, p;
   double ridge, floor;
 };
 
-__device__ inline void stage_chunk(const AnnealCtx& C, AnnealShared& S, int base, int rows) {
-  // 32-lane group per row, 16 rows per sweep, 4 sweeps per 64-row chunk.
-  constexpr int kSweeps = kARows / (kAThreads / 32);
-  const int g = threadIdx.x >> 5, j = threadIdx.x & 31;
-  const bool isx = j < C.d, isy = j == C.d;
-  const double* src = isx ? C.X + ((size_t)base + g) * C.d + j : C.y + base + g;
-  const size_t stride = isx ? 16 * (size_t)C.d : 16;
-  const int lim = (isx || isy) ? rows : 0;
-  double v[kSweeps];
+// Chunk streaming: slot i of a thread holds element tid + 512 i of the
+// chunk's x values (row-major, contiguous in X) followed by its y values.
+// Chunks alternate between the two xs buffers; buf names the current one.
+struct Stream {
+  int row[kSlots], col[kSlots];  // row < 0: unused slot; col == d: y
+};
+
+__device__ inline void stream_init(const AnnealCtx& C, Stream& X) {
+  const int nx = kARows * C.d;
 #pragma unroll
-  for (int i = 0; i < kSweeps; ++i) {
-    v[i] = (g + 16 * i < lim) ? *src : 0.0;
-    src += stride;
+  for (int i = 0; i < kSlots; ++i) {
+    const int e = threadIdx.x + kAThreads * i;
+    if (e < nx) {
+      X.row[i] = e / C.d;
+      X.col[i] = e - X.row[i] * C.d;
+    } else if (e < nx + kARows) {
+      X.row[i] = e - nx;
+      X.col[i] = C.d;
+    } else {
+      X.row[i] = -1;
+      X.col[i] = 0;
+    }
   }
-  double* dst = S.xs + g * kAStr + j;
+}
+
+__device__ inline void stream_load(const AnnealCtx& C, const Stream& X, int base, double* v) {
+  const int rows = min(kARows, C.n - base);
 #pragma unroll
-  for (int i = 0; i < kSweeps; ++i)
-    if (g + 16 * i < lim) dst[16 * kAStr * i] = v[i];
+  for (int i = 0; i < kSlots; ++i) {
+    v[i] = 0.0;
+    const int r = X.row[i];
+    if (r >= 0 && r < rows)
+      v[i] = X.col[i] < C.d ? C.X[(size_t)(base + r) * C.d + X.col[i]] : C.y[base + r];
+  }
+}
+
+__device__ inline void stream_store(const Stream& X, double* xs, const double* v) {
+#pragma unroll
+  for (int i = 0; i < kSlots; ++i)
+    if (X.row[i] >= 0) xs[X.row[i] * kAStr + X.col[i]] = v[i];
+}
+
+// Synchronous staging of one chunk into xs (used once, by the hard stats).
+__device__ inline void stage_chunk(const AnnealCtx& C, const Stream& X, double* xs, int base) {
+  double v[kSlots];
+  stream_load(C, X, base, v);
+  stream_store(X, xs, v);
+}
+
+// U tile of the wave (rows 16 w .. 16 w + 15 of the chunk, w < 4) against
+// every start's w: lane (hi, lo) gets u for rows 16 w + hi + 4 r, start lo.
+__device__ inline f64x4 gate_scores(const AnnealShared& S, const double* xs, int d, int w) {
+  const int lane = threadIdx.x & 63, lo = lane & 15, hi = lane >> 4;
+  f64x4 u = {0.0, 0.0, 0.0, 0.0};
+  for (int ks = 0; ks < (d + 3) / 4; ++ks) {
+    const int j = 4 * ks + hi;
+    const double a = j < d ? xs[(16 * w + lo) * kAStr + j] : 0.0;
+    const double b = j < d ? S.theta[lo][j] : 0.0;
+    u = mfma4(a, b, u);
+  }
+  return u;
+}
+
+// One chunk of the wave's NT product tiles: acc[s][mt] += Q^T (rows x
+// start-sides 16 mt ..) times Psi (rows x columns of tile s), Psi entries
+// F[row][oa] F[row][ob].  Operands of the next 4-row step are loaded before
+// the current step's MFMAs.
+template <int NT>
+__device__ inline void gram_chunk(const AnnealShared& S, int lo, int hi, const int* oa,
+                                  const int* ob, f64x4 (&acc)[2][2]) {
+  const double* q = &S.u.fit.Q[hi][0];
+  const double* f = &S.u.fit.F[hi][0];
+  double a0 = q[lo], a1 = q[16 + lo], fa[NT], fb[NT];
+#pragma unroll
+  for (int s = 0; s < NT; ++s) {
+    fa[s] = f[oa[s]];
+    fb[s] = f[ob[s]];
+  }
+#pragma unroll 4
+  for (int ks = 0; ks < kARows / 4; ++ks) {
+    q += 4 * kQStr;
+    f += 4 * kFStr;
+    const bool more = ks + 1 < kARows / 4;
+    double n0 = 0.0, n1 = 0.0, na[NT], nb[NT];
+    if (more) {
+      n0 = q[lo];
+      n1 = q[16 + lo];
+    }
+#pragma unroll
+    for (int s = 0; s < NT; ++s) {
+      na[s] = more ? f[oa[s]] : 0.0;
+      nb[s] = more ? f[ob[s]] : 0.0;
+    }
+#pragma unroll
+    for (int s = 0; s < NT; ++s) {
+      const double bv = fa[s] * fb[s];
+      acc[s][0] = mfma4(a0, bv, acc[s][0]);
+      acc[s][1] = mfma4(a1, bv, acc[s][1]);
+    }
+    a0 = n0;
+    a1 = n1;
+#pragma unroll
+    for (int s = 0; s < NT; ++s) {
+      fa[s] = na[s];
+      fb[s] = nb[s];
+    }
+  }
 }
 
 // Both sides' augmented systems of every start; kind 0: soft gates at T,
-// 1: hard gates.  Rows beyond the node get gate 2 (weight 0 on both sides).
-__device__ void anneal_fit(const AnnealCtx& C, AnnealShared& S, int kind, double T, double floor) {
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int col = lane & 15, kq = lane >> 4;
-  const int d = C.d, p = C.p, n = C.n, nf = S.nf;
-  const bool live = col < nf;
-  const int fc = live ? S.core[col] : 0;
-  const int fa = live ? S.fa[fc] : 0, fb = live ? S.fb[fc] : -1;
-  f64x4 acc[2][2];
-  double e1[2][2], ey[2][2], sm[2][2][3];
-  int badr[2] = {0, 0};
+// 1: hard gates.  Rows beyond the node and rows with an invalid gate get
+// weight 0 on both sides; an invalid gate marks its start bad.
+// The pass functions are not inlined so that each gets its own register
+// allocation (the kernel keeps 2 workgroups of 8 waves per CU).  Both
+// streaming passes take the current xs buffer and return the next pass's.
+__device__ __attribute__((noinline)) int anneal_fit(AnnealCtx C, AnnealShared& S, int buf,
+                                                    int kind, double T, double floor) {
+  Stream X;
+  stream_init(C, X);
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lo = lane & 15, hi = lane >> 4;
+  const int d = C.d, p = C.p, m = p + 1, n = C.n;
+  const int ntri = m * (m + 1) / 2, ntiles = (ntri + 15) >> 4;
+  const double invT = 1.0 / T;
+  // The wave's product tiles nt = wave, wave + 8 (nw of them); lane column
+  // t = 16 nt + lo is Psi entry (a, b), t = tri(a, b); padding columns read
+  // the zero column kFStr - 1 of F.
+  const int nw = (wave + 8 < ntiles) ? 2 : (wave < ntiles ? 1 : 0);
+  int oa[2], ob[2];
 #pragma unroll
-  for (int r = 0; r < 2; ++r)
-#pragma unroll
-    for (int sd = 0; sd < 2; ++sd) {
-      acc[r][sd] = f64x4{0.0, 0.0, 0.0, 0.0};
-      e1[r][sd] = ey[r][sd] = 0.0;
-      sm[r][sd][0] = sm[r][sd][1] = sm[r][sd][2] = 0.0;
+  for (int s = 0; s < 2; ++s) {
+    const int t = 16 * (wave + 8 * s) + lo;
+    int a = kFStr - 1, b = kFStr - 1;
+    if (t < ntri) {
+      a = 0;
+      while (tri(a + 1, 0) <= t) ++a;
+      b = t - tri(a, 0);
     }
-  for (int base = 0; base < n; base += kARows) {
-    const int rows = min(kARows, n - base);
-    __syncthreads();
-    stage_chunk(C, S, base, rows);
-    __syncthreads();
-    {  // row phase: lane = row, the wave's 2 starts
-      const bool valid = lane < rows;
-      const double* xr = S.xs + lane * kAStr;
-      const double yv = valid ? xr[d] : 0.0;
+    oa[s] = a;
+    ob[s] = b;
+  }
+  f64x4 acc[2][2];
 #pragma unroll
-      for (int rr = 0; rr < 2; ++rr) {
-        const int r = 2 * wave + rr;
-        double pi = 2.0;
-        if (valid) {
-          double u = 0.0;
-          for (int j = 0; j < d; ++j) u = fma(xr[j], S.theta[r][j], u);
-          const double z = S.theta[r][d];
+  for (int s = 0; s < 2; ++s) acc[s][0] = acc[s][1] = f64x4{0.0, 0.0, 0.0, 0.0};
+  int bad = 0;
+  const int chunks = (n + kARows - 1) / kARows;
+  for (int c = 0; c < chunks; ++c) {
+    const int rows = min(kARows, n - c * kARows);
+    const double* xs = S.xs[buf];
+    double pf[kSlots];
+    stream_load(C, X, (c + 1 < chunks ? c + 1 : 0) * kARows, pf);
+    if (wave < 4) {
+      const f64x4 u = gate_scores(S, xs, d, wave);
+      const double z = S.theta[lo][d];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = 16 * wave + hi + 4 * r;
+        double q0 = 0.0, q1 = 0.0;
+        if (row < rows) {
+          const double uu = u[r];
+          double pi;
           if (kind == 1) {
-            pi = (isfinite(u) && isfinite(z)) ? (double)(u >= z) : __builtin_nan("");
+            pi = (isfinite(uu) && isfinite(z)) ? (double)(uu >= z) : __builtin_nan("");
           } else {
-            const double score = (u - z) / T;
-            pi = isfinite(score) ? agate(score) : __builtin_nan("");
+            const double score = (uu - z) * invT;
+            pi = isfinite(score) ? fast_gate(score) : __builtin_nan("");
           }
           const bool ok = isfinite(pi) && pi >= 0.0 && pi <= 1.0;
-          badr[rr] |= !ok;
-          if (!ok) pi = 2.0;
-          const double q0 = pi <= 1.0 ? fmax(1.0 - pi, floor) : 0.0;
-          const double q1 = pi <= 1.0 ? fmax(pi, floor) : 0.0;
-          sm[rr][0][0] += q0;
-          sm[rr][0][1] += q0 * yv;
-          sm[rr][0][2] += q0 * yv * yv;
-          sm[rr][1][0] += q1;
-          sm[rr][1][1] += q1 * yv;
-          sm[rr][1][2] += q1 * yv * yv;
+          bad |= !ok;
+          if (ok) {
+            q0 = fmax(1.0 - pi, floor);
+            q1 = fmax(pi, floor);
+          }
         }
-        S.qs[r][lane] = pi;
+        S.u.fit.Q[row][2 * lo] = q0;
+        S.u.fit.Q[row][2 * lo + 1] = q1;
+      }
+    } else {
+      // [phi, y, 0 ...] of every row: 4 threads per row.
+      const int t = tid - 256, row = t >> 2;
+      const double* xr = xs + row * kAStr;
+      for (int f = t & 3; f < kFStr; f += 4) {
+        double v = 0.0;
+        if (f == p) {
+          v = xr[d];
+        } else if (f < p) {
+          const int a = S.fa[f], b = S.fb[f];
+          v = a >= 0 ? xr[a] : 1.0;
+          if (b >= 0) v *= xr[b];
+        }
+        S.u.fit.F[row][f] = v;
       }
     }
     __syncthreads();
-    // MFMA phase: 4 rows per step; the wave's 2 starts share the features.
-#pragma unroll 1
-    for (int sq = 0; sq < kARows / 4; ++sq) {
-      const int row = 4 * sq + kq;
-      const bool valid = row < rows;
-      const double* xr = S.xs + row * kAStr;
-      double v = 0.0;
-      if (valid && live) {
-        v = xr[fa];
-        if (fb >= 0) v *= xr[fb];
-      }
-      const double yv = valid ? xr[d] : 0.0;
-#pragma unroll
-      for (int rr = 0; rr < 2; ++rr) {
-        const double pi = S.qs[2 * wave + rr][row];
-        const double q0 = pi <= 1.0 ? fmax(1.0 - pi, floor) : 0.0;
-        const double q1 = pi <= 1.0 ? fmax(pi, floor) : 0.0;
-        acc[rr][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(q0 * v, v, acc[rr][0], 0, 0, 0);
-        acc[rr][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(q1 * v, v, acc[rr][1], 0, 0, 0);
-        e1[rr][0] += q0 * v;
-        ey[rr][0] += q0 * yv * v;
-        e1[rr][1] += q1 * v;
-        ey[rr][1] += q1 * yv * v;
-      }
-    }
+    if (nw == 2) gram_chunk<2>(S, lo, hi, oa, ob, acc);
+    else if (nw == 1) gram_chunk<1>(S, lo, hi, oa, ob, acc);
+    stream_store(X, S.xs[buf ^ 1], pf);
+    __syncthreads();
+    buf ^= 1;
   }
-  // Assemble the wave's 4 systems.
-  const int cb = S.cbias;
+  // The systems overwrite F and Q: every wave passed the loop's last barrier.
 #pragma unroll
-  for (int rr = 0; rr < 2; ++rr) {
-    const int b = wave_or(badr[rr]);
-    if (lane == 0) S.bad[2 * wave + rr] = b;
+  for (int s = 0; s < 2; ++s) {
+    const int t = 16 * (wave + 8 * s) + lo;
+    if (s < nw && t < ntri)
 #pragma unroll
-    for (int sd = 0; sd < 2; ++sd) {
-      double* G = S.G[2 * (2 * wave + rr) + sd];
-      // f64 16x16x4 output: lane (col, kq) holds rows kq + 4 i, column col.
+      for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int a = kq + 4 * i;
-        if (a < nf && col < nf && a <= col) {
-          const int ca = S.core[a], cc = S.core[col];
-          G[ca >= cc ? tri(ca, cc) : tri(cc, ca)] = acc[rr][sd][i];
-        }
-      }
-      double a1 = e1[rr][sd], ay = ey[rr][sd];
-      a1 += __shfl_xor(a1, 16, kWave);
-      a1 += __shfl_xor(a1, 32, kWave);
-      ay += __shfl_xor(ay, 16, kWave);
-      ay += __shfl_xor(ay, 32, kWave);
-      if (kq == 0 && live) {
-        const int cf = S.core[col];
-        G[tri(p, cf)] = ay;
-        if (cb >= 0) G[cf >= cb ? tri(cf, cb) : tri(cb, cf)] = a1;
-      }
-      const double t0 = wave_sum(sm[rr][sd][0]), t1 = wave_sum(sm[rr][sd][1]),
-                   t2 = wave_sum(sm[rr][sd][2]);
-      if (lane == 0) {
-        if (cb >= 0) {
-          G[tri(cb, cb)] = t0;
-          G[tri(p, cb)] = t1;
-        }
-        G[tri(p, p)] = t2;
-      }
-    }
+        for (int r = 0; r < 4; ++r) S.u.G[16 * mt + hi + 4 * r][t] = acc[s][mt][r];
+  }
+  if (wave < 4) {
+    bad |= __shfl_xor(bad, 16, kWave);
+    bad |= __shfl_xor(bad, 32, kWave);
+    if (hi == 0) S.badw[wave][lo] = bad;
   }
   __syncthreads();
+  if (tid < kAR) S.bad[tid] = S.badw[0][tid] | S.badw[1][tid] | S.badw[2][tid] | S.badw[3][tid];
+  __syncthreads();
+  return buf;
 }
 
 // Crout Cholesky and substitutions of one system per 32-lane group (lane i
@@ -277,13 +380,13 @@ __device__ int group_solve(double* M, AnnealShared& S, int p, double& x) {
 }
 
 // All 32 systems, 16 at a time: ridge lambda = ridge tr(G)/p, factor, solve.
-__device__ void anneal_solve(const AnnealCtx& C, AnnealShared& S) {
+__device__ __attribute__((noinline)) void anneal_solve(AnnealCtx C, AnnealShared& S) {
   const int g = threadIdx.x >> 5, i = threadIdx.x & 31;
   const int p = C.p;
   for (int round = 0; round < 2 * kAR / kAGroups; ++round) {
     const int sys = round * kAGroups + g;
     const int r = sys >> 1, side = sys & 1;
-    double* M = S.G[sys];
+    double* M = S.u.G[sys];
     double tr = 0.0;
     for (int k = i; k < p; k += 32) tr += M[tri(k, k)];
     for (int o = 16; o > 0; o >>= 1) tr += __shfl_xor(tr, o, 32);
@@ -306,98 +409,121 @@ __device__ void anneal_solve(const AnnealCtx& C, AnnealShared& S) {
 }
 
 // Residual statistics, gradient and direction of every start at T.
-__device__ void anneal_stats(const AnnealCtx& C, AnnealShared& S, double T) {
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+__device__ __attribute__((noinline)) int anneal_stats(AnnealCtx C, AnnealShared& S, int buf,
+                                                      double T) {
+  Stream X;
+  stream_init(C, X);
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lo = lane & 15, hi = lane >> 4;
   const int d = C.d, p = C.p, n = C.n;
-  const double inv = 1.0 / (n * T);
-  const int gr = (tid >> 4) & 15, gj = tid & 15, gh = tid >> 8;  // start, column, row half
-  double loss[2] = {0.0, 0.0}, gz[2] = {0.0, 0.0}, cnt[2] = {0.0, 0.0};
-  double g = 0.0;
-  for (int base = 0; base < n; base += kARows) {
-    const int rows = min(kARows, n - base);
-    __syncthreads();
-    stage_chunk(C, S, base, rows);
-    __syncthreads();
-    {
-      const bool valid = lane < rows;
-      const double* xr = S.xs + lane * kAStr;
-      double phi[kAP];
+  const double inv = 1.0 / (n * T), invT = 1.0 / T;
+  double loss = 0.0, gz = 0.0, cnt = 0.0;  // start lo, this lane's rows
+  f64x4 g = {0.0, 0.0, 0.0, 0.0};          // gradient tile: start hi + 4 r, dim lo
+  const int chunks = (n + kARows - 1) / kARows;
+  for (int c = 0; c < chunks; ++c) {
+    const int rows = min(kARows, n - c * kARows);
+    const double* xs = S.xs[buf];
+    double pf[kSlots];
+    stream_load(C, X, (c + 1 < chunks ? c + 1 : 0) * kARows, pf);
+    if (wave < 4) {
+      const f64x4 u = gate_scores(S, xs, d, wave);
+      const double z = S.theta[lo][d];
 #pragma unroll
-      for (int f = 0; f < kAP; ++f) {
-        double v = 0.0;
-        if (valid && f < p) {
-          const int a = S.fa[f], b = S.fb[f];
-          v = a >= 0 ? xr[a] : 1.0;
-          if (b >= 0) v *= xr[b];
+      for (int r = 0; r < 4; ++r) S.u.st.pi[16 * wave + hi + 4 * r][lo] = fast_gate((u[r] - z) * invT);
+    } else {
+      // Predictions of rows 16 rt .. 16 rt + 15 for all 32 start-sides.
+      const int rt = wave - 4;
+      const double* xr = xs + (16 * rt + lo) * kAStr;
+      f64x4 P0 = {0.0, 0.0, 0.0, 0.0}, P1 = {0.0, 0.0, 0.0, 0.0};
+      for (int ks = 0; ks < (p + 3) / 4; ++ks) {
+        const int f = 4 * ks + hi;
+        double a = 0.0, b0 = 0.0, b1 = 0.0;
+        if (f < p) {
+          const int fa = S.fa[f], fb = S.fb[f];
+          a = fa >= 0 ? xr[fa] : 1.0;
+          if (fb >= 0) a *= xr[fb];
+          b0 = S.beta[lo >> 1][lo & 1][f];
+          b1 = S.beta[8 + (lo >> 1)][lo & 1][f];
         }
-        phi[f] = v;
+        P0 = mfma4(a, b0, P0);
+        P1 = mfma4(a, b1, P1);
       }
-      const double yv = valid ? xr[d] : 0.0;
 #pragma unroll
-      for (int rr = 0; rr < 2; ++rr) {
-        const int r = 2 * wave + rr;
-        double c = 0.0;
-        if (valid) {
-          double pl = 0.0, pr = 0.0;
-#pragma unroll
-          for (int f = 0; f < kAP; ++f)
-            if (f < p) {
-              pl = fma(phi[f], S.beta[r][0][f], pl);
-              pr = fma(phi[f], S.beta[r][1][f], pr);
-            }
-          double u = 0.0;
-          for (int j = 0; j < d; ++j) u = fma(xr[j], S.theta[r][j], u);
-          const double pi = agate((u - S.theta[r][d]) / T);
-          double l = yv - pl, rq = yv - pr;
-          l *= l;
-          rq *= rq;
-          c = (rq - l) * pi * (1.0 - pi) * inv;
-          loss[rr] += (1.0 - pi) * l + pi * rq;
-          gz[rr] -= c;
-          cnt[rr] += pi >= 0.5 ? 1.0 : 0.0;
-        }
-        S.qs[r][lane] = c;
+      for (int r = 0; r < 4; ++r) {
+        const int row = 16 * rt + hi + 4 * r;
+        S.u.st.P[row][lo] = P0[r];
+        S.u.st.P[row][16 + lo] = P1[r];
       }
     }
     __syncthreads();
-    if (gj < d)
-      for (int r = gh; r < rows; r += 2) g = fma(S.qs[gr][r], S.xs[r * kAStr + gj], g);
-  }
+    // Wave w: rows 8 w .. 8 w + 7 as the gradient's 2 k-steps; the lane's
+    // residual term is its own MFMA operand.
 #pragma unroll
-  for (int rr = 0; rr < 2; ++rr) {
-    const double a = wave_sum(loss[rr]), b = wave_sum(gz[rr]), c = wave_sum(cnt[rr]);
-    if (lane == 0) {
-      S.acc[2 * wave + rr][0] = a;
-      S.acc[2 * wave + rr][1] = b;
-      S.acc[2 * wave + rr][2] = c;
+    for (int j = 0; j < 2; ++j) {
+      const int row = 8 * wave + 4 * j + hi;
+      double cv = 0.0;
+      if (row < rows) {
+        const double pi = S.u.st.pi[row][lo];
+        const double yv = xs[row * kAStr + d];
+        double l = yv - S.u.st.P[row][2 * lo], rq = yv - S.u.st.P[row][2 * lo + 1];
+        l *= l;
+        rq *= rq;
+        cv = (rq - l) * pi * (1.0 - pi) * inv;
+        loss += (1.0 - pi) * l + pi * rq;
+        gz -= cv;
+        cnt += pi >= 0.5 ? 1.0 : 0.0;
+      }
+      const double xv = lo < d ? xs[row * kAStr + lo] : 0.0;
+      g = mfma4(cv, xv, g);
     }
+    stream_store(X, S.xs[buf ^ 1], pf);
+    __syncthreads();
+    buf ^= 1;
   }
-  S.gw[gh][gr][gj] = g;
+  loss += __shfl_xor(loss, 16, kWave);
+  loss += __shfl_xor(loss, 32, kWave);
+  gz += __shfl_xor(gz, 16, kWave);
+  gz += __shfl_xor(gz, 32, kWave);
+  cnt += __shfl_xor(cnt, 16, kWave);
+  cnt += __shfl_xor(cnt, 32, kWave);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) S.u.red[wave][hi + 4 * r][lo] = g[r];
+  if (hi == 0) {
+    S.u.red[wave][lo][kADim] = loss;
+    S.u.red[wave][lo][kADim + 1] = gz;
+    S.u.red[wave][lo][kADim + 2] = cnt;
+  }
+  __syncthreads();
+  if (tid < kAR * kADim) {
+    const int r = tid >> 4, j = tid & 15;
+    double v = 0.0;
+    for (int w = 0; w < 8; ++w) v += S.u.red[w][r][j];
+    S.gsum[r][j] = v;
+  } else if (tid < kAR * kADim + 3 * kAR) {
+    const int e = tid - kAR * kADim, r = e / 3, k = e - 3 * r;
+    double v = 0.0;
+    for (int w = 0; w < 8; ++w) v += S.u.red[w][r][kADim + k];
+    S.acc[r][k] = v;
+  }
   __syncthreads();
   if (tid < kAR) {
     const int r = tid;
     const double* w = S.theta[r];
     const double gzr = S.acc[r][1];
-    double gwv[kADim];
     double ww = 0.0, wg = 0.0;
-#pragma unroll
-    for (int j = 0; j < kADim; ++j) {
-      gwv[j] = j < d ? S.gw[0][r][j] + S.gw[1][r][j] : 0.0;
-      if (j < d) {
-        ww += w[j] * w[j];
-        wg += w[j] * gwv[j];
-      }
+    for (int j = 0; j < d; ++j) {
+      ww += w[j] * w[j];
+      wg += w[j] * S.gsum[r][j];
     }
     const double proj = wg / ww;
     double pp = 0.0, wp = 0.0;
-#pragma unroll
-    for (int j = 0; j < kADim; ++j)
-      if (j < d) {
-        const double v = -gwv[j] + proj * w[j];
-        S.pdir[r][j] = v;
-        pp += v * v;
-        wp += w[j] * v;
-      }
+    for (int j = 0; j < d; ++j) {
+      const double v = -S.gsum[r][j] + proj * w[j];
+      S.pdir[r][j] = v;
+      pp += v * v;
+      wp += w[j] * v;
+    }
     S.pdir[r][d] = -gzr;
     S.geo[r][0] = ww;
     S.geo[r][1] = wp;
@@ -409,21 +535,27 @@ __device__ void anneal_stats(const AnnealCtx& C, AnnealShared& S, double T) {
                       : 0.0;
   }
   __syncthreads();
+  return buf;
 }
 
-// [SSE_L, SSE_R, n_right, bad] of every start's hard partition and fit.
-__device__ void anneal_hard_stats(const AnnealCtx& C, AnnealShared& S) {
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+// [SSE_L, SSE_R, n_right, bad] of every start's hard partition and fit; a
+// wave per 2 starts, a lane per row (runs once, after the last iteration).
+__device__ __attribute__((noinline)) void anneal_hard_stats(AnnealCtx C, AnnealShared& S) {
+  Stream X;
+  stream_init(C, X);
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int d = C.d, p = C.p, n = C.n;
   double sl[2] = {0.0, 0.0}, sr[2] = {0.0, 0.0}, cnt[2] = {0.0, 0.0};
   int bad[2] = {0, 0};
+  double* xs = S.xs[0];
   for (int base = 0; base < n; base += kARows) {
     const int rows = min(kARows, n - base);
     __syncthreads();
-    stage_chunk(C, S, base, rows);
+    stage_chunk(C, X, xs, base);
     __syncthreads();
     if (lane < rows) {
-      const double* xr = S.xs + lane * kAStr;
+      const double* xr = xs + lane * kAStr;
 #pragma unroll
       for (int rr = 0; rr < 2; ++rr) {
         const int r = 2 * wave + rr;
@@ -466,7 +598,7 @@ __device__ void anneal_hard_stats(const AnnealCtx& C, AnnealShared& S) {
 
 }  // namespace
 
-__global__ __launch_bounds__(kAThreads) void anneal_kernel(art_split_args args) {
+__global__ __launch_bounds__(kAThreads, 4) void anneal_kernel(art_split_args args) {
   extern __shared__ __align__(16) unsigned char anneal_smem[];
   AnnealShared& S = *reinterpret_cast<AnnealShared*>(anneal_smem);
   const art_split_args& a = args;
@@ -475,8 +607,8 @@ __global__ __launch_bounds__(kAThreads) void anneal_kernel(art_split_args args) 
   const int node = a.node[q0];
   const int n = a.n, d = a.d, p = a.basis.p;
   AnnealCtx C;
-  C.X = a.X + (size_t)node * n * d;
-  C.y = a.y + (size_t)node * n;
+  C.X = (gf64*)(a.X + (size_t)node * n * d);
+  C.y = (gf64*)(a.y + (size_t)node * n);
   C.n = n;
   C.d = d;
   C.p = p;
@@ -486,19 +618,18 @@ __global__ __launch_bounds__(kAThreads) void anneal_kernel(art_split_args args) 
     const int r = tid / (d + 1), j = tid - r * (d + 1);
     S.theta[r][j] = a.theta0[(size_t)(q0 + r) * (d + 1) + j];
   }
-  if (tid == 0) {
-    int nf = 0, cb = -1;
-    for (int f = 0; f < p; ++f) {
-      int fa, fb;
-      basis_factors(a.basis, f, fa, fb);
-      S.fa[f] = (signed char)fa;
-      S.fb[f] = (signed char)fb;
-      if (fa < 0) cb = f;
-      else S.core[nf++] = (signed char)f;
-    }
-    S.nf = nf;
-    S.cbias = cb;
+  if (tid < p) {
+    int fa, fb;
+    basis_factors(a.basis, tid, fa, fb);
+    S.fa[tid] = (signed char)fa;
+    S.fb[tid] = (signed char)fb;
   }
+  {
+    Stream X;
+    stream_init(C, X);
+    stage_chunk(C, X, S.xs[0], 0);  // the first pass's first chunk
+  }
+  int buf = 0;
   __syncthreads();
   if (tid < kAR) {  // unit w
     double nw = 0.0;
@@ -513,9 +644,9 @@ __global__ __launch_bounds__(kAThreads) void anneal_kernel(art_split_args args) 
   const int MI = a.max_iters;
   for (int k = 0; k < K; ++k) {
     const double T = a.t0 * pow(a.t1 / a.t0, (double)k / (double)max(K - 1, 1));
-    anneal_fit(C, S, 0, T, C.floor);
+    buf = anneal_fit(C, S, buf, 0, T, C.floor);
     anneal_solve(C, S);
-    anneal_stats(C, S, T);
+    buf = anneal_stats(C, S, buf, T);
     if (tid < kAR) {
       const int r = tid;
       if (!err) {
@@ -543,7 +674,7 @@ __global__ __launch_bounds__(kAThreads) void anneal_kernel(art_split_args args) 
   __syncthreads();
   const bool refit = a.hard_refit != 0;
   if (refit) {
-    anneal_fit(C, S, 1, 1.0, 0.0);
+    buf = anneal_fit(C, S, buf, 1, 1.0, 0.0);
     anneal_solve(C, S);
   }
   anneal_hard_stats(C, S);
@@ -601,10 +732,9 @@ extern "C" int art_split_anneal_batched(const art_split_args* args, int problems
   if (!args || problems < kAR || problems % kAR)
     return fail(ART_ERR_ARG, "art_split_anneal_batched: problems must be a positive multiple of 16");
   const int p = args->basis.p, d = args->d;
-  const int nf = p - (args->basis.bias ? 1 : 0);
-  if (d < 1 || d > kADim || nf < 1 || nf > 16 || p > kAP || args->n < 1)
-    return fail(ART_ERR_ARG, "art_split_anneal_batched: needs 1 <= d <= 16 and at most 16 "
-                             "non-constant features");
+  if (d < 1 || d > kADim || p < 1 || p > kAP || args->n < 1)
+    return fail(ART_ERR_ARG, "art_split_anneal_batched: needs 1 <= d <= 16 and at most 17 "
+                             "basis features");
   if (args->anneal_iters < 1 || !(args->t0 > 0.0) || !(args->t1 > 0.0))
     return fail(ART_ERR_ARG, "art_split_anneal_batched: needs iterations and positive temperatures");
   if (!args->X || !args->y || !args->theta0 || !args->node || !args->theta_out ||

@@ -191,7 +191,7 @@ int art_split_optimize(const art_split_args* args, int problems, hipStream_t str
  * starts.  Each side's system is its own floored weighted Gram
  * (max(1 - pi, floor) and max(pi, floor)), as in the two weighted fits of
  * art/objectives.py:237-303, so the results equal art_split_optimize mode 1
- * per start.  Needs d <= 16 and at most 16 non-constant features.  Same
+ * per start.  Needs d <= 16 and at most 17 basis features.  Same
  * outputs as art_split_optimize mode 1 (history row 0 = soft loss per
  * iteration). */
 int art_split_anneal_batched(const art_split_args* args, int problems, hipStream_t stream);

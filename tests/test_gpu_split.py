@@ -377,6 +377,41 @@ def test_batched_annealer_matches_the_per_start_kernel(torch_gpu, degree):
     np.testing.assert_allclose(two["theta"][7, 1], O.project_unit_w(th), rtol=1e-9, atol=1e-10)
 
 
+def test_batched_annealer_at_full_width_matches_the_oracle(torch_gpu):
+    # d = 16 affine (p = 17: [phi, y] fills all 171 packed products), 16
+    # starts of each of 2 nodes in one workgroup each, 700 rows (a partial
+    # last chunk).
+    from art import AffineRidgeModel, BatchedOptimizerWorkspace, SoftObliqueSplitter
+
+    rng = np.random.default_rng(91)
+    d = 16
+    X = rng.uniform(-1, 1, (2, 700, d))
+    y = np.exp(-2.0 * ((X[..., :3] - .2) ** 2).sum(-1)) - .5 * np.exp(
+        -3.0 * ((X[..., 3:6] + .3) ** 2).sum(-1))
+    s = SoftObliqueSplitter(AffineRidgeModel(1e-4), .2, n_restarts=16, min_side_points=3,
+                            random_state=9)
+    with BatchedOptimizerWorkspace(s, X, y) as ws:
+        starts = ws._device_starts(np.tile(np.arange(2), 16)).reshape(16, 2, d + 1)
+        one = ws.anneal(starts, t0=.2, t1=.02, iters=12, step=.05, batched=False)
+        two = ws.anneal(starts, t0=.2, t1=.02, iters=12, step=.05, batched=True)
+    np.testing.assert_array_equal(one["ok"], two["ok"])
+    np.testing.assert_allclose(two["theta"], one["theta"], rtol=1e-9, atol=1e-10)
+    np.testing.assert_allclose(two["soft_loss_history"], one["soft_loss_history"], rtol=1e-9,
+                               atol=1e-12)
+    np.testing.assert_array_equal(two["n_right"], one["n_right"])
+    so = settings_of(s, .2)
+    for r, j in ((0, 0), (15, 1), (6, 1)):
+        th, losses = O.annealed(X[j], y[j], starts[r, j], so, iters=12, t0=.2, t1=.02,
+                                step=.05)
+        np.testing.assert_allclose(two["theta"][r, j], O.project_unit_w(th), rtol=1e-9,
+                                   atol=1e-10)
+        np.testing.assert_allclose(two["soft_loss_history"][r, j], losses, rtol=1e-9,
+                                   atol=1e-12)
+        h = O.hard_result(X[j], y[j], th, so)
+        assert two["loss"][r, j] == pytest.approx(h["loss"], rel=1e-9, abs=1e-14)
+        np.testing.assert_allclose(two["beta"][r, j], h["beta"], rtol=1e-8, atol=1e-10)
+
+
 @pytest.mark.parametrize("n,d,ties", [(8192, 16, False), (301, 3, False), (1, 2, False),
                                        (64, 31, False), (8193, 3, False), (10000, 2, False),
                                        (2, 1, False), (400, 3, True), (9000, 2, True)])
