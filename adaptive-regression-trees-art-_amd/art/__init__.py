@@ -14,22 +14,38 @@ from .optimizers import AdaptiveAlpha, LineSearchResult
 from .splitters import (SoftObliqueSplitter, SplitNotFoundError, SplitResult, project_unit_w,
                         project_unit_w_gradient)
 from .batched_optimizer import BatchedOptimizerWorkspace, optimize_batch
+from .domain import BoxDomain, PolytopeRegion, split_region
+from .sampling import HitAndRunSampler
+from .temperature import TemperatureConfig, estimate_temperature
+from .tree import LeafNode, RegressionTree, SplitNode
+from .builder import RegressionTreeBuilder, TreeBuildResult
 
 __all__ = [
     "AdaptiveAlpha",
     "AffineRidgeModel",
     "BatchedCholeskyWorkspace",
     "BatchedOptimizerWorkspace",
+    "BoxDomain",
     "FeaturePlan",
+    "HitAndRunSampler",
+    "LeafNode",
     "LeafRidgeBatch",
     "LineSearchResult",
     "PolynomialRidgeModel",
+    "PolytopeRegion",
+    "RegressionTree",
+    "RegressionTreeBuilder",
     "RidgeSolveResult",
     "SoftObliqueSplitter",
     "SplitNotFoundError",
+    "SplitNode",
     "SplitResult",
+    "TemperatureConfig",
+    "TreeBuildResult",
     "build_features",
+    "estimate_temperature",
     "optimize_batch",
     "project_unit_w",
     "project_unit_w_gradient",
+    "split_region",
 ]

@@ -93,6 +93,8 @@ _PROTOTYPES = {
     "art_split_init": (_i, [_p, _i, _i, _p, _i, ctypes.c_uint64, ctypes.c_uint64, _p, _p]),
     "art_har_sample": (_i, [ctypes.POINTER(HarArgs), _p]),
     "art_philox4x32_10": (_i, [_p, _p, _i64, _p, _p]),
+    "art_tree_predict": (_i, [_p, _i64, _p, _p, _p, _i, _p, Basis, _p, _p]),
+    "art_knn_median": (_i, [_p, _p, _p, _i, _i, _i, _p, _p]),
 }
 
 

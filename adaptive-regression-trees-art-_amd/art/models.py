@@ -11,7 +11,7 @@ matrix-vector product on the stored coefficients.
 from __future__ import annotations
 
 from dataclasses import dataclass, field
-from typing import Literal
+from typing import Literal, Protocol, runtime_checkable
 
 import numpy as np
 
@@ -20,6 +20,24 @@ from .features import _columns, feature_count
 RidgeSolver = Literal["auto", "normal", "qr", "svd"]
 RIDGE_SOLVERS = ("auto", "normal", "qr", "svd")
 DEFAULT_AUTO_RCOND_THRESHOLD = 1e-10
+MODEL_CONDITION_WARNING_THRESHOLD = 1e10
+
+
+class RegressionModel(Protocol):
+    """What the tree builder needs of a node model (art/models.py:58-68)."""
+
+    def fit(self, X, y): ...
+
+    def predict(self, X): ...
+
+    def clone(self): ...
+
+
+@runtime_checkable
+class WeightedRegressionModel(RegressionModel, Protocol):
+    """A node model with weighted fits (art/models.py:71-82)."""
+
+    def fit_weighted(self, X, y, weights, weight_floor=1e-12): ...
 
 
 @dataclass(frozen=True)
@@ -165,3 +183,73 @@ def model_basis(model):
     if isinstance(model, PolynomialRidgeModel):
         return int(model.degree), bool(model.include_bias), False
     raise TypeError("GPU fits support AffineRidgeModel and PolynomialRidgeModel.")
+
+
+def model_effective_dimension(model, input_dimension):
+    """Fitting dimension of a model for sample budgets (art/models.py:166-178)."""
+    method = getattr(model, "effective_dimension", None)
+    if method is None:
+        raise TypeError(f"{type(model).__name__} does not define effective_dimension; "
+                        "provide an explicit sample_count policy to the tree builder.")
+    dimension = int(method(input_dimension))
+    if dimension < 1:
+        raise ValueError("Model effective_dimension must be at least 1.")
+    return dimension
+
+
+def ridge_solve_diagnostics(model):
+    """Scalar diagnostics of a model's last ridge solve (art/models.py:181-199)."""
+    result = model if isinstance(model, RidgeSolveResult) else getattr(model, "solve_result_",
+                                                                       None)
+    if not isinstance(result, RidgeSolveResult):
+        return None
+    return {"solver_requested": result.solver_requested, "solver_used": result.solver_used,
+            "condition_estimator": result.condition_estimator,
+            "cond_estimate": result.cond_estimate, "ridge_effective": result.ridge_effective,
+            "rank": result.rank, "fallback_reason": result.fallback_reason}
+
+
+def fit_models(template, samples, stream=None):
+    """Fit one clone of a GPU ridge template per (X, y) pair in a single batched
+    launch (uniform weights, as the builder's node fits); returns the fitted
+    models and their predictions on their own rows.
+
+    This is the cohort form of the builder's per-node ``_fit_node_model``
+    (art/builder.py:812-829): every node of a cohort becomes one leaf of a
+    ``LeafRidgeBatch`` (FP64 Gram and Cholesky, trace-scaled ridge).  A node
+    whose factorization fails raises ``numpy.linalg.LinAlgError`` as the
+    reference's normal-equation solve does.
+    """
+    import torch
+
+    from .leaf_ridge import LeafRidgeBatch
+
+    degree, bias, bias_last = model_basis(template)
+    if not samples:
+        return [], []
+    d = int(np.asarray(samples[0][0]).shape[1])
+    sizes = np.array([len(y) for _, y in samples], dtype=np.int64)
+    offsets = np.concatenate([[0], np.cumsum(sizes)])
+    X = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=float) for x, _ in samples]))
+    y = np.ascontiguousarray(np.concatenate([np.asarray(v, dtype=float).reshape(-1)
+                                             for _, v in samples]))
+    dev = torch.device("cuda", torch.cuda.current_device())
+    with LeafRidgeBatch(offsets, d, degree=degree, include_bias=bias, bias_last=bias_last,
+                        stream=stream) as batch:
+        Xt = torch.from_numpy(X).to(dev)
+        coef, status, lam = batch.fit(Xt, torch.from_numpy(y).to(dev),
+                                      torch.ones(len(y), dtype=torch.float64, device=dev),
+                                      template.ridge, 1e-12)
+        coef, status, lam = coef.cpu().numpy(), status.cpu().numpy(), lam.cpu().numpy()
+    models, predictions = [], []
+    for j, (x, _) in enumerate(samples):
+        if status[j] != 0:
+            raise np.linalg.LinAlgError(
+                "Normal-equation Cholesky factorization failed; increase ridge regularization.")
+        m = template.clone()
+        m.coef_ = coef[j].copy()
+        m.solve_result_ = RidgeSolveResult(m.coef_, m.solver, "cholesky", None, None,
+                                           float(lam[j]))
+        models.append(m)
+        predictions.append(m.predict(x))
+    return models, predictions

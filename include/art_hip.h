@@ -268,6 +268,29 @@ int art_philox4x32_10(const uint32_t* counters, const uint32_t* keys, int64_t co
                       uint32_t* out, hipStream_t stream);
 
 
+/* ---------------------------------------------------------------------
+ * Tree inference: out[r] = prediction of the flattened tree at X[r, :]
+ * (X (n, d) row-major).  Split node k: w = node_w[k*d .. k*d+d), z =
+ * node_z[k], children node_child[2k] (left) and node_child[2k+1] (right);
+ * a link c < 0 names leaf -1-c.  root is the root link (a leaf when < 0,
+ * then the split arrays may be NULL).  Leaf l's model is
+ * leaf_coef[l*p .. l*p+p) on the basis (d <= 32, degree 1 or 2).  A point
+ * goes right when w.x - z >= 0.  Replaces RegressionTree.predict and
+ * SplitNode.route (art/tree.py:45-50,124-165). */
+int art_tree_predict(const double* X, int64_t n, const double* node_w, const double* node_z,
+                     const int* node_child, int root, const double* leaf_coef, art_basis basis,
+                     double* out, hipStream_t stream);
+
+/* out[s] = median over the points of set s of the distance to their nearest
+ * other point (mode 0), or median over all point pairs of their distance
+ * (mode 1).  Set s holds rows [offsets[s], offsets[s+1]) of X (rows, d);
+ * offsets lives on the device and host_offsets is its host copy (2 to 4096
+ * points per set).  An even count averages the two middle distances, as
+ * numpy.median does.  Replaces median_nearest_neighbor_distance and
+ * median_pairwise_distance (art/temperature.py:73-118). */
+int art_knn_median(const double* X, const int64_t* offsets, const int64_t* host_offsets, int sets,
+                   int d, int mode, double* out, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif
