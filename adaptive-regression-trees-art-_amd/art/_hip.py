@@ -68,6 +68,8 @@ class HarArgs(ctypes.Structure):
         ("seed", ctypes.c_uint64),
         *[(name, ctypes.c_int) for name in ("P", "d", "chains", "n", "first", "stride",
                                             "steps")],
+        ("seeds", ctypes.c_void_p),
+        ("shape_stride", ctypes.c_int),
     ]
 
 

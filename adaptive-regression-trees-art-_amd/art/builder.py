@@ -327,20 +327,24 @@ class RegressionTreeBuilder:
 
         Draws are taken node by node in list order (top-up sampling, then the
         validation permutation), exactly as preparing the nodes one at a time
-        would take them (art/builder.py:358-396); the fits, which draw
-        nothing, then run together.
+        would take them (art/builder.py:358-396); the chains of the top-ups
+        run together (``_top_up_cohort``), and so do the fits, which draw
+        nothing.
         """
         plans, requests = [], []
-        for work in jobs:
+        # A validation split draws its permutation right after its node's
+        # top-up, so those builds top up node by node.
+        validating = self.validation_fraction > 0.0
+        topped = [None] * len(jobs) if validating else self._top_up_cohort(jobs)
+        for k, work in enumerate(jobs):
+            top = self._top_up_one(work) if validating else topped[k]
             plan = {"work": work, "failed": None, "fit_rows": None}
-            if work.parent is not None:
-                inherited = work.samples
-                try:
-                    work.samples = self._top_up_child(work.region, inherited.X, inherited.y)
-                except (RuntimeError, ValueError) as exc:
-                    plan["failed"] = exc
+            if isinstance(top, Exception):
+                plan["failed"] = top
+            elif top is not None:
+                work.samples = top
             samples = work.samples
-            if plan["failed"] is None and self.validation_fraction > 0.0:
+            if plan["failed"] is None and validating:
                 n = samples.X.shape[0]
                 n_val = min(max(1, int(round(self.validation_fraction * n))), n - 1)
                 plan["fit_rows"] = self._rng.permutation(n)[n_val:]
@@ -486,6 +490,148 @@ class RegressionTreeBuilder:
             X, y = samples.X[mask], samples.y[mask]
             yield _NodeWork(child.region, _NodeSamples(X, y, n_inherited=len(X), n_new=0),
                             child.depth, child.node_id, parent=node, side=side)
+
+    def _top_up_cohort(self, jobs):
+        """Top up every child job of a cohort; returns per job its new
+        samples, the exception that made sampling fail, or None (root).
+
+        Draws follow the node-by-node order of ``_top_up_child`` (subset,
+        start row, pilot seed, main seed; then the oracle), but the chains of
+        all nodes run together: one launch for the pilots, one for the main
+        chains.  A pilot that fails would have skipped its node's main-seed
+        draw, so the cohort then restarts from the saved generator state and
+        runs node by node.
+        """
+        children = [w for w in jobs if w.parent is not None]
+        if len(children) < 2:
+            return [self._top_up_one(w) for w in jobs]
+        saved = copy.deepcopy(self._rng.bit_generator.state)
+        plans = []
+        for w in jobs:
+            if w.parent is None:
+                plans.append(None)
+                continue
+            X = np.asarray(w.samples.X, dtype=float)
+            y = np.asarray(w.samples.y, dtype=float).reshape(-1)
+            if X.shape[0] == 0:
+                plans.append(RuntimeError("Cannot seed child sampling without an inherited point."))
+                continue
+            if X.shape[0] > self._target_samples:
+                keep = self._rng.choice(X.shape[0], size=self._target_samples, replace=False)
+                X, y = X[keep], y[keep]
+            missing = self._target_samples - X.shape[0]
+            plan = {"work": w, "X": X, "y": y, "missing": missing}
+            if missing:
+                plan["x0"] = X[int(self._rng.integers(0, X.shape[0]))]
+                if self.isotropic_sampling:
+                    plan["pilot_seed"] = self._next_seed()
+                plan["seed"] = self._next_seed()
+            plans.append(plan)
+        sampling = [p for p in plans if isinstance(p, dict) and p["missing"]]
+        with self._timing_context("sampling"):
+            ok = self._sample_cohort(sampling)
+        if not ok:
+            self._rng.bit_generator.state = saved
+            return [self._top_up_one(w) for w in jobs]
+        out = []
+        for p in plans:
+            if p is None or isinstance(p, Exception):
+                out.append(p)
+                continue
+            if not p["missing"]:
+                out.append(_NodeSamples(p["X"], p["y"], p["X"].shape[0], 0,
+                                        _sampling_info("inherited_only")))
+                continue
+            if isinstance(p["new"], Exception):
+                out.append(p["new"])
+                continue
+            y_new = self._query_oracle(p["new"])
+            out.append(_NodeSamples(np.vstack([p["X"], p["new"]]),
+                                    np.concatenate([p["y"], y_new]), p["X"].shape[0],
+                                    p["missing"], p["info"]))
+        return out
+
+    def _sample_cohort(self, plans):
+        """Run the sampling of ``_sample_region_untimed`` for every plan in
+        batched launches; sets plan["new"] (array or exception) and
+        plan["info"].  False when a pilot failure changes the draw order."""
+        from .sampling import sample_hit_and_run_many
+
+        d = self.dimension
+        if not plans:
+            return True
+        if not self.isotropic_sampling:
+            if not isinstance(self.sampler, HitAndRunSampler):
+                for p in plans:
+                    try:
+                        X = np.asarray(self.sampler.sample(p["work"].region, p["missing"],
+                                                           random_state=p["seed"], x0=p["x0"]),
+                                       dtype=float)
+                        p["new"] = self._checked(p["work"].region, X, p["missing"])
+                    except (RuntimeError, ValueError) as exc:
+                        p["new"] = exc
+                    p["info"] = _sampling_info(type(self.sampler).__name__,
+                                               getattr(self.sampler, "thinning", None))
+                return True
+            got = sample_hit_and_run_many([self.sampler] * len(plans),
+                                          [p["work"].region for p in plans],
+                                          [p["missing"] for p in plans],
+                                          [p["seed"] for p in plans], [p["x0"] for p in plans])
+            for p, X in zip(plans, got):
+                p["new"] = X if isinstance(X, Exception) else self._checked(p["work"].region, X,
+                                                                            p["missing"])
+                p["info"] = _sampling_info(type(self.sampler).__name__, self.sampler.thinning)
+            return True
+        pilot_length = self.isotropic_pilot_multiplier * d
+        pilot_sampler = replace(self.sampler, burn_in=0, thinning=1,
+                                direction_eigenvectors=None, direction_eigenvalues=None)
+        pilots = sample_hit_and_run_many([pilot_sampler] * len(plans),
+                                         [p["work"].region for p in plans],
+                                         [pilot_length] * len(plans),
+                                         [p["pilot_seed"] for p in plans],
+                                         [p["x0"] for p in plans])
+        shaped = []
+        for p, pilot in zip(plans, pilots):
+            if isinstance(pilot, Exception):
+                return False
+            try:
+                vectors, values = sample_covariance_eigendecomposition(pilot)
+                safe, floor = floor_covariance_eigenvalues(
+                    values, self.sampler.direction_eigenvalue_floor)
+            except (RuntimeError, ValueError):
+                return False
+            saturated = bool(np.any(values <= floor))
+            warning = "covariance_eigenvalue_floor_saturated"
+            p["info"] = {**_sampling_info("isotropic_hit_and_run", self.sampler.thinning,
+                                          pilot_length, float(np.max(safe) / np.min(safe)),
+                                          (warning,) if saturated else ()),
+                         "sampling_covariance_eigenvalue_floor": floor,
+                         "sampling_covariance_floor_saturated": saturated}
+            shaped.append((replace(self.sampler, direction_eigenvectors=vectors,
+                                   direction_eigenvalues=values), pilot[-1]))
+        got = sample_hit_and_run_many([sm for sm, _ in shaped], [p["work"].region for p in plans],
+                                      [p["missing"] for p in plans], [p["seed"] for p in plans],
+                                      [x for _, x in shaped])
+        for p, X in zip(plans, got):
+            p["new"] = X if isinstance(X, Exception) else self._checked(p["work"].region, X,
+                                                                        p["missing"])
+        return True
+
+    def _checked(self, region, X, n):
+        if X.shape != (n, self.dimension):
+            return RuntimeError(f"Sampler returned shape {X.shape}; expected "
+                                f"{(n, self.dimension)}.")
+        if not np.all(region.contains(X)):
+            return RuntimeError("Sampler returned points outside the region.")
+        return X
+
+    def _top_up_one(self, work):
+        if work.parent is None:
+            return None
+        try:
+            return self._top_up_child(work.region, work.samples.X, work.samples.y)
+        except (RuntimeError, ValueError) as exc:
+            return exc
 
     def _top_up_child(self, region, inherited_X, inherited_y) -> _NodeSamples:
         """Keep at most target_samples inherited rows and sample the rest."""

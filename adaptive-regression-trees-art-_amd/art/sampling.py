@@ -213,12 +213,14 @@ class PolytopeBatch:
         self.last_kernel_ms = None
 
     def sample(self, x0, n, *, first, stride, seed=0, shape=None, directions=None,
-               uniforms=None, out=None, timed=False):
+               uniforms=None, out=None, timed=False, seeds=None):
         """Samples (P, n, d) on the device for starts x0 (P, C, d) (host or device).
 
         Chain c records after moves first, first + stride, ...; record k is
         sample k*C + c.  ``directions (P, S, C, d)`` / ``uniforms (P, S, C)``
-        replace the Philox stream when given.
+        replace the Philox stream when given.  ``shape`` is one (d, d)
+        shaping matrix or one per polytope (P, d, d); ``seeds`` (P,) gives
+        each polytope the stream a one-polytope call with that seed draws.
         """
         torch, P, d = self.torch, self.P, self.d
         dev = dict(dtype=torch.float64, device=self.device)
@@ -240,10 +242,19 @@ class PolytopeBatch:
         keep = [x0]
         if shape is not None:
             sh = torch.as_tensor(np.asarray(shape, dtype=float), **dev).contiguous()
-            if tuple(sh.shape) != (d, d):
-                raise ValueError(f"shape must be ({d}, {d}).")
+            if tuple(sh.shape) == (P, d, d):
+                a.shape_stride = d * d
+            elif tuple(sh.shape) != (d, d):
+                raise ValueError(f"shape must be ({d}, {d}) or ({P}, {d}, {d}).")
             a.shape = _hip.ptr(sh)
             keep.append(sh)
+        if seeds is not None:
+            sd = np.ascontiguousarray(np.asarray(seeds, dtype=np.uint64).reshape(-1))
+            if sd.shape[0] != P:
+                raise ValueError(f"seeds must have shape ({P},).")
+            st = torch.from_numpy(sd.view(np.int64)).to(self.device)
+            a.seeds = _hip.ptr(st)
+            keep.append(st)
         a.steps = 0
         if directions is not None:
             dr = torch.as_tensor(directions, **dev).contiguous()
@@ -458,3 +469,55 @@ def sample_polytopes(regions, n, *, n_chains=64, burn_in=0, thinning=10, seed=0,
         x0 = np.repeat(x0[:, None, :], chains, axis=1)
     stride = max(1, int(thinning))
     return batch.sample(x0, n, first=int(burn_in) + stride, stride=stride, seed=seed, shape=shape)
+
+
+def sample_hit_and_run_many(samplers, regions, ns, random_states, x0s):
+    """``samplers[k].sample(regions[k], ns[k], random_states[k], x0s[k])`` for
+    every k, with one GPU launch per (burn_in, thinning) group.
+
+    Every request must give its start x0.  Request k draws exactly what its
+    own ``sample`` call draws (one Philox seed from random_states[k]) and gets
+    the same samples: per-polytope keys and shaping matrices make the chains
+    of one launch independent, and a chain recording fewer samples than the
+    group's longest is the prefix of the longer run.  Returns a list of (n_k,
+    d) host arrays, or the exception a request raised, per request.
+    """
+    out = [None] * len(regions)
+    groups = {}
+    for k, (s, region, n, x0) in enumerate(zip(samplers, regions, ns, x0s)):
+        try:
+            if n < 0:
+                raise ValueError("n must be nonnegative.")
+            if s.burn_in < 0:
+                raise ValueError("burn_in must be nonnegative.")
+            if s.thinning < 1:
+                raise ValueError("thinning must be at least 1.")
+            rng = as_rng(random_states[k])
+            shape = s._shaping(region.dimension)
+            x = np.asarray(x0, dtype=float).reshape(-1)
+            if x.shape[0] != region.dimension:
+                raise ValueError(f"x0 must have shape ({region.dimension},), got {x.shape}.")
+            if not bool(region.contains(x, tol=s.feasibility_tol)[0]):
+                raise ValueError("x0 is not feasible for the region.")
+            if int(n) == 0:
+                out[k] = np.empty((0, region.dimension))
+                continue
+            seed = int(rng.integers(0, np.iinfo(np.int64).max))
+        except (ValueError, RuntimeError) as exc:
+            out[k] = exc
+            continue
+        groups.setdefault((s.burn_in, s.thinning, region.dimension), []).append(
+            (k, region, int(n), x, seed, shape))
+    for (burn_in, thinning, d), items in groups.items():
+        n_max = max(n for _, _, n, _, _, _ in items)
+        shapes = None
+        if any(sh is not None for *_, sh in items):
+            shapes = np.stack([np.eye(d) if sh is None else sh for *_, sh in items])
+        batch = PolytopeBatch([region for _, region, *_ in items])
+        x0 = np.stack([x for _, _, _, x, _, _ in items])[:, None, :]
+        res = batch.sample(x0, n_max, first=burn_in + 1, stride=thinning,
+                           seeds=[seed for *_, seed, _ in items], shape=shapes)
+        host = res.cpu().numpy()
+        for i, (k, _, n, *_rest) in enumerate(items):
+            out[k] = host[i, :n].copy()
+    return out

@@ -166,3 +166,46 @@ def split_workload(n_nodes=2048, n=8192, d=16, n_bumps=8, seed=3):
     for k in range(n_nodes):
         y[k] = BumpMixture.random(rng, n_bumps, d)(X[k])
     return X, y
+
+
+class PlainFitOracle:
+    """The 2-D plain-fit target: f(x) = sin(3 x_1) cos(2 x_2)
+    + 0.5 exp(-8 |x - (0.3, -0.2)|^2), plus N(0, noise^2) from a seeded
+    generator (every call draws fresh noise, like a real experiment)."""
+
+    def __init__(self, noise=1e-3, seed=0):
+        self.noise = float(noise)
+        self.rng = np.random.default_rng(seed)
+
+    @staticmethod
+    def clean(X):
+        X = np.asarray(X, dtype=float)
+        r2 = (X[..., 0] - 0.3) ** 2 + (X[..., 1] + 0.2) ** 2
+        return np.sin(3.0 * X[..., 0]) * np.cos(2.0 * X[..., 1]) + 0.5 * np.exp(-8.0 * r2)
+
+    def __call__(self, X):
+        f = self.clean(X)
+        return f + self.noise * self.rng.normal(size=np.shape(f))
+
+
+def plain_fit_builder(seed=0, node_batch_size=32, error_threshold=1e-2, **overrides):
+    """Builder of the plain ART fit: [-1, 1]^2, 4000 samples per node, soft
+    oblique splits at T = 0.05 on the batched device optimizer (BFS cohorts),
+    quadratic ridge leaves (lambda = 1e-6), max depth 6, at least 64 points
+    per side, relative L2 stopping error 1e-2."""
+    from .builder import RegressionTreeBuilder
+    from .domain import BoxDomain
+    from .models import PolynomialRidgeModel
+    from .splitters import SoftObliqueSplitter
+    from .temperature import TemperatureConfig
+
+    opts = dict(
+        domain=BoxDomain.hypercube(2, -1.0, 1.0), oracle=PlainFitOracle(seed=seed),
+        oracle_vectorized=True, model_template=PolynomialRidgeModel(2, ridge=1e-6),
+        splitter=SoftObliqueSplitter(PolynomialRidgeModel(2, ridge=1e-6), 0.05,
+                                     min_side_points=64, optimizer="cupy_batched_cholesky"),
+        temperature_config=TemperatureConfig(strategy="splitter"),
+        error_threshold=error_threshold, sample_count=4000, max_depth=6, traversal="bfs",
+        node_batch_size=node_batch_size, random_state=seed)
+    opts.update(overrides)
+    return RegressionTreeBuilder(**opts)

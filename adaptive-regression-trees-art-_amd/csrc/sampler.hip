@@ -79,14 +79,16 @@ __global__ __launch_bounds__(64) void har_kernel(art_har_args a) {
   const int m = (int)(offs[poly + 1] - r0);
   cf64* A = (cf64*)(a.A + r0 * d);
   cf64* b = (cf64*)(a.b + r0);
-  cf64* shape = (cf64*)a.shape;
+  cf64* shape = (cf64*)(a.shape ? a.shape + (size_t)poly * a.shape_stride : nullptr);
   cf64* box_lo = (cf64*)(a.box_lo ? a.box_lo + (size_t)poly * d : nullptr);
   cf64* box_hi = (cf64*)(a.box_hi ? a.box_hi + (size_t)poly * d : nullptr);
   const int full = a.n / C, rem = a.n % C;
   const int records = full + (c < rem ? 1 : 0);
   const int moves = a.first + (records - 1) * a.stride;
   const int W = har_calls_per_step(d);
-  const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
+  const uint64_t key = a.seeds ? a.seeds[poly] : a.seed;
+  const uint32_t k0 = (uint32_t)key, k1 = (uint32_t)(key >> 32);
+  const uint32_t stream_poly = a.seeds ? 0u : (uint32_t)poly;
   const bool fixed = a.directions != nullptr;
 
   double x[DM];
@@ -110,7 +112,7 @@ __global__ __launch_bounds__(64) void har_kernel(art_har_args a) {
       for (int q = 0; q < WM; ++q) {
         uint4 r = make_uint4(0u, 0u, 0u, 0u);
         if (EXACT || q < W)
-          r = philox4x32_10(make_uint4((uint32_t)(s * W + q), (uint32_t)c, (uint32_t)poly, 0u), k0,
+          r = philox4x32_10(make_uint4((uint32_t)(s * W + q), (uint32_t)c, stream_poly, 0u), k0,
                             k1);
         w[4 * q] = r.x;
         w[4 * q + 1] = r.y;
@@ -260,6 +262,8 @@ extern "C" int art_har_sample(const art_har_args* a, hipStream_t stream) {
     return fail(ART_ERR_ARG, "art_har_sample: box_lo and box_hi go together");
   if ((a->directions == nullptr) != (a->uniforms == nullptr))
     return fail(ART_ERR_ARG, "art_har_sample: directions and uniforms go together");
+  if (a->shape_stride < 0 || (a->shape_stride > 0 && (!a->shape || a->shape_stride < a->d * a->d)))
+    return fail(ART_ERR_ARG, "art_har_sample: shape_stride must be 0 or at least d*d with a shape");
   const int full = a->n / a->chains, rem = a->n % a->chains;
   const int64_t moves = a->first + (int64_t)(full + (rem ? 1 : 0) - 1) * a->stride;
   if (moves > (int64_t)0x7fffffff / har_calls_per_step(32))

@@ -38,9 +38,10 @@ def parse():
     ap.add_argument("--leaves", type=int, default=4096)
     ap.add_argument("--chunk-rows", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--only", choices=("all", "ridge", "sampler", "split"), default="all",
+    ap.add_argument("--only", choices=("all", "ridge", "sampler", "split", "tree"), default="all",
                     help="run one leg (profiling); the JSON line keeps the ridge metric")
     ap.add_argument("--sampler-steps", type=int, default=3)
+    ap.add_argument("--tree-steps", type=int, default=3)
     ap.add_argument("--polytopes", type=int, default=65536)
     ap.add_argument("--split-nodes", type=int, default=2048,
                     help="nodes of the annealed split leg (64 restarts each)")
@@ -269,6 +270,78 @@ def bench_split(args):
     return result
 
 
+def bench_tree(args):
+    """Plain-fit leg: whole tree builds of the 2-D plain ART fit (4000 samples
+    per node, soft oblique T=0.05 on the batched device optimizer, quadratic
+    leaves, depth <= 6, >= 64 points per side), then 256 hit-and-run samples
+    in every leaf polytope (one launch); a step is one build plus the leaf
+    sampling."""
+    import torch
+
+    from art.sampling import PolytopeBatch
+    from art.synthetic import PlainFitOracle, plain_fit_builder
+
+    steps = max(1, args.tree_steps)
+
+    # Leaf sampling starts: the leaf's own sample mean lies inside its convex
+    # region (the region is the intersection of half-spaces containing them).
+    def leaf_samples(res):
+        leaves = list(res.tree.iter_leaves())
+        x0 = np.stack([leaf.metadata["sample_centroid"] for leaf in leaves])
+        batch = PolytopeBatch([leaf.region for leaf in leaves])
+        return batch.sample(torch.from_numpy(x0[:, None, :]).to("cuda"), 256, first=1,
+                            stride=20, seed=7), len(leaves)
+
+    def build(seed):
+        res = plain_fit_builder(seed=seed, store_leaf_representative=True,
+                                profile_build_timing=True).build()
+        samples, n_leaves = leaf_samples(res)
+        torch.cuda.synchronize()
+        return res, samples, n_leaves
+
+    for w in range(max(0, args.warmup)):
+        build(1000 + w)
+    times, results = [], []
+    for k in range(steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res, samples, n_leaves = build(k)
+        times.append(time.perf_counter() - t0)
+        results.append((res, samples, n_leaves))
+    ms = 1e3 * float(np.mean(times))
+    res, samples, n_leaves = results[0]
+    tree = res.tree
+    Xt = np.random.default_rng(11).uniform(-1, 1, (20000, 2))
+    pred = tree.predict(Xt)
+    host = tree._predict_models(Xt)
+    rmse = float(np.sqrt(np.mean((pred - PlainFitOracle.clean(Xt)) ** 2)))
+    leaves = list(tree.iter_leaves())
+    inside = all(leaf.region.contains(samples[i].cpu().numpy()).all()
+                 for i, leaf in enumerate(leaves))
+    print(f"[bench] tree {ms:.1f} ms/build, {tree.num_leaves()} leaves, rmse {rmse:.2e}",
+          file=sys.stderr, flush=True)
+    result = {
+        "metric": "tree builds/s", "value": 1e3 / ms, "unit": "tree builds/s",
+        "ms_per_step": ms, "steps": steps,
+        "latency_ms": {"p50": float(np.percentile(times, 50)) * 1e3,
+                       "p99": float(np.percentile(times, 99)) * 1e3},
+        "dtype": "f64",
+        "config": {"workload": ("plain ART fit: d=2 on [-1,1]^2, 4000 uniform samples per node "
+                                "(seed k for build k), f = sin(3x1)cos(2x2) + 0.5 exp(-8|x-(0.3,"
+                                "-0.2)|^2) + N(0, 1e-6) noise, soft oblique T=0.05 (batched "
+                                "device optimizer, BFS cohorts of 32), quadratic ridge 1e-6, "
+                                "depth <= 6, >= 64 points per side, relative L2 stop 1e-2, "
+                                "256 hit-and-run samples per leaf polytope"),
+                   "nodes": tree.num_nodes(), "leaves": n_leaves,
+                   "oracle_queries": res.oracle_queries},
+        "build_timing": tree.metadata.get("build_timing"),
+        "test_rmse_vs_clean_target": rmse,
+        "gpu_vs_host_prediction_max_abs": float(np.max(np.abs(pred - host))),
+        "leaf_samples_inside_regions": bool(inside),
+    }
+    return result
+
+
 def rel_err(a, b):
     return np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-300)
 
@@ -292,6 +365,9 @@ def main():
         return
     if args.only == "split":
         print(json.dumps(bench_split(args)))
+        return
+    if args.only == "tree":
+        print(json.dumps(bench_tree(args)))
         return
     wl = leaf_ridge_workload(n_leaves=args.leaves, seed=2)
     dev = torch.device("cuda", 0)

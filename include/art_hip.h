@@ -255,6 +255,12 @@ typedef struct {
   int first;
   int stride;
   int steps;
+  /* Per-polytope streams and shapes (both optional): seeds (P,) gives
+   * polytope q the key seeds[q] and counter (move * W + w, chain, 0, 0), the
+   * stream a one-polytope call with seed seeds[q] draws; shape_stride > 0
+   * reads polytope q's shaping matrix at shape + q * shape_stride. */
+  const uint64_t* seeds;
+  int shape_stride;
 } art_har_args;
 
 int art_har_sample(const art_har_args* args, hipStream_t stream);

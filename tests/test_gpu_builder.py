@@ -246,3 +246,58 @@ def test_batched_bfs_build_end_to_end(torch_gpu):
     err = np.sqrt(np.mean((gpu - oracle(X)) ** 2))
     assert err < 0.3
     assert tree.metadata["traversal"] == "bfs" and tree.metadata["node_batch_size"] == 4
+
+
+def test_many_chain_requests_equal_their_single_calls(torch_gpu):
+    # One launch for several regions, lengths, seeds and shapes gives each
+    # request exactly the samples of its own HitAndRunSampler.sample call.
+    from art.domain import BoxDomain, split_region
+    from art.sampling import HitAndRunSampler, sample_hit_and_run_many
+
+    box = BoxDomain.hypercube(3, -1.0, 1.0).as_region()
+    left, right = split_region(box, np.array([0.6, 0.8, 0.0]), 0.1)
+    lower, _ = split_region(left, np.array([0.0, 0.3, -1.0]) / np.hypot(0.3, 1.0), -0.2)
+    regions = [box, left, right, lower]
+    rng = np.random.default_rng(3)
+    V = np.linalg.qr(rng.normal(size=(3, 3)))[0]
+    samplers = [HitAndRunSampler(burn_in=0, thinning=7),
+                HitAndRunSampler(burn_in=0, thinning=7, direction_eigenvectors=V,
+                                 direction_eigenvalues=np.array([1.0, 0.2, 0.05])),
+                HitAndRunSampler(burn_in=0, thinning=7),
+                HitAndRunSampler(burn_in=0, thinning=7, direction_eigenvectors=np.eye(3),
+                                 direction_eigenvalues=np.array([0.3, 1.0, 0.6]))]
+    starts = [np.zeros(3), np.array([-0.5, -0.2, 0.0]), np.array([0.6, 0.5, 0.1]),
+              np.array([-0.6, -0.4, 0.5])]
+    ns, seeds = [40, 13, 0, 77], [11, 12, 13, 14]
+    got = sample_hit_and_run_many(samplers, regions, ns, seeds, starts)
+    for s, r, n, seed, x0, g in zip(samplers, regions, ns, seeds, starts, got):
+        np.testing.assert_array_equal(g, s.sample(r, n, random_state=seed, x0=x0))
+        assert r.contains(g).all()
+
+
+@pytest.mark.parametrize("isotropic", [True, False])
+def test_cohort_top_ups_equal_node_by_node_top_ups(torch_gpu, isotropic):
+    from art.builder import RegressionTreeBuilder
+    from art.domain import BoxDomain
+    from art.models import PolynomialRidgeModel
+    from art.sampling import HitAndRunSampler
+
+    class NodeByNode(RegressionTreeBuilder):
+        def _top_up_cohort(self, jobs):
+            return [self._top_up_one(w) for w in jobs]
+
+    def build(cls):
+        return cls(domain=BoxDomain.hypercube(2, -1.0, 1.0), oracle=_target,
+                   oracle_vectorized=True, model_template=PolynomialRidgeModel(2, ridge=1e-8),
+                   splitter=MedianAxisSplitter(), error_threshold=0.02, sample_count=150,
+                   max_depth=4, sampler=HitAndRunSampler(burn_in=0, thinning=4),
+                   isotropic_sampling=isotropic, random_state=21, traversal="bfs",
+                   node_batch_size=8, store_samples=True).build()
+
+    a, b = build(RegressionTreeBuilder), build(NodeByNode)
+    na, nb = list(a.tree.iter_nodes()), list(b.tree.iter_nodes())
+    assert [n.node_id for n in na] == [n.node_id for n in nb] and len(na) > 7
+    assert a.oracle_queries == b.oracle_queries
+    for u, v in zip(na, nb):
+        np.testing.assert_array_equal(u.metadata["X"], v.metadata["X"])
+        assert u.metadata["sampling_method"] == v.metadata["sampling_method"]
