@@ -70,6 +70,7 @@ class HarArgs(ctypes.Structure):
                                             "steps")],
         ("seeds", ctypes.c_void_p),
         ("shape_stride", ctypes.c_int),
+        ("row_bound", ctypes.c_int),
     ]
 
 

@@ -503,7 +503,9 @@ class RegressionTreeBuilder:
         runs node by node.
         """
         children = [w for w in jobs if w.parent is not None]
-        if len(children) < 2:
+        # Only the GPU sampler itself batches; a subclass with its own sample
+        # (or any other sampler) is called node by node.
+        if len(children) < 2 or type(self.sampler).sample is not HitAndRunSampler.sample:
             return [self._top_up_one(w) for w in jobs]
         saved = copy.deepcopy(self._rng.bit_generator.state)
         plans = []
@@ -561,18 +563,6 @@ class RegressionTreeBuilder:
         if not plans:
             return True
         if not self.isotropic_sampling:
-            if not isinstance(self.sampler, HitAndRunSampler):
-                for p in plans:
-                    try:
-                        X = np.asarray(self.sampler.sample(p["work"].region, p["missing"],
-                                                           random_state=p["seed"], x0=p["x0"]),
-                                       dtype=float)
-                        p["new"] = self._checked(p["work"].region, X, p["missing"])
-                    except (RuntimeError, ValueError) as exc:
-                        p["new"] = exc
-                    p["info"] = _sampling_info(type(self.sampler).__name__,
-                                               getattr(self.sampler, "thinning", None))
-                return True
             got = sample_hit_and_run_many([self.sampler] * len(plans),
                                           [p["work"].region for p in plans],
                                           [p["missing"] for p in plans],

@@ -210,6 +210,7 @@ class PolytopeBatch:
             self.box_lo = None if lo is None else torch.from_numpy(lo).to(self.device)
             self.box_hi = None if hi is None else torch.from_numpy(hi).to(self.device)
         self.host_offsets = offsets
+        self.max_rows = int(np.max(np.diff(offsets))) if self.P else 0
         self.last_kernel_ms = None
 
     def sample(self, x0, n, *, first, stride, seed=0, shape=None, directions=None,
@@ -270,6 +271,7 @@ class PolytopeBatch:
         elif tuple(out.shape) != (P, n, d) or out.dtype != torch.float64 or not out.is_contiguous():
             raise ValueError(f"out must be a contiguous float64 tensor of shape ({P}, {n}, {d}).")
         a.samples = _hip.ptr(out)
+        a.row_bound = self.max_rows + 1
         a.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
         a.P, a.d, a.chains, a.n, a.first, a.stride = P, d, C, n, first, stride
         with torch.cuda.stream(self.stream):

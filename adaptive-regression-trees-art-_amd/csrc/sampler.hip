@@ -64,10 +64,93 @@ __global__ __launch_bounds__(256) void philox_kernel(const uint4* __restrict__ c
 typedef const __attribute__((address_space(4))) double cf64;
 typedef const __attribute__((address_space(4))) int64_t ci64;
 
+// Direction (shaped) and chord uniform of move s of chain c: the Philox
+// stream or the fixed arrays.  Both kernels call this, so a chain draws the
+// same bits whichever kernel runs it.
+template <int DM, bool EXACT>
+__device__ __forceinline__ void move_randoms(const art_har_args& a, cf64* shape, int d, int W,
+                                             int poly, uint32_t stream_poly, int c, int s,
+                                             uint32_t k0, uint32_t k1, double (&dir)[DM],
+                                             double& u) {
+  constexpr int WM = har_calls_per_step(DM);
+  const int C = a.chains;
+  if (a.directions != nullptr) {
+    const size_t row = ((size_t)poly * a.steps + s) * C + c;
+    const double* dp = a.directions + row * d;
+#pragma unroll
+    for (int j = 0; j < DM; ++j) dir[j] = (EXACT || j < d) ? dp[j] : 0.0;
+    u = a.uniforms[row];
+  } else {
+    uint32_t w[4 * WM];
+#pragma unroll
+    for (int q = 0; q < WM; ++q) {
+      uint4 r = make_uint4(0u, 0u, 0u, 0u);
+      if (EXACT || q < W)
+        r = philox4x32_10(make_uint4((uint32_t)(s * W + q), (uint32_t)c, stream_poly, 0u), k0,
+                          k1);
+      w[4 * q] = r.x;
+      w[4 * q + 1] = r.y;
+      w[4 * q + 2] = r.z;
+      w[4 * q + 3] = r.w;
+    }
+    // Box-Muller pairs from words (2q, 2q+1): u1 in (0, 1] keeps the log
+    // finite; the hardware sine and cosine take revolutions, so
+    // sin(2 pi u2) is v_sin_f32(u2).
+#pragma unroll
+    for (int q = 0; q < (DM + 1) / 2; ++q) {
+      const float u1 = (float)((w[2 * q] >> 8) + 1u) * 0x1p-24f;
+      const float u2 = (float)(w[2 * q + 1] >> 8) * 0x1p-24f;
+      const float r = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));
+      dir[2 * q] = (double)(r * __builtin_amdgcn_cosf(u2));
+      if (2 * q + 1 < DM) dir[2 * q + 1] = (double)(r * __builtin_amdgcn_sinf(u2));
+    }
+    // The chord uniform follows the ceil(d/2) pairs.
+    uint32_t wa, wb;
+    if (EXACT) {
+      wa = w[2 * ((DM + 1) / 2)];
+      wb = w[2 * ((DM + 1) / 2) + 1];
+    } else {
+      const int P = (d + 1) / 2;
+      wa = wb = 0u;
+#pragma unroll
+      for (int q = 0; q < (DM + 1) / 2; ++q)
+        if (q == P - 1) {
+          wa = w[2 * q + 2];
+          wb = w[2 * q + 3];
+        }
+    }
+    u = ((double)(wa >> 5) * 67108864.0 + (double)(wb >> 6)) * 0x1p-53;
+    if (!EXACT) {
+#pragma unroll
+      for (int j = 0; j < DM; ++j)
+        if (j >= d) dir[j] = 0.0;
+    }
+  }
+  if (shape) {
+    double t[DM];
+#pragma unroll
+    for (int i = 0; i < DM; ++i) {
+      double acc = 0.0;
+#pragma unroll
+      for (int j = 0; j < DM; ++j)
+        if (EXACT || (i < d && j < d)) acc = fma(shape[i * d + j], dir[j], acc);
+      t[i] = acc;
+    }
+#pragma unroll
+    for (int i = 0; i < DM; ++i) dir[i] = t[i];
+  }
+}
+
+__device__ __forceinline__ double rcp2(double v) {
+  double r = __builtin_amdgcn_rcp(v);
+  r = fma(r, fma(-v, r, 1.0), r);
+  r = fma(r, fma(-v, r, 1.0), r);
+  return r;
+}
+
 // DM = the compile-time dimension; EXACT: d == DM (no per-column predicates).
 template <int DM, bool EXACT>
 __global__ __launch_bounds__(64) void har_kernel(art_har_args a) {
-  constexpr int WM = har_calls_per_step(DM);
   const int d = EXACT ? DM : a.d;
   const int C = a.chains;
   const int wpp = (C + 63) / 64;
@@ -89,7 +172,6 @@ __global__ __launch_bounds__(64) void har_kernel(art_har_args a) {
   const uint64_t key = a.seeds ? a.seeds[poly] : a.seed;
   const uint32_t k0 = (uint32_t)key, k1 = (uint32_t)(key >> 32);
   const uint32_t stream_poly = a.seeds ? 0u : (uint32_t)poly;
-  const bool fixed = a.directions != nullptr;
 
   double x[DM];
   const double* x0 = a.x0 + ((size_t)poly * C + c) * d;
@@ -100,71 +182,7 @@ __global__ __launch_bounds__(64) void har_kernel(art_har_args a) {
   for (int s = 0; s < moves; ++s) {
     double dir[DM];
     double u;
-    if (fixed) {
-      const size_t row = ((size_t)poly * a.steps + s) * C + c;
-      const double* dp = a.directions + row * d;
-#pragma unroll
-      for (int j = 0; j < DM; ++j) dir[j] = (EXACT || j < d) ? dp[j] : 0.0;
-      u = a.uniforms[row];
-    } else {
-      uint32_t w[4 * WM];
-#pragma unroll
-      for (int q = 0; q < WM; ++q) {
-        uint4 r = make_uint4(0u, 0u, 0u, 0u);
-        if (EXACT || q < W)
-          r = philox4x32_10(make_uint4((uint32_t)(s * W + q), (uint32_t)c, stream_poly, 0u), k0,
-                            k1);
-        w[4 * q] = r.x;
-        w[4 * q + 1] = r.y;
-        w[4 * q + 2] = r.z;
-        w[4 * q + 3] = r.w;
-      }
-      // Box-Muller pairs from words (2q, 2q+1): u1 in (0, 1] keeps the log
-      // finite; the hardware sine and cosine take revolutions, so
-      // sin(2 pi u2) is v_sin_f32(u2).
-#pragma unroll
-      for (int q = 0; q < (DM + 1) / 2; ++q) {
-        const float u1 = (float)((w[2 * q] >> 8) + 1u) * 0x1p-24f;
-        const float u2 = (float)(w[2 * q + 1] >> 8) * 0x1p-24f;
-        const float r = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));
-        dir[2 * q] = (double)(r * __builtin_amdgcn_cosf(u2));
-        if (2 * q + 1 < DM) dir[2 * q + 1] = (double)(r * __builtin_amdgcn_sinf(u2));
-      }
-      // The chord uniform follows the ceil(d/2) pairs.
-      uint32_t wa, wb;
-      if (EXACT) {
-        wa = w[2 * ((DM + 1) / 2)];
-        wb = w[2 * ((DM + 1) / 2) + 1];
-      } else {
-        const int P = (d + 1) / 2;
-        wa = wb = 0u;
-#pragma unroll
-        for (int q = 0; q < (DM + 1) / 2; ++q)
-          if (q == P - 1) {
-            wa = w[2 * q + 2];
-            wb = w[2 * q + 3];
-          }
-      }
-      u = ((double)(wa >> 5) * 67108864.0 + (double)(wb >> 6)) * 0x1p-53;
-      if (!EXACT) {
-#pragma unroll
-        for (int j = 0; j < DM; ++j)
-          if (j >= d) dir[j] = 0.0;
-      }
-    }
-    if (shape) {
-      double t[DM];
-#pragma unroll
-      for (int i = 0; i < DM; ++i) {
-        double acc = 0.0;
-#pragma unroll
-        for (int j = 0; j < DM; ++j)
-          if (EXACT || (i < d && j < d)) acc = fma(shape[i * d + j], dir[j], acc);
-        t[i] = acc;
-      }
-#pragma unroll
-      for (int i = 0; i < DM; ++i) dir[i] = t[i];
-    }
+    move_randoms<DM, EXACT>(a, shape, d, W, poly, stream_poly, c, s, k0, k1, dir, u);
     double tlo = -INFINITY, thi = INFINITY;
     if (box_lo) {
       // Axis rows x_j <= hi_j and -x_j <= -lo_j: slack and direction product
@@ -219,6 +237,124 @@ __global__ __launch_bounds__(64) void har_kernel(art_har_args a) {
   }
 }
 
+// One chain per polytope (the scalar sampler): the wave splits each block
+// of 64 moves.  Lane l first draws move s0 + l's direction and uniform and
+// the move's state-free quantities -- A_i.d with its reciprocal for every
+// row and the reciprocal of each direction component -- into LDS; then
+// every lane replays the 64 moves of the one chain from LDS (the same values
+// in every lane), so a move's critical path is only the slacks, the chord
+// and the step.  Each quantity is computed by the same code as in
+// har_kernel, so the chain is bit-identical to har_kernel's.
+// LDS entry of a move: dir[DM], 1/dir[DM], u, A_i.d[m], 1/(A_i.d)[m].
+constexpr int kChainBlock = 64;
+
+template <int DM, bool EXACT>
+__global__ __launch_bounds__(64) void har_chain_kernel(art_har_args a, int E) {
+  extern __shared__ double chain_lds[];
+  const int d = EXACT ? DM : a.d;
+  const int poly = blockIdx.x, lane = (int)threadIdx.x;
+  ci64* offs = (ci64*)a.row_offsets;
+  const int64_t r0 = offs[poly];
+  const int m = (int)(offs[poly + 1] - r0);
+  cf64* A = (cf64*)(a.A + r0 * d);
+  cf64* b = (cf64*)(a.b + r0);
+  cf64* shape = (cf64*)(a.shape ? a.shape + (size_t)poly * a.shape_stride : nullptr);
+  cf64* box_lo = (cf64*)(a.box_lo ? a.box_lo + (size_t)poly * d : nullptr);
+  cf64* box_hi = (cf64*)(a.box_hi ? a.box_hi + (size_t)poly * d : nullptr);
+  const int moves = a.first + (a.n - 1) * a.stride;
+  const int W = har_calls_per_step(d);
+  const uint64_t key = a.seeds ? a.seeds[poly] : a.seed;
+  const uint32_t k0 = (uint32_t)key, k1 = (uint32_t)(key >> 32);
+  const uint32_t stream_poly = a.seeds ? 0u : (uint32_t)poly;
+
+  double x[DM];
+  const double* x0 = a.x0 + (size_t)poly * d;
+#pragma unroll
+  for (int j = 0; j < DM; ++j) x[j] = (EXACT || j < d) ? x0[j] : 0.0;
+  double* out = a.samples + (size_t)poly * a.n * d;
+
+  for (int s0 = 0; s0 < moves; s0 += kChainBlock) {
+    const int cnt = min(kChainBlock, moves - s0);
+    if (lane < cnt) {
+      double dir[DM], u;
+      move_randoms<DM, EXACT>(a, shape, d, W, poly, stream_poly, 0, s0 + lane, k0, k1, dir, u);
+      double* e = chain_lds + (size_t)lane * E;
+#pragma unroll
+      for (int j = 0; j < DM; ++j)
+        if (EXACT || j < d) {
+          e[j] = dir[j];
+          e[DM + j] = rcp2(dir[j]);
+        }
+      e[2 * DM] = u;
+      double* ea = e + 2 * DM + 1;
+      for (int i = 0; i < m; ++i) {
+        double ad = 0.0;
+#pragma unroll
+        for (int j = 0; j < DM; ++j)
+          if (EXACT || j < d) ad = fma(A[i * d + j], dir[j], ad);
+        ea[i] = ad;
+        ea[m + i] = rcp2(ad);
+      }
+    }
+    __syncthreads();
+    for (int k = 0; k < cnt; ++k) {
+      const double* e = chain_lds + (size_t)k * E;
+      double tlo = -INFINITY, thi = INFINITY;
+      if (box_lo) {
+#pragma unroll
+        for (int j = 0; j < DM; ++j)
+          if (EXACT || j < d) {
+            const double dj = e[j], r = e[DM + j];
+            const double rl = (box_lo[j] - x[j]) * r;
+            const double rh = (box_hi[j] - x[j]) * r;
+            const bool order = rl < rh;
+            const double lo_c = order ? rl : rh;
+            const double hi_c = order ? rh : rl;
+            const bool nz = dj != 0.0;
+            tlo = (nz && lo_c > tlo) ? lo_c : tlo;
+            thi = (nz && hi_c < thi) ? hi_c : thi;
+          }
+      }
+      const double* ea = e + 2 * DM + 1;
+#pragma unroll 2
+      for (int i = 0; i < m; ++i) {
+        double ax = 0.0;
+#pragma unroll
+        for (int j = 0; j < DM; ++j)
+          if (EXACT || j < d) ax = fma(A[i * d + j], x[j], ax);
+        const double ad = ea[i];
+        const double ratio = (b[i] - ax) * ea[m + i];
+        tlo = (ad < 0.0 && ratio > tlo) ? ratio : tlo;
+        thi = (ad > 0.0 && ratio < thi) ? ratio : thi;
+      }
+      const double t = (thi - tlo) * e[2 * DM] + tlo;
+#pragma unroll
+      for (int j = 0; j < DM; ++j) x[j] += e[j] * t;
+      const int done = s0 + k + 1;
+      if (lane == 0 && done >= a.first && (done - a.first) % a.stride == 0) {
+        double* o = out + (size_t)((done - a.first) / a.stride) * d;
+#pragma unroll
+        for (int j = 0; j < DM; ++j)
+          if (EXACT || j < d) o[j] = x[j];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// LDS doubles per move of har_chain_kernel for dimension-bucket DM and m rows.
+inline int chain_entry(int DM, int m) { return 2 * DM + 1 + 2 * m; }
+constexpr int kChainLdsBytes = 64 * 1024;
+
+template <int DM, bool EXACT>
+void launch_chain(const art_har_args& a, int max_rows, hipStream_t stream) {
+  const int E = chain_entry(DM, max_rows);
+  const size_t bytes = sizeof(double) * (size_t)kChainBlock * E;
+  hipFuncSetAttribute((const void*)har_chain_kernel<DM, EXACT>,
+                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  har_chain_kernel<DM, EXACT><<<a.P, 64, bytes, stream>>>(a, E);
+}
+
 template <int DM, bool EXACT>
 void launch_har(const art_har_args& a, hipStream_t stream) {
   const int blocks = a.P * ((a.chains + 63) / 64);
@@ -230,6 +366,14 @@ using HarLaunch = void (*)(const art_har_args&, hipStream_t);
 template <int... Ds>
 constexpr HarLaunch exact_launch(int d, std::integer_sequence<int, Ds...>) {
   HarLaunch table[] = {launch_har<Ds + 1, true>...};
+  return table[d - 1];
+}
+
+using ChainLaunch = void (*)(const art_har_args&, int, hipStream_t);
+
+template <int... Ds>
+constexpr ChainLaunch exact_chain(int d, std::integer_sequence<int, Ds...>) {
+  ChainLaunch table[] = {launch_chain<Ds + 1, true>...};
   return table[d - 1];
 }
 
@@ -273,9 +417,20 @@ extern "C" int art_har_sample(const art_har_args* a, hipStream_t stream) {
   if ((int64_t)a->P * ((a->chains + 63) / 64) > 0x7fffffff)
     return fail(ART_ERR_ARG, "art_har_sample: grid too large");
   const int d = a->d;
-  if (d <= 16) exact_launch(d, std::make_integer_sequence<int, 16>{})(*a, stream);
-  else if (d <= 24) launch_har<24, false>(*a, stream);
-  else launch_har<32, false>(*a, stream);
+  const int bucket = d <= 16 ? d : (d <= 24 ? 24 : 32);
+  const int rows = a->row_bound - 1;
+  if (a->chains == 1 && rows >= 0 &&
+      sizeof(double) * kChainBlock * chain_entry(bucket, rows) <= (size_t)kChainLdsBytes) {
+    if (d <= 16) exact_chain(d, std::make_integer_sequence<int, 16>{})(*a, rows, stream);
+    else if (d <= 24) launch_chain<24, false>(*a, rows, stream);
+    else launch_chain<32, false>(*a, rows, stream);
+  } else if (d <= 16) {
+    exact_launch(d, std::make_integer_sequence<int, 16>{})(*a, stream);
+  } else if (d <= 24) {
+    launch_har<24, false>(*a, stream);
+  } else {
+    launch_har<32, false>(*a, stream);
+  }
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? ART_OK : hip_fail(e, "har_kernel launch");
 }

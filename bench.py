@@ -270,6 +270,80 @@ def bench_split(args):
     return result
 
 
+def cpu_plain_fit_builder(max_depth):
+    """The plain-fit builder on the host: the oracle's FP64 normal-equation
+    ridge for node fits, its soft oblique optimizer (one Gaussian-direction,
+    median-offset start per node) for splits and its scalar hit-and-run chain
+    with NumPy draws for top-ups -- the reference's CPU path restated."""
+    from dataclasses import dataclass, field
+
+    from art.sampling import HitAndRunSampler, as_rng
+    from art.splitters import (SoftObliqueSplitter, SplitNotFoundError, SplitResult,
+                               project_unit_w)
+    from art.synthetic import plain_fit_builder
+    from oracle import ridge as R
+    from oracle import sampling as S
+    from oracle import split as O
+
+    @dataclass
+    class HostQuadRidge:
+        ridge: float = 1e-6
+        coef_: np.ndarray = field(default=None, repr=False)
+
+        def clone(self):
+            return HostQuadRidge(self.ridge)
+
+        def effective_dimension(self, d):
+            return 1 + d + d * (d + 1) // 2
+
+        def fit(self, X, y):
+            return self.fit_weighted(X, y, np.ones(len(y)))
+
+        def fit_weighted(self, X, y, weights, weight_floor=1e-12):
+            phi = R.features(np.asarray(X, dtype=float), 2, True, False)
+            self.coef_, _ = R.solve_weighted_ridge(phi, y, weights, self.ridge, weight_floor)
+            return self
+
+        def predict(self, X):
+            return R.features(np.atleast_2d(X), 2, True, False) @ self.coef_
+
+    class HostSplitter(SoftObliqueSplitter):
+        def split(self, X, y, parent_model=None, parent_loss=None, prepared_design=None,
+                  hard_refit=True, **kw):
+            rng = self._rng()
+            w = rng.normal(size=X.shape[1])
+            w /= np.linalg.norm(w)
+            theta0 = np.append(w, np.median(X @ w))
+            so = O.Settings(temperature=self.temperature, ridge=self.model_template.ridge,
+                            min_side_points=self.min_side_points, degree=2, bias_last=False)
+            res = O.optimize(X, y, theta0, so)
+            theta = project_unit_w(res["theta"])
+            right = X @ theta[:-1] - theta[-1] >= 0.0
+            n_right = int(right.sum())
+            if min(n_right, len(y) - n_right) < so.required_side_points(len(y)):
+                raise SplitNotFoundError("min_side_points", "A side has too few points.")
+            return SplitResult(theta[:-1], float(theta[-1]), None, None, None,
+                               float(parent_loss), None, None, len(y) - n_right, n_right,
+                               res["converged"], res["n_iters"], res["stop_reason"], {})
+
+    class HostHitAndRun(HitAndRunSampler):
+        def sample(self, region, n, random_state=None, x0=None):
+            rng = as_rng(random_state)
+            steps = self.burn_in + n * self.thinning
+            dirs = rng.normal(size=(steps, region.dimension))
+            shape = self._shaping(region.dimension)
+            if shape is not None:
+                dirs = dirs @ shape.T
+            return S.scalar_sample(region.A, region.b, x0, n, self.burn_in, self.thinning, dirs,
+                                   rng.random(steps))
+
+    model = HostQuadRidge(1e-6)
+    return plain_fit_builder(
+        seed=0, max_depth=max_depth, model_template=model,
+        splitter=HostSplitter(model, 0.05, min_side_points=64),
+        sampler=HostHitAndRun(burn_in=0, thinning=20), profile_build_timing=True)
+
+
 def bench_tree(args):
     """Plain-fit leg: whole tree builds of the 2-D plain ART fit (4000 samples
     per node, soft oblique T=0.05 on the batched device optimizer, quadratic
@@ -339,6 +413,24 @@ def bench_tree(args):
         "gpu_vs_host_prediction_max_abs": float(np.max(np.abs(pred - host))),
         "leaf_samples_inside_regions": bool(inside),
     }
+    if not args.no_cpu:
+        # A bounded host sample: the same build cut at depth 2 (7 nodes),
+        # single-threaded, scaled per node to the GPU tree's node count.
+        from threadpoolctl import threadpool_limits
+
+        with threadpool_limits(1):
+            t0 = time.perf_counter()
+            cpu = cpu_plain_fit_builder(max_depth=2).build()
+            dt = time.perf_counter() - t0
+        nodes = cpu.tree.num_nodes()
+        per_node = dt / nodes
+        result["cpu_baseline"] = {
+            "value": 1.0 / (per_node * tree.num_nodes()), "unit": "tree builds/s", "cores": 1,
+            "kind": "port",
+            "sample": (f"the same build cut at depth 2 ({nodes} nodes) on the host: FP64 normal-"
+                       f"equation fits, the oracle's soft oblique optimizer, a scalar NumPy "
+                       f"hit-and-run chain; {dt:.1f} s, {per_node * 1e3:.0f} ms per node, scaled "
+                       f"to the GPU tree's {tree.num_nodes()} nodes")}
     return result
 
 
@@ -465,6 +557,7 @@ def main():
     torch.cuda.empty_cache()
     if args.only == "all":
         result["hit_and_run"] = bench_sampler(args)
+        result["tree_builds"] = bench_tree(args)
     print(json.dumps(result))
 
 

@@ -261,6 +261,10 @@ typedef struct {
    * reads polytope q's shaping matrix at shape + q * shape_stride. */
   const uint64_t* seeds;
   int shape_stride;
+  /* 1 + the largest row count of any polytope in A, or 0 when not known.
+   * With one chain per polytope and a known bound, a wave splits each block
+   * of 64 moves of the chain (same samples, shorter critical path). */
+  int row_bound;
 } art_har_args;
 
 int art_har_sample(const art_har_args* args, hipStream_t stream);

@@ -229,3 +229,45 @@ def test_drop_in_samplers(torch_gpu):
     timed.sample(box, 6, random_state=43, x0=np.array([[0., 0.], [.25, -.25]]))
     assert {"slack_matmul_seconds", "constraint_product_matmul_seconds",
             "random_generation_seconds", "transitions_seconds"} <= set(timed.last_phase_timings)
+
+
+@pytest.mark.parametrize("d,k,shaped", [(2, 6, False), (3, 0, False), (5, 9, True),
+                                        (12, 30, False), (20, 4, True)])
+def test_single_chains_equal_chain_zero_of_a_two_chain_run(torch_gpu, d, k, shaped):
+    # One chain per polytope runs on the wave-cooperative kernel, two chains on
+    # the lane-per-chain kernel; chain 0 draws the same Philox stream in both,
+    # so its records must agree bit for bit.
+    from art.sampling import PolytopeBatch
+
+    rng = np.random.default_rng(100 + d)
+    regs = [_cut_box(rng, d, k) for _ in range(3)]
+    batch = PolytopeBatch(([r[0] for r in regs], [r[1] for r in regs]))
+    shape = None
+    if shaped:
+        V = np.linalg.qr(rng.normal(size=(d, d)))[0]
+        shape = S.shaping_matrix(V, rng.uniform(0.1, 2.0, d), 1e-4)
+    n, first, stride, seed = 37, 3, 5, 77
+    one = batch.sample(np.stack([r[2] for r in regs])[:, None], n, first=first, stride=stride,
+                       seed=seed, shape=shape).cpu().numpy()
+    starts = np.stack([np.stack([r[2], r[2]]) for r in regs])
+    two = batch.sample(starts, 2 * n, first=first, stride=stride, seed=seed,
+                       shape=shape).cpu().numpy()
+    np.testing.assert_array_equal(one, two[:, 0::2])
+    for (A, b, _), pts in zip(regs, one):
+        assert np.all(pts @ A.T <= b + 1e-9)
+
+
+def test_single_chain_with_cuts_matches_the_scalar_oracle(torch_gpu):
+    from art.sampling import PolytopeBatch
+
+    rng = np.random.default_rng(61)
+    regs = [_cut_box(rng, 3, 5) for _ in range(2)]
+    n, burn, thin = 25, 4, 3
+    steps = burn + n * thin
+    dirs, us = rng.normal(size=(2, steps, 1, 3)), rng.random((2, steps, 1))
+    got = PolytopeBatch(([r[0] for r in regs], [r[1] for r in regs])).sample(
+        np.stack([r[2] for r in regs])[:, None], n, first=burn + 1, stride=thin,
+        directions=dirs, uniforms=us).cpu().numpy()
+    for p, (A, b, star) in enumerate(regs):
+        exp = S.scalar_sample(A, b, star, n, burn, thin, dirs[p, :, 0], us[p, :, 0])
+        np.testing.assert_allclose(got[p], exp, rtol=1e-12, atol=1e-12)
