@@ -228,6 +228,7 @@ size_t leaf_stage_smem_bytes(int d) {
 // weights; it has no FP64 LDS buffers.
 constexpr int kMixedRows = 256;
 constexpr int kMixedMaxD = 16;
+constexpr int kMixedFlush = 1;  // 32-row steps per FP32 accumulation
 __host__ __device__ inline int rows32_stride(int d) { return d + 3; }
 size_t mixed_stage_smem_bytes(int d) {
   const size_t f32 = (size_t)kMixedRows * (rows32_stride(d) + 1) * sizeof(float);
@@ -604,8 +605,8 @@ __global__ __launch_bounds__(128) void solve_rows_kernel(
 // Mixed precision leaf fits.
 //
 // 1. gram_leaf_mixed_kernel: the augmented Gram [Phi | y]^T W [Phi | y] by
-//    split-bf16 MFMA (v = hi + lo, hi*hi + hi*lo + lo*hi, 32 rows per
-//    v_mfma_f32_16x16x32_bf16), accumulated in FP32 over one 32-row step and
+//    split-bf16 MFMA (v = sqrt(w) phi = hi + mid + lo, six products, 32 rows
+//    per v_mfma_f32_16x16x32_bf16), accumulated in FP32 per 32-row step and
 //    then in FP64, plus the exact FP64 trace per chunk.  Partials have the
 //    FP64 kernel's layout, so the same solve kernel factors them.
 // 2. solve_rows_kernel (with factor output): beta0 = (G~ + lambda I)^-1 h~,
@@ -629,11 +630,14 @@ __device__ inline unsigned pack_bf16(float a, float b) {
   return __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2));
 }
 
-// Split a pair of floats into packed bf16 hi and lo words: v = hi + lo + O(2^-18 v).
-__device__ inline void split_pair(float a, float b, unsigned& hi, unsigned& lo) {
+// Split a pair of floats into packed bf16 hi, mid and lo words:
+// v = hi + mid + lo + O(2^-27 v).  Each remainder v - hi (then r - mid) is
+// exact in FP32: it has at most 16 significant bits.
+__device__ inline void split3_pair(float a, float b, unsigned& hi, unsigned& mid, unsigned& lo) {
   hi = pack_bf16(a, b);
-  const float ha = __uint_as_float(hi << 16), hb = __uint_as_float(hi & 0xffff0000u);
-  lo = pack_bf16(a - ha, b - hb);
+  const float ra = a - __uint_as_float(hi << 16), rb = b - __uint_as_float(hi & 0xffff0000u);
+  mid = pack_bf16(ra, rb);
+  lo = pack_bf16(ra - __uint_as_float(mid << 16), rb - __uint_as_float(mid & 0xffff0000u));
 }
 
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
@@ -644,14 +648,18 @@ typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 // the exact FP64 trace of the Gram (which fixes lambda) from
 // |phi|^2 = bias + s1 [+ (s1^2 + s2)/2 for the quadratic basis],
 // s1 = sum x_i^2, s2 = sum x_i^4, checks the weight, and writes the row to
-// an FP32 table [x, 1, y, 0] with its floored FP32 weight.  Wave w then takes
-// the 32-row steps s = w, w+4, ...: lane (col, kq) holds rows 32 s + 4 e + kq
-// (e < 8; the four row groups of one LDS read are adjacent rows) of features
-// 16 t + col as a = w phi and b = phi, each split v = hi + lo in bf16, in the
-// operand layout of v_mfma_f32_16x16x32_bf16.  A tile pair accumulates
-// hi*lo + lo*hi + hi*hi in FP32, added into FP64 every second step.
+// an FP32 table [x, 1, y, 0] with the square root of its floored weight.
+// Wave w then takes the 32-row steps s = w, w+4, ...: lane (col, kq) holds
+// rows 32 s + 4 e + kq (e < 8; the four row groups of one LDS read are
+// adjacent rows) of features 16 t + col as v = sqrt(w) phi, split
+// v = hi + mid + lo in bf16, in the operand layout of
+// v_mfma_f32_16x16x32_bf16; both operands of every tile pair are these v.
+// A tile pair accumulates the six products down to 2^-16 (hi*lo, lo*hi,
+// mid*mid, hi*mid, mid*hi, hi*hi: Gram entries to ~1e-8 relative) in FP32
+// and is added into FP64 after every kMixedFlush steps.  The two-term split
+// (~1e-5) sent five times as many leaves to the FP64 path.
 template <int T, int DC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void gram_leaf_mixed_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DC > 0 ? 3 : 2))) void gram_leaf_mixed_kernel(
     const double* __restrict__ X, const double* __restrict__ y, const double* __restrict__ w,
     Basis B, double floor, const LeafChunk* __restrict__ chunks, double* __restrict__ partials,
     double* __restrict__ tpart, int* __restrict__ flags) {
@@ -731,7 +739,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void g
         row[d] = 1.0f;
         row[d + 1] = (float)yr;
         row[d + 2] = 0.0f;
-        w32[tid] = (float)wt;
+        w32[tid] = (float)sqrt(wt);
       } else {
         for (int i = 0; i < S32; ++i) row[i] = 0.0f;
         w32[tid] = 0.0f;
@@ -743,8 +751,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void g
     for (int s = wave; s < steps; s += 4) {
       const float* base = rows32 + 32 * s * S32;
       const float* wb = w32 + 32 * s + kq;
-      // Rows 8k + kq and 8k + 4 + kq share one packed word of each operand.
-      u32x4 ahw[T], alw[T], bhw[T], blw[T];
+      // Both operands are sqrt(w) phi (G = sum w phi phi^T is symmetric), so
+      // each value is split once.  Rows 8k + kq and 8k + 4 + kq share one
+      // packed word.
+      u32x4 ahw[T], amw[T], alw[T];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const float w0 = wb[8 * k], w1 = wb[8 * k + 4];
@@ -752,46 +762,38 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void g
         for (int t = 0; t < T; ++t) {
           const float p0 = base[fa[t] + 8 * k * S32] * base[fb[t] + 8 * k * S32];
           const float p1 = base[fa[t] + (8 * k + 4) * S32] * base[fb[t] + (8 * k + 4) * S32];
-          unsigned hi, lo;
-          split_pair(w0 * p0, w1 * p1, hi, lo);
+          unsigned hi, mid, lo;
+          split3_pair(w0 * p0, w1 * p1, hi, mid, lo);
           ahw[t][k] = hi;
+          amw[t][k] = mid;
           alw[t][k] = lo;
-          split_pair(p0, p1, hi, lo);
-          bhw[t][k] = hi;
-          blw[t][k] = lo;
         }
       }
-      bf16x8 ah[T], al[T], bh[T], bl[T];
+      bf16x8 ah[T], am[T], al[T];
 #pragma unroll
       for (int t = 0; t < T; ++t) {
         ah[t] = __builtin_bit_cast(bf16x8, ahw[t]);
+        am[t] = __builtin_bit_cast(bf16x8, amw[t]);
         al[t] = __builtin_bit_cast(bf16x8, alw[t]);
-        bh[t] = __builtin_bit_cast(bf16x8, bhw[t]);
-        bl[t] = __builtin_bit_cast(bf16x8, blw[t]);
       }
-      // Three passes over the tile pairs keep NP independent accumulator
-      // chains in flight; small terms first.
+      // Terms down to 2^-16: hi*lo + lo*hi + mid*mid + hi*mid + mid*hi +
+      // hi*hi, small first; each pass over the tile pairs keeps NP
+      // independent accumulator chains in flight.
       {
-        int q = 0;
+        const bf16x8* A1[6] = {ah, al, am, ah, am, ah};
+        const bf16x8* B1[6] = {al, ah, am, am, ah, ah};
 #pragma unroll
-        for (int t1 = 0; t1 < T; ++t1)
+        for (int term = 0; term < 6; ++term) {
+          int q = 0;
 #pragma unroll
-          for (int t2 = t1; t2 < T; ++t2, ++q)
-            z[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[t1], bl[t2], z[q], 0, 0, 0);
-        q = 0;
+          for (int t1 = 0; t1 < T; ++t1)
 #pragma unroll
-        for (int t1 = 0; t1 < T; ++t1)
-#pragma unroll
-          for (int t2 = t1; t2 < T; ++t2, ++q)
-            z[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[t1], bh[t2], z[q], 0, 0, 0);
-        q = 0;
-#pragma unroll
-        for (int t1 = 0; t1 < T; ++t1)
-#pragma unroll
-          for (int t2 = t1; t2 < T; ++t2, ++q)
-            z[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[t1], bh[t2], z[q], 0, 0, 0);
+            for (int t2 = t1; t2 < T; ++t2, ++q)
+              z[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1[term][t1], B1[term][t2], z[q],
+                                                             0, 0, 0);
+        }
       }
-      if (++pending == 2) {
+      if (++pending == kMixedFlush) {
 #pragma unroll
         for (int q = 0; q < NP; ++q)
 #pragma unroll

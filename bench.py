@@ -519,7 +519,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "bf16x3-mfma/f32-acc/f64-refine",
+        "dtype": "f64" if args.precision == "fp64" else "bf16 3-way split (6 MFMA products)/f32-acc/f64-refine",
         "data": "synthetic (seeded, generated on host, resident in HBM before timing)",
         "config": {
             "workload": ("batched weighted ridge: 4096 leaves, d=8, quadratic p=45, "
