@@ -55,6 +55,8 @@ class SplitArgs(ctypes.Structure):
         ("t0", ctypes.c_double), ("t1", ctypes.c_double), ("step", ctypes.c_double),
         ("theta_out", ctypes.c_void_p), ("beta_out", ctypes.c_void_p), ("hard", ctypes.c_void_p),
         ("info", ctypes.c_void_p), ("history", ctypes.c_void_p), ("scalars", ctypes.c_void_p),
+        ("row_offsets", ctypes.c_void_p), ("min_side_points", ctypes.c_int),
+        ("min_side_fraction", ctypes.c_double),
     ]
 
 
@@ -94,6 +96,8 @@ _PROTOTYPES = {
     "art_split_optimize": (_i, [ctypes.POINTER(SplitArgs), _i, _p]),
     "art_split_anneal_batched": (_i, [ctypes.POINTER(SplitArgs), _i, _p]),
     "art_split_init": (_i, [_p, _i, _i, _p, _i, ctypes.c_uint64, ctypes.c_uint64, _p, _p]),
+    "art_split_init_ranges": (_i, [_p, _p, _i, _p, _i, ctypes.c_uint64, ctypes.c_uint64, _p,
+                                   _p]),
     "art_har_sample": (_i, [ctypes.POINTER(HarArgs), _p]),
     "art_philox4x32_10": (_i, [_p, _p, _i64, _p, _p]),
     "art_tree_predict": (_i, [_p, _i64, _p, _p, _p, _i, _p, Basis, _p, _p]),

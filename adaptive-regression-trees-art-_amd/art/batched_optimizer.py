@@ -106,6 +106,16 @@ class BatchedOptimizerWorkspace:
         return ws
 
     def _allocate(self, splitter, capacity, n, d, *, reorder, stream=None):
+        self._setup(splitter, capacity, n, d, reorder=reorder, stream=stream)
+        torch = self.torch
+        f64 = dict(dtype=torch.float64, device=self.device)
+        self.x = torch.empty((self.K, self.n, self.d), **f64)
+        self.yd = torch.empty((self.K, self.n), **f64)
+        self.host_X = np.empty((self.K, self.n, self.d))
+        self.host_y = np.empty((self.K, self.n))
+        self.X, self.y = self.host_X[:0], self.host_y[:0]
+
+    def _setup(self, splitter, capacity, n, d, *, reorder, stream=None):
         self.lib = None
         self.K, self.n, self.d = _dimensions(capacity, n, d)
         _validate_settings(splitter, self.d, reorder)
@@ -126,13 +136,8 @@ class BatchedOptimizerWorkspace:
         self.device = torch.device("cuda", torch.cuda.current_device())
         self.stream = torch.cuda.current_stream() if stream is None else stream
         f64 = dict(dtype=torch.float64, device=self.device)
-        self.x = torch.empty((self.K, self.n, self.d), **f64)
-        self.yd = torch.empty((self.K, self.n), **f64)
         self.temp = torch.ones(self.K, **f64)
-        self.host_X = np.empty((self.K, self.n, self.d))
-        self.host_y = np.empty((self.K, self.n))
         self.temperature = np.ones(self.K)
-        self.X, self.y = self.host_X[:0], self.host_y[:0]
         self.all_ids = np.arange(self.K, dtype=np.int32)
         self.fit_sizes = []
         self.last_launch_problems = 0
@@ -232,6 +237,7 @@ class BatchedOptimizerWorkspace:
             a.anneal_iters, a.t0, a.t1, a.step = int(anneal[0]), *map(float, anneal[1:])
         a.theta_out, a.beta_out, a.hard = _hip.ptr(out_theta), _hip.ptr(out_beta), _hip.ptr(hard)
         a.info, a.history, a.scalars = _hip.ptr(info), _hip.ptr(hist), _hip.ptr(scal)
+        self._ragged_args(a)
         with torch.cuda.stream(self.stream):
             if timing is not None:
                 ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
@@ -321,8 +327,8 @@ class BatchedOptimizerWorkspace:
                                                 diagnostics=meta)
                 continue
             if not hard_refit:
-                results[i] = _partition_result(theta[q], self.n, hard[q, 3], parent_loss[i],
-                                               meta["converged"], meta)
+                results[i] = _partition_result(theta[q], self._rows(j), hard[q, 3],
+                                               parent_loss[i], meta["converged"], meta)
                 continue
             fb = bool(hard[q, 7])
             b = best[i]
@@ -332,7 +338,8 @@ class BatchedOptimizerWorkspace:
             for i in range(count):
                 if best[i] is not None:
                     q, _, _, meta = best[i]
-                    results[i] = self._export(q, theta[q], beta[q], hard[q], meta, parent_loss[i])
+                    results[i] = self._export(q, theta[q], beta[q], hard[q], meta, parent_loss[i],
+                                              int(ids[i]))
         return results
 
     def anneal(self, theta0=None, *, nodes=None, t0=0.2, t1=0.01, iters=200, step=0.05,
@@ -450,10 +457,10 @@ class BatchedOptimizerWorkspace:
             logical_line_search_candidates=int(info[q, 5]),
             speculative_line_search_candidates=int(info[q, 4] - info[q, 5]),
             final_alpha=float(scal[q, 2]), hard_refit=bool(hard_refit),
-            n_left=self.n - int(hard[q, 3]), n_right=int(hard[q, 3]),
+            n_left=self._rows(j) - int(hard[q, 3]), n_right=int(hard[q, 3]),
             left_loss=float(hard[q, 1]) if hard_refit else None,
             right_loss=float(hard[q, 2]) if hard_refit else None,
-            required_side_points=self.required,
+            required_side_points=self._required(j),
         )
         meta["final_soft_metadata"] = {
             "temperature": float(self.temperature[j]),
@@ -468,7 +475,7 @@ class BatchedOptimizerWorkspace:
             meta.update(min_side_fallback=True, fallback_iteration=int(info[q, 3]))
         return meta
 
-    def _export(self, q, theta, beta, hard, meta, parent_loss):
+    def _export(self, q, theta, beta, hard, meta, parent_loss, j):
         models = []
         for side in range(2):
             model = self.s.model_template.clone()
@@ -484,8 +491,18 @@ class BatchedOptimizerWorkspace:
                            right_model=models[1], loss=float(hard[0]),
                            parent_loss=float(parent_loss), split_gain=gain,
                            relative_split_gain=gain / max(float(parent_loss), 1e-12),
-                           n_left=self.n - right, n_right=right, converged=meta["converged"],
+                           n_left=self._rows(j) - right, n_right=right, converged=meta["converged"],
                            n_iters=meta["n_iters"], stop_reason=meta["stop_reason"], metadata=meta)
+
+    # Node sizes: every slot holds n rows here; the ragged workspace overrides.
+    def _rows(self, j):
+        return self.n
+
+    def _required(self, j):
+        return self.required
+
+    def _ragged_args(self, a):
+        pass
 
     def close(self):
         if getattr(self, "lib", None) is not None:
@@ -505,6 +522,88 @@ class BatchedOptimizerWorkspace:
             self.close()
         except Exception:
             pass
+
+
+class RaggedOptimizerWorkspace(BatchedOptimizerWorkspace):
+    """Nodes of uneven size in one device launch.
+
+    Node k owns rows [offsets[k], offsets[k+1]) of X (N, d) and y (N,), kept
+    concatenated on the device; each problem reads its own range and needs
+    max(min_side_points, ceil(min_side_fraction * n_k)) points per side.
+    ``optimize`` keeps the signature, results and metadata of
+    ``BatchedOptimizerWorkspace.optimize``.  This is the uneven-cohort form
+    the reference's workspace lacks (it requires (K, n, d) blocks,
+    art/_cuda_batched_optimizer.py:88-103); the batched annealer needs equal
+    sizes, so ``anneal`` runs one workgroup per start here.
+    """
+
+    def __init__(self, splitter, X, y, offsets, *, temperature=None):
+        X = np.ascontiguousarray(X, dtype=float)
+        y = np.ascontiguousarray(y, dtype=float).reshape(-1)
+        offsets = np.ascontiguousarray(offsets, dtype=np.int64).reshape(-1)
+        if X.ndim != 2 or X.shape[0] != y.shape[0] or offsets.size < 2:
+            raise ValueError("X/y must have shapes (N,d)/(N,) and offsets (K+1,).")
+        sizes = np.diff(offsets)
+        if offsets[0] != 0 or offsets[-1] != X.shape[0] or np.any(sizes < 2):
+            raise ValueError("offsets must run from 0 to N with at least two rows per node.")
+        if not np.isfinite(X).all() or not np.isfinite(y).all():
+            raise ValueError("X and y must be finite.")
+        K = sizes.size
+        temp = np.broadcast_to(np.asarray(splitter.temperature if temperature is None
+                                          else temperature, dtype=float), (K,))
+        if not np.isfinite(temp).all() or np.any(temp <= 0):
+            raise ValueError("Temperatures must be finite and positive.")
+        self._setup(splitter, K, int(sizes.max()), X.shape[1], reorder=True)
+        self.sizes = sizes
+        self.required_k = [splitter._required_side_points(int(n)) for n in sizes]
+        torch = self.torch
+        with torch.cuda.stream(self.stream):
+            self.x = torch.from_numpy(X).to(self.device)
+            self.yd = torch.from_numpy(y).to(self.device)
+            self.offsets_dev = torch.from_numpy(offsets).to(self.device)
+            self.temperature[:] = temp
+            self.temp.copy_(torch.from_numpy(self.temperature.copy()))
+        self.X, self.y = X, y
+        self.offsets = offsets
+        self.k = K
+
+    def load(self, *args, **kwargs):
+        raise RuntimeError("A ragged workspace takes its data at construction.")
+
+    def _rows(self, j):
+        return int(self.sizes[j])
+
+    def _required(self, j):
+        return self.required_k[j]
+
+    def _ragged_args(self, a):
+        a.n = 0
+        a.row_offsets = _hip.ptr(self.offsets_dev)
+        a.min_side_points = int(self.s.min_side_points)
+        a.min_side_fraction = float(self.s.min_side_fraction)
+
+    def _device_starts(self, nodes, timing=None):
+        torch, P = self.torch, len(nodes)
+        if P == 0:
+            return np.empty((0, self.d + 1))
+        node_t = torch.as_tensor(np.ascontiguousarray(nodes, dtype=np.int32), device=self.device)
+        out = torch.empty((P, self.d + 1), dtype=torch.float64, device=self.device)
+        with torch.cuda.stream(self.stream):
+            _hip.check(self.lib.art_split_init_ranges(
+                _hip.ptr(self.x), _hip.ptr(self.offsets_dev), self.d, _hip.ptr(node_t), P,
+                self.seed & 0xFFFFFFFFFFFFFFFF, self._init_calls, _hip.ptr(out),
+                _hip.stream_handle(self.stream)))
+        self._init_calls += 1
+        self.stream.synchronize()
+        return out.cpu().numpy()
+
+    def anneal(self, theta0=None, **kwargs):
+        kwargs["batched"] = False
+        return super().anneal(theta0, **kwargs)
+
+    def close(self):
+        super().close()
+        self.offsets_dev = None
 
 
 def optimize_batch(splitter, X, y, *, theta0=None, parent_loss=None, temperature=None,

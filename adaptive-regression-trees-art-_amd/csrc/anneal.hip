@@ -737,6 +737,8 @@ extern "C" int art_split_anneal_batched(const art_split_args* args, int problems
                              "basis features");
   if (args->anneal_iters < 1 || !(args->t0 > 0.0) || !(args->t1 > 0.0))
     return fail(ART_ERR_ARG, "art_split_anneal_batched: needs iterations and positive temperatures");
+  if (args->row_offsets)
+    return fail(ART_ERR_ARG, "art_split_anneal_batched: nodes must have n rows each");
   if (!args->X || !args->y || !args->theta0 || !args->node || !args->theta_out ||
       !args->beta_out || !args->hard || !args->info || !args->history || !args->scalars)
     return fail(ART_ERR_ARG, "art_split_anneal_batched: null device pointer");

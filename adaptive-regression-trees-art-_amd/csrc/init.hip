@@ -145,12 +145,15 @@ __device__ unsigned long long key_above(const RowKeys& keys, int n, InitShared& 
 }
 
 __global__ __launch_bounds__(kInitThreads) void split_init_kernel(
-    const double* __restrict__ X, int n, int d, const int* __restrict__ node, uint64_t seed,
-    uint64_t counter, double* __restrict__ theta) {
+    const double* __restrict__ X, int n_all, const int64_t* __restrict__ row_offsets, int d,
+    const int* __restrict__ node, uint64_t seed, uint64_t counter, double* __restrict__ theta) {
   __shared__ InitShared S;
   extern __shared__ unsigned long long init_keys[];
   const int q = blockIdx.x, tid = threadIdx.x;
-  const double* Xn = X + (size_t)node[q] * n * d;
+  const int k = node[q];
+  const int64_t r0 = row_offsets ? row_offsets[k] : (int64_t)k * n_all;
+  const int n = row_offsets ? (int)(row_offsets[k + 1] - r0) : n_all;
+  const double* Xn = X + (size_t)r0 * d;
   const int pairs = (d + 1) / 2;
   if (tid < pairs) {
     const uint4 r = init_philox(make_uint4((uint32_t)tid, (uint32_t)q, (uint32_t)counter,
@@ -199,18 +202,42 @@ __global__ __launch_bounds__(kInitThreads) void split_init_kernel(
 
 using namespace art;
 
+namespace {
+
+// Nodes of at most kInitLdsRows rows stage their keys in dynamic LDS; the
+// allocation covers the largest possible node (n unknown to the host for
+// row ranges).
+int launch_init(const double* X, int n, const int64_t* row_offsets, int d, const int* node,
+                int problems, uint64_t seed, uint64_t counter, double* theta,
+                hipStream_t stream) {
+  const int smem = (int)(sizeof(unsigned long long) *
+                         (row_offsets ? kInitLdsRows : (n <= kInitLdsRows ? n : 0)));
+  hipError_t e = hipFuncSetAttribute((const void*)split_init_kernel,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)(sizeof(unsigned long long) * kInitLdsRows));
+  if (e != hipSuccess) return hip_fail(e, "split_init_kernel shared memory attribute");
+  split_init_kernel<<<problems, kInitThreads, smem, stream>>>(X, n, row_offsets, d, node, seed,
+                                                              counter, theta);
+  e = hipGetLastError();
+  return e == hipSuccess ? ART_OK : hip_fail(e, "split_init_kernel launch");
+}
+
+}  // namespace
+
 extern "C" int art_split_init(const double* X, int n, int d, const int* node, int problems,
                               uint64_t seed, uint64_t counter, double* theta,
                               hipStream_t stream) {
   if (!X || !node || !theta || n < 1 || d < 1 || d > kSplitInitMaxD || problems < 0)
     return fail(ART_ERR_ARG, "art_split_init: bad arguments (1 <= d <= 32, n >= 1)");
   if (problems == 0) return ART_OK;
-  const int smem = (int)(sizeof(unsigned long long) * (n <= kInitLdsRows ? n : 0));
-  hipError_t e = hipFuncSetAttribute((const void*)split_init_kernel,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)(sizeof(unsigned long long) * kInitLdsRows));
-  if (e != hipSuccess) return hip_fail(e, "split_init_kernel shared memory attribute");
-  split_init_kernel<<<problems, kInitThreads, smem, stream>>>(X, n, d, node, seed, counter, theta);
-  e = hipGetLastError();
-  return e == hipSuccess ? ART_OK : hip_fail(e, "split_init_kernel launch");
+  return launch_init(X, n, nullptr, d, node, problems, seed, counter, theta, stream);
+}
+
+extern "C" int art_split_init_ranges(const double* X, const int64_t* row_offsets, int d,
+                                     const int* node, int problems, uint64_t seed,
+                                     uint64_t counter, double* theta, hipStream_t stream) {
+  if (!X || !row_offsets || !node || !theta || d < 1 || d > kSplitInitMaxD || problems < 0)
+    return fail(ART_ERR_ARG, "art_split_init_ranges: bad arguments (1 <= d <= 32)");
+  if (problems == 0) return ART_OK;
+  return launch_init(X, 0, row_offsets, d, node, problems, seed, counter, theta, stream);
 }

@@ -587,10 +587,12 @@ __global__ __launch_bounds__(256) void split_optimize_kernel(art_split_args args
   const int tid = threadIdx.x;
   const art_split_args* a = &args;
   const int node = a->node[prob];
-  const int n = a->n, d = a->d;
+  const int d = a->d;
+  const int64_t r0 = a->row_offsets ? a->row_offsets[node] : (int64_t)node * a->n;
+  const int n = a->row_offsets ? (int)(a->row_offsets[node + 1] - r0) : a->n;
   SplitCtx C;
-  C.X = a->X + (size_t)node * n * d;
-  C.y = a->y + (size_t)node * n;
+  C.X = a->X + (size_t)r0 * d;
+  C.y = a->y + r0;
   C.n = n;
   C.d = d;
   C.B = a->basis;
@@ -635,7 +637,8 @@ __global__ __launch_bounds__(256) void split_optimize_kernel(art_split_args args
   int reason = kMaxIters, converged = 0, it = 0, feasible_it = -1, error = 0;
   long physical = 0, logical = 0;
   double alpha_ctrl = clip_alpha(a, a->alpha0);
-  const int required = a->required;
+  const int required =
+      a->row_offsets ? max(a->min_side_points, (int)ceil(a->min_side_fraction * n)) : a->required;
   const int S_ = a->max_backtracks + 1;
   const int H = a->H;
 
@@ -849,7 +852,8 @@ extern "C" int art_split_optimize(const art_split_args* args, int problems, hipS
   const int d = args->d;
   if (p < 1 || p > kSplitMaxP || d < 1 || d > kSplitMaxD)
     return fail(ART_ERR_ARG, "art_split_optimize: requires 1 <= p <= 32 and 1 <= d <= 31");
-  if (args->n < 1 || args->H < 1 || args->max_iters < 0 || args->max_backtracks < 0 ||
+  if ((!args->row_offsets && args->n < 1) || args->H < 1 || args->max_iters < 0 ||
+      args->max_backtracks < 0 ||
       args->max_failures < 1 || args->mode < 0 || args->mode > 2 || args->anneal_iters < 0)
     return fail(ART_ERR_ARG, "art_split_optimize: invalid sizes or controls");
   if (!args->X || !args->y || !args->temperature || !args->theta0 || !args->node ||

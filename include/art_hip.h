@@ -175,6 +175,14 @@ typedef struct {
   int* info;
   double* history;
   double* scalars;
+  /* Uneven node sizes (optional): node slot k owns rows
+   * [row_offsets[k], row_offsets[k+1]) of X (rows, d) and y (rows,); n is
+   * then unused and each problem needs
+   * max(min_side_points, ceil(min_side_fraction * n_k)) points per side
+   * (art/splitters.py:750-752) instead of `required`.  NULL: n rows each. */
+  const int64_t* row_offsets;
+  int min_side_points;
+  double min_side_fraction;
 } art_split_args;
 
 /* Runs every problem of a cohort to completion in one launch (p <= 32,
@@ -205,6 +213,13 @@ int art_split_anneal_batched(const art_split_args* args, int problems, hipStream
  * (art/_cuda_batched_optimizer.cu:25-45, art/_cuda_batched_optimizer.py:291-305). */
 int art_split_init(const double* X, int n, int d, const int* node, int problems, uint64_t seed,
                    uint64_t counter, double* theta, hipStream_t stream);
+
+/* art_split_init for nodes of uneven size: node slot k owns rows
+ * [row_offsets[k], row_offsets[k+1]) of X (rows, d); row_offsets is a device
+ * array.  Same stream and outputs. */
+int art_split_init_ranges(const double* X, const int64_t* row_offsets, int d, const int* node,
+                          int problems, uint64_t seed, uint64_t counter, double* theta,
+                          hipStream_t stream);
 
 /* Frees a plan's device buffers (synchronizes the device).
  * Replaces BatchedCholeskyWorkspace.close (art/_cuda_batched_cholesky.py:339-352). */

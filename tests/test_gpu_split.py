@@ -459,3 +459,48 @@ def test_random_start_metadata_reports_device_initialization(torch_gpu):
         np.testing.assert_allclose(start, O.device_start(X[j], ws.seed, j, 0), rtol=1e-12,
                                    atol=1e-13)
         check_against_oracle(r, s, X[j], y[j], start, .7)
+
+
+def test_ragged_nodes_match_the_oracle_node_by_node(torch_gpu):
+    # Three nodes of 150, 301 and 77 rows in one launch; each result equals the
+    # oracle's run on that node alone (its own min-side requirement too).
+    from art import AffineRidgeModel, SoftObliqueSplitter
+    from art.batched_optimizer import RaggedOptimizerWorkspace
+
+    rng = np.random.default_rng(88)
+    sizes = [150, 301, 77]
+    Xs = [rng.uniform(-1, 1, (n, 2)) for n in sizes]
+    ys = [np.where(X[:, 0] + .3 * X[:, 1] > .05, 1. + X[:, 1], -.5 + .4 * X[:, 0]) for X in Xs]
+    offsets = np.concatenate([[0], np.cumsum(sizes)])
+    s = SoftObliqueSplitter(AffineRidgeModel(1e-6), .3, max_iters=25, min_side_points=5,
+                            min_side_fraction=.1)
+    theta0 = np.array([[.9, .3, 0.], [.2, 1., .1], [1., -.2, .05]])
+    with RaggedOptimizerWorkspace(s, np.vstack(Xs), np.concatenate(ys), offsets) as ws:
+        out = ws.optimize(theta0)
+        starts = ws._device_starts([2, 0, 1])
+    for j in range(3):
+        r = out[j]
+        check_against_oracle(r, s, Xs[j], ys[j], theta0[j], .3)
+        assert r.n_left + r.n_right == sizes[j]
+        assert r.metadata["required_side_points"] == max(5, int(np.ceil(.1 * sizes[j])))
+    for q, j in enumerate([2, 0, 1]):
+        np.testing.assert_allclose(starts[q], O.device_start(Xs[j], ws.seed, q, 0), rtol=1e-12,
+                                   atol=1e-13)
+
+
+def test_ragged_layout_of_equal_nodes_equals_the_block_layout(torch_gpu):
+    from art import AffineRidgeModel, BatchedOptimizerWorkspace, SoftObliqueSplitter, SplitResult
+    from art.batched_optimizer import RaggedOptimizerWorkspace
+
+    X, y, theta = problems()
+    s = SoftObliqueSplitter(AffineRidgeModel(1e-4), .7, max_iters=20, min_side_points=2)
+    with BatchedOptimizerWorkspace(s, X, y) as ws:
+        a = ws.optimize(theta)
+    offsets = np.arange(4) * X.shape[1]
+    with RaggedOptimizerWorkspace(s, X.reshape(-1, 2), y.reshape(-1), offsets) as ws:
+        b = ws.optimize(theta)
+    for u, v in zip(a, b):
+        assert type(u) is type(v)
+        if isinstance(u, SplitResult):
+            np.testing.assert_array_equal(np.r_[u.w, u.z], np.r_[v.w, v.z])
+            assert u.loss == v.loss and u.n_right == v.n_right
