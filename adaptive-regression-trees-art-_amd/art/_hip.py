@@ -97,11 +97,15 @@ _PROTOTYPES = {
     "art_split_anneal_batched": (_i, [ctypes.POINTER(SplitArgs), _i, _p]),
     "art_split_init": (_i, [_p, _i, _i, _p, _i, ctypes.c_uint64, ctypes.c_uint64, _p, _p]),
     "art_split_init_ranges": (_i, [_p, _p, _i, _p, _i, ctypes.c_uint64, ctypes.c_uint64, _p,
-                                   _p]),
+                                   _p, _p]),
     "art_har_sample": (_i, [ctypes.POINTER(HarArgs), _p]),
     "art_philox4x32_10": (_i, [_p, _p, _i64, _p, _p]),
     "art_tree_predict": (_i, [_p, _i64, _p, _p, _p, _i, _p, Basis, _p, _p]),
+    "art_leaf_gram": (_i, [_p, _p, _p, _p, ctypes.c_double, _p, _p, _p]),
+    "art_gram_size": (_i, [_i]),
+    "art_gram_solve": (_i, [_p, _p, _i, Basis, ctypes.c_double, _p, _p, _p, _p]),
     "art_knn_median": (_i, [_p, _p, _p, _i, _i, _i, _p, _p]),
+    "art_tree_leaf": (_i, [_p, _i64, _i, _p, _p, _p, _i, _p, _p]),
 }
 
 

@@ -582,16 +582,25 @@ class RaggedOptimizerWorkspace(BatchedOptimizerWorkspace):
         a.min_side_points = int(self.s.min_side_points)
         a.min_side_fraction = float(self.s.min_side_fraction)
 
-    def _device_starts(self, nodes, timing=None):
+    def _device_starts(self, nodes, timing=None, keys=None):
+        """Random starts of node slots; keys (one uint64 per start) make each
+        start depend on its key alone, not on its position in the call."""
         torch, P = self.torch, len(nodes)
         if P == 0:
             return np.empty((0, self.d + 1))
         node_t = torch.as_tensor(np.ascontiguousarray(nodes, dtype=np.int32), device=self.device)
         out = torch.empty((P, self.d + 1), dtype=torch.float64, device=self.device)
+        key_t = None
+        if keys is not None:
+            kv = np.ascontiguousarray(np.asarray(keys, dtype=np.uint64).reshape(-1))
+            if kv.shape[0] != P:
+                raise ValueError("keys must give one key per start.")
+            key_t = torch.from_numpy(kv.view(np.int64)).to(self.device)
         with torch.cuda.stream(self.stream):
             _hip.check(self.lib.art_split_init_ranges(
                 _hip.ptr(self.x), _hip.ptr(self.offsets_dev), self.d, _hip.ptr(node_t), P,
-                self.seed & 0xFFFFFFFFFFFFFFFF, self._init_calls, _hip.ptr(out),
+                self.seed & 0xFFFFFFFFFFFFFFFF, self._init_calls,
+                0 if key_t is None else _hip.ptr(key_t), _hip.ptr(out),
                 _hip.stream_handle(self.stream)))
         self._init_calls += 1
         self.stream.synchronize()

@@ -99,6 +99,22 @@ class LeafRidgeBatch:
             _hip.ptr(self.ridge_effective), _hip.stream_handle(self.stream)))
         return self.coefficients, self.status, self.ridge_effective
 
+    def grams(self, X, y, w, weight_floor=1e-12):
+        """Each leaf's FP64 augmented system, without ridge: a device tensor
+        (K, gram_size) in the kernels' tile layout, and its invalid-weight
+        flags (K,).  Systems of disjoint row sets add (rank-k refits)."""
+        torch = self.torch
+        if self.plan is None:
+            raise RuntimeError("The plan is closed.")
+        self._check_inputs(X, y, w)
+        size = self.lib.art_gram_size(self.p)
+        grams = torch.empty((self.K, size), dtype=torch.float64, device=self.device)
+        bad = torch.empty(self.K, dtype=torch.int32, device=self.device)
+        _hip.check(self.lib.art_leaf_gram(self.plan, _hip.ptr(X), _hip.ptr(y), _hip.ptr(w),
+                                          float(weight_floor), _hip.ptr(grams), _hip.ptr(bad),
+                                          _hip.stream_handle(self.stream)))
+        return grams, bad
+
     def fallback_count(self):
         """Leaves that took the exact FP64 path in the last mixed fit (synchronizes)."""
         count = ctypes.c_int()
@@ -133,3 +149,27 @@ class LeafRidgeBatch:
             self.close()
         except Exception:
             pass
+
+
+def solve_grams(grams, bad, d, ridge, *, degree=2, include_bias=True, bias_last=False,
+                stream=None):
+    """Fit every resident system of ``LeafRidgeBatch.grams`` (summed over any
+    number of row sets): returns device tensors (coefficients (K, p), status
+    (K,), ridge_effective (K,)), as ``LeafRidgeBatch.fit`` in FP64."""
+    torch = _hip.torch_device()
+    lib = _hip.load()
+    p = _columns(int(d), degree, include_bias, bias_last).shape[1]
+    K = int(grams.shape[0])
+    if grams.dtype != torch.float64 or not grams.is_contiguous() or grams.dim() != 2 \
+            or grams.shape[1] != lib.art_gram_size(p):
+        raise ValueError("grams must be a contiguous float64 (K, gram_size) device tensor.")
+    stream = torch.cuda.current_stream() if stream is None else stream
+    f64 = dict(dtype=torch.float64, device=grams.device)
+    coef = torch.empty((K, p), **f64)
+    status = torch.empty(K, dtype=torch.int32, device=grams.device)
+    lam = torch.empty(K, **f64)
+    _hip.check(lib.art_gram_solve(_hip.ptr(grams), _hip.ptr(bad.contiguous()), K,
+                                  _hip.Basis(int(d), p, int(degree), int(include_bias),
+                                             int(bias_last)), float(ridge), _hip.ptr(coef),
+                                  _hip.ptr(status), _hip.ptr(lam), _hip.stream_handle(stream)))
+    return coef, status, lam

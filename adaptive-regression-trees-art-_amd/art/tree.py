@@ -110,6 +110,7 @@ class FlatTree:
             else:
                 links[id(node)] = -1 - len(leaves)
                 leaves.append(node)
+        self.leaf_nodes = leaves
         degree, bias, bias_last = basis
         p = int(np.asarray(leaves[0].model.coef_).shape[0])
         w = np.zeros((max(len(splits), 1), d))
@@ -133,6 +134,19 @@ class FlatTree:
         self.d, self.device = d, dev
         self.basis = _hip.Basis(d, p, degree, int(bias), int(bias_last))
         self.n_splits, self.n_leaves = len(splits), len(leaves)
+
+    def leaves(self, X, out=None):
+        """Leaf index (breadth-first leaf numbering) of device rows X (n, d)."""
+        from . import _hip
+
+        torch = self.torch
+        n = int(X.shape[0])
+        if out is None:
+            out = torch.empty(n, dtype=torch.int32, device=self.device)
+        _hip.check(self.lib.art_tree_leaf(
+            _hip.ptr(X), n, self.d, _hip.ptr(self.w), _hip.ptr(self.z), _hip.ptr(self.child),
+            self.root, _hip.ptr(out), _hip.stream_handle(torch.cuda.current_stream())))
+        return out
 
     def predict(self, X, out=None):
         """Predictions at device rows X (n, d) float64; returns a device tensor."""

@@ -146,7 +146,8 @@ __device__ unsigned long long key_above(const RowKeys& keys, int n, InitShared& 
 
 __global__ __launch_bounds__(kInitThreads) void split_init_kernel(
     const double* __restrict__ X, int n_all, const int64_t* __restrict__ row_offsets, int d,
-    const int* __restrict__ node, uint64_t seed, uint64_t counter, double* __restrict__ theta) {
+    const int* __restrict__ node, uint64_t seed, uint64_t counter,
+    const uint64_t* __restrict__ start_keys, double* __restrict__ theta) {
   __shared__ InitShared S;
   extern __shared__ unsigned long long init_keys[];
   const int q = blockIdx.x, tid = threadIdx.x;
@@ -156,9 +157,13 @@ __global__ __launch_bounds__(kInitThreads) void split_init_kernel(
   const double* Xn = X + (size_t)r0 * d;
   const int pairs = (d + 1) / 2;
   if (tid < pairs) {
-    const uint4 r = init_philox(make_uint4((uint32_t)tid, (uint32_t)q, (uint32_t)counter,
-                                           (uint32_t)(counter >> 32)),
-                                (uint32_t)seed, (uint32_t)(seed >> 32));
+    // Keyed problems draw from their own key with counter (pair, 0, 0, 0), so
+    // a problem's start does not depend on its position in the launch.
+    const uint64_t key = start_keys ? start_keys[q] : seed;
+    const uint4 ctr = start_keys ? make_uint4((uint32_t)tid, 0u, 0u, 0u)
+                           : make_uint4((uint32_t)tid, (uint32_t)q, (uint32_t)counter,
+                                        (uint32_t)(counter >> 32));
+    const uint4 r = init_philox(ctr, (uint32_t)key, (uint32_t)(key >> 32));
     const double u1 = ((double)(r.x >> 5) * 67108864.0 + (double)(r.y >> 6) + 1.0) * 0x1p-53;
     const double u2 = ((double)(r.z >> 5) * 67108864.0 + (double)(r.w >> 6)) * 0x1p-53;
     const double rad = sqrt(-2.0 * log(u1));
@@ -208,8 +213,8 @@ namespace {
 // allocation covers the largest possible node (n unknown to the host for
 // row ranges).
 int launch_init(const double* X, int n, const int64_t* row_offsets, int d, const int* node,
-                int problems, uint64_t seed, uint64_t counter, double* theta,
-                hipStream_t stream) {
+                int problems, uint64_t seed, uint64_t counter, const uint64_t* start_keys,
+                double* theta, hipStream_t stream) {
   const int smem = (int)(sizeof(unsigned long long) *
                          (row_offsets ? kInitLdsRows : (n <= kInitLdsRows ? n : 0)));
   hipError_t e = hipFuncSetAttribute((const void*)split_init_kernel,
@@ -217,7 +222,7 @@ int launch_init(const double* X, int n, const int64_t* row_offsets, int d, const
                                      (int)(sizeof(unsigned long long) * kInitLdsRows));
   if (e != hipSuccess) return hip_fail(e, "split_init_kernel shared memory attribute");
   split_init_kernel<<<problems, kInitThreads, smem, stream>>>(X, n, row_offsets, d, node, seed,
-                                                              counter, theta);
+                                                              counter, start_keys, theta);
   e = hipGetLastError();
   return e == hipSuccess ? ART_OK : hip_fail(e, "split_init_kernel launch");
 }
@@ -230,14 +235,16 @@ extern "C" int art_split_init(const double* X, int n, int d, const int* node, in
   if (!X || !node || !theta || n < 1 || d < 1 || d > kSplitInitMaxD || problems < 0)
     return fail(ART_ERR_ARG, "art_split_init: bad arguments (1 <= d <= 32, n >= 1)");
   if (problems == 0) return ART_OK;
-  return launch_init(X, n, nullptr, d, node, problems, seed, counter, theta, stream);
+  return launch_init(X, n, nullptr, d, node, problems, seed, counter, nullptr, theta, stream);
 }
 
 extern "C" int art_split_init_ranges(const double* X, const int64_t* row_offsets, int d,
                                      const int* node, int problems, uint64_t seed,
-                                     uint64_t counter, double* theta, hipStream_t stream) {
+                                     uint64_t counter, const uint64_t* start_keys, double* theta,
+                                     hipStream_t stream) {
   if (!X || !row_offsets || !node || !theta || d < 1 || d > kSplitInitMaxD || problems < 0)
     return fail(ART_ERR_ARG, "art_split_init_ranges: bad arguments (1 <= d <= 32)");
   if (problems == 0) return ART_OK;
-  return launch_init(X, 0, row_offsets, d, node, problems, seed, counter, theta, stream);
+  return launch_init(X, 0, row_offsets, d, node, problems, seed, counter, start_keys, theta,
+                     stream);
 }

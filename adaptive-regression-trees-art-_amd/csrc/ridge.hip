@@ -526,7 +526,10 @@ __global__ __launch_bounds__(128) void solve_rows_kernel(
   __shared__ double bc[2];
   __shared__ double red[2];
   __shared__ int ired[2];
-  const SystemDesc S = systems[blockIdx.x];
+  // Without a system table, system k is chunk k alone (resident systems).
+  const SystemDesc S = systems ? systems[blockIdx.x]
+                               : SystemDesc{(int)blockIdx.x, (int)blockIdx.x + 1,
+                                            (int)blockIdx.x, 0};
   if (mask != nullptr && !mask[S.out]) return;
   const int i = threadIdx.x, nt = blockDim.x;
   const int nel = (p + 1) * (p + 2) / 2;
@@ -598,6 +601,25 @@ __global__ __launch_bounds__(128) void solve_rows_kernel(
   if (i == 0) {
     status[S.out] = st;
     lambda[S.out] = lam;
+  }
+}
+
+// Resident systems: leaf j's augmented system is the sum of its chunks'
+// partial tiles (chunks goff[j] .. goff[j+1]); bad[j] ORs their weight flags.
+__global__ __launch_bounds__(256) void reduce_leaf_partials_kernel(
+    const double* __restrict__ partials, const int* __restrict__ flags,
+    const int* __restrict__ goff, int total, double* __restrict__ grams, int* __restrict__ bad) {
+  const int j = blockIdx.x;
+  const int c0 = goff[j], c1 = goff[j + 1];
+  for (int e = threadIdx.x; e < total; e += blockDim.x) {
+    double v = 0.0;
+    for (int c = c0; c < c1; ++c) v += partials[(size_t)c * total + e];
+    grams[(size_t)j * total + e] = v;
+  }
+  if (threadIdx.x == 0) {
+    int b = 0;
+    for (int c = c0; c < c1; ++c) b |= flags[c];
+    bad[j] = b;
   }
 }
 
@@ -1573,6 +1595,53 @@ int art_gated_ridge_fit(art_ridge_plan* P, const double* phi, const double* y, c
   const int rc = launch_solve(P, 2 * count, nchunks, ridge, beta, status, lambda, stream);
   timing_mark(P, 2, stream);
   return rc;
+}
+
+int art_leaf_gram(art_ridge_plan* P, const double* X, const double* y, const double* w,
+                  double weight_floor, double* grams, int* bad, hipStream_t stream) {
+  if (!P || P->kind != 0) return fail(ART_ERR_ARG, "art_leaf_gram: not a leaf plan");
+  if (!X || !y || !w || !grams || !bad) return fail(ART_ERR_ARG, "art_leaf_gram: null pointer");
+  if (!(weight_floor >= 0.0) || std::isinf(weight_floor))
+    return fail(ART_ERR_ARG, "art_leaf_gram: weight_floor must be finite and >= 0");
+  if (P->p > kRowsMaxP) return fail(ART_ERR_ARG, "art_leaf_gram: needs p <= 120");
+  if (P->T <= 6 && P->basis.d <= kMaxStagedD) {
+    launch_gram_leaf_f64(P, X, y, w, weight_floor, nullptr, stream);
+  } else {
+    PolySource src{X, y, w, P->basis, weight_floor};
+    TableChunks map{P->d_chunks};
+    const int groups = (P->npairs + kGramMaxQ - 1) / kGramMaxQ;
+    gram_f64_kernel<<<dim3(P->nchunks, groups, 1), dim3(256), 0, stream>>>(
+        src, map, P->nchunks, P->T, P->npairs, P->d_partials, P->d_flags);
+  }
+  ART_HIP(hipGetLastError());
+  reduce_leaf_partials_kernel<<<dim3(P->K), dim3(256), 0, stream>>>(
+      P->d_partials, P->d_flags, P->d_goff, P->npairs * 256, grams, bad);
+  ART_HIP(hipGetLastError());
+  return ART_OK;
+}
+
+int art_gram_size(int p) {
+  if (p < 1) return -1;
+  const int T = tiles_for(p);
+  return T * (T + 1) / 2 * 256;
+}
+
+int art_gram_solve(const double* grams, const int* bad, int K, art_basis basis, double ridge,
+                   double* beta, int* status, double* lambda, hipStream_t stream) {
+  if (K < 0 || !grams || !bad || !beta || !status || !lambda)
+    return fail(ART_ERR_ARG, "art_gram_solve: bad arguments");
+  const int p = basis.p;
+  if (p < 1 || p > kRowsMaxP) return fail(ART_ERR_ARG, "art_gram_solve: needs 1 <= p <= 120");
+  if (!(ridge >= 0.0) || std::isinf(ridge))
+    return fail(ART_ERR_ARG, "art_gram_solve: ridge must be finite and >= 0");
+  if (K == 0) return ART_OK;
+  const int T = tiles_for(p), npairs = T * (T + 1) / 2;
+  const size_t lds = (size_t)(p + 1) * (p + 2) / 2 * sizeof(double);
+  solve_rows_kernel<<<dim3(K), dim3(64 * ((p + 63) / 64)), lds, stream>>>(
+      nullptr, K, T, npairs, p, ridge, grams, bad, nullptr, nullptr, nullptr, beta, status,
+      lambda);
+  ART_HIP(hipGetLastError());
+  return ART_OK;
 }
 
 int art_leaf_fallback_count(art_ridge_plan* P, int* count) {

@@ -70,6 +70,28 @@ __global__ __launch_bounds__(256) void tree_predict_kernel(
   out[r] = leaf_value<D>(x, B, leaf_coef + (size_t)(-1 - node) * B.p);
 }
 
+template <int D>
+__global__ __launch_bounds__(256) void tree_leaf_kernel(
+    const double* __restrict__ X, int64_t n, int d, const double* __restrict__ node_w,
+    const double* __restrict__ node_z, const int* __restrict__ node_child, int root,
+    int* __restrict__ leaf) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double x[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) x[j] = j < d ? X[r * d + j] : 0.0;
+  int node = root;
+  while (node >= 0) {
+    const double* w = node_w + (size_t)node * d;
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+      if (j < d) s = fma(w[j], x[j], s);
+    node = node_child[2 * node + ((s - node_z[node]) >= 0.0 ? 1 : 0)];
+  }
+  leaf[r] = -1 - node;
+}
+
 __device__ inline uint64_t dist_key(double v) {
   const uint64_t b = (uint64_t)__double_as_longlong(v);
   return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
@@ -256,4 +278,25 @@ extern "C" int art_knn_median(const double* X, const int64_t* offsets, const int
   knn_median_kernel<<<sets, kKnnThreads, 0, stream>>>(X, offsets, d, mode, out);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? ART_OK : hip_fail(e, "knn_median_kernel launch");
+}
+
+extern "C" int art_tree_leaf(const double* X, int64_t n, int d, const double* node_w,
+                             const double* node_z, const int* node_child, int root, int* leaf,
+                             hipStream_t stream) {
+  if (n < 0 || d < 1 || d > 32) return fail(ART_ERR_ARG, "art_tree_leaf: needs 1 <= d <= 32");
+  if (root >= 0 && (!node_w || !node_z || !node_child))
+    return fail(ART_ERR_ARG, "art_tree_leaf: null split arrays");
+  if (n == 0) return ART_OK;
+  if (!X || !leaf) return fail(ART_ERR_ARG, "art_tree_leaf: null pointer");
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  if (d <= 8)
+    tree_leaf_kernel<8><<<blocks, 256, 0, stream>>>(X, n, d, node_w, node_z, node_child, root, leaf);
+  else if (d <= 16)
+    tree_leaf_kernel<16><<<blocks, 256, 0, stream>>>(X, n, d, node_w, node_z, node_child, root,
+                                                     leaf);
+  else
+    tree_leaf_kernel<32><<<blocks, 256, 0, stream>>>(X, n, d, node_w, node_z, node_child, root,
+                                                     leaf);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? ART_OK : hip_fail(e, "tree_leaf_kernel launch");
 }

@@ -216,10 +216,29 @@ int art_split_init(const double* X, int n, int d, const int* node, int problems,
 
 /* art_split_init for nodes of uneven size: node slot k owns rows
  * [row_offsets[k], row_offsets[k+1]) of X (rows, d); row_offsets is a device
- * array.  Same stream and outputs. */
+ * array.  keys (problems,), optional: problem q then draws with key keys[q]
+ * and counter (pair, 0, 0, 0), independent of its position in the call;
+ * NULL gives art_split_init's stream. */
 int art_split_init_ranges(const double* X, const int64_t* row_offsets, int d, const int* node,
-                          int problems, uint64_t seed, uint64_t counter, double* theta,
-                          hipStream_t stream);
+                          int problems, uint64_t seed, uint64_t counter, const uint64_t* keys,
+                          double* theta, hipStream_t stream);
+
+/* Resident systems for rank-k refits.  art_leaf_gram writes each leaf's
+ * FP64 augmented system [Phi | y]^T W [Phi | y] (the fit's Gram and
+ * right-hand side, no ridge) to grams (K, art_gram_size(p)) in tile layout
+ * -- tile pair (t1 <= t2, t1 outermost) element (a, b) is
+ * G[16 t1 + a][16 t2 + b] of the (p+1)-column augmented matrix -- and ORs
+ * the leaf's invalid-weight flags into bad (K,).  Systems add: the system of
+ * old rows plus the system of new rows is the system of all rows.
+ * art_gram_solve fits every resident system as art_leaf_ridge_fit's FP64
+ * path does (trace-scaled ridge, Cholesky, the same status codes).  No
+ * reference counterpart: the reference refits leaves from their rows
+ * (art/builder.py:812-883). */
+int art_leaf_gram(art_ridge_plan* plan, const double* X, const double* y, const double* w,
+                  double weight_floor, double* grams, int* bad, hipStream_t stream);
+int art_gram_size(int p);
+int art_gram_solve(const double* grams, const int* bad, int K, art_basis basis, double ridge,
+                   double* beta, int* status, double* lambda, hipStream_t stream);
 
 /* Frees a plan's device buffers (synchronizes the device).
  * Replaces BatchedCholeskyWorkspace.close (art/_cuda_batched_cholesky.py:339-352). */
@@ -305,6 +324,11 @@ int art_philox4x32_10(const uint32_t* counters, const uint32_t* keys, int64_t co
 int art_tree_predict(const double* X, int64_t n, const double* node_w, const double* node_z,
                      const int* node_child, int root, const double* leaf_coef, art_basis basis,
                      double* out, hipStream_t stream);
+
+/* leaf[r] = index of the leaf that X[r, :] reaches in the flattened tree of
+ * art_tree_predict (the same routing rule and leaf numbering). */
+int art_tree_leaf(const double* X, int64_t n, int d, const double* node_w, const double* node_z,
+                  const int* node_child, int root, int* leaf, hipStream_t stream);
 
 /* out[s] = median over the points of set s of the distance to their nearest
  * other point (mode 0), or median over all point pairs of their distance

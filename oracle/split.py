@@ -345,11 +345,12 @@ def initial_theta(X, w):
     return np.concatenate([w, [float(np.median(X @ w))]])
 
 
-def device_start(X, seed, q, counter):
+def device_start(X, seed, q, counter, key=None):
     """Random start of problem q of an art_split_init call (see module doc).
 
     Pair i uses Philox4x32-10 counter (i, q, counter_lo, counter_hi), key
-    (seed_lo, seed_hi): u1 = ((w0 >> 5) 2^26 + (w1 >> 6) + 1) 2^-53,
+    (seed_lo, seed_hi); a keyed start (key given) uses counter (i, 0, 0, 0)
+    and key (key_lo, key_hi) instead.  u1 = ((w0 >> 5) 2^26 + (w1 >> 6) + 1) 2^-53,
     u2 = ((w2 >> 5) 2^26 + (w3 >> 6)) 2^-53, normals r cos(2 pi u2),
     r sin(2 pi u2), r = sqrt(-2 ln u1).
     """
@@ -358,6 +359,8 @@ def device_start(X, seed, q, counter):
     X = np.asarray(X, dtype=float)
     d = X.shape[1]
     pairs = (d + 1) // 2
+    if key is not None:
+        seed, q, counter = int(key), 0, 0
     ctr = np.array([[i, q, counter & 0xFFFFFFFF, (counter >> 32) & 0xFFFFFFFF]
                     for i in range(pairs)], dtype=np.uint64).astype(np.uint32)
     key = np.tile(np.array([seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF],

@@ -301,3 +301,27 @@ def test_cohort_top_ups_equal_node_by_node_top_ups(torch_gpu, isotropic):
     for u, v in zip(na, nb):
         np.testing.assert_array_equal(u.metadata["X"], v.metadata["X"])
         assert u.metadata["sampling_method"] == v.metadata["sampling_method"]
+
+
+@pytest.mark.parametrize("d", [2, 9, 20])
+def test_tree_leaf_ids_match_host_routing(torch_gpu, d):
+    from art.models import AffineRidgeModel
+    from art.tree import RegressionTree
+
+    torch, rng = torch_gpu, np.random.default_rng(400 + d)
+
+    def make():
+        m = AffineRidgeModel(1e-6)
+        m.coef_ = rng.normal(size=d + 1)
+        return m
+
+    tree = RegressionTree(_random_tree(rng, d, 4, make))
+    flat = tree.flatten()
+    X = rng.uniform(-1, 1, (3000, d))
+    ids = flat.leaves(torch.as_tensor(X, device="cuda")).cpu().numpy()
+    host = np.array([next(i for i, leaf in enumerate(flat.leaf_nodes)
+                          if leaf is tree.leaf_for_point(x)) for x in X])
+    margin = np.min(np.abs(np.stack([X @ n.w - n.z for n in tree.iter_nodes()
+                                     if hasattr(n, "w")])), axis=0)
+    ok = margin > 1e-12
+    np.testing.assert_array_equal(ids[ok], host[ok])

@@ -380,3 +380,69 @@ def test_mixed_rejects_wide_bases(torch_gpu):
         with pytest.raises(ValueError, match="mixed"):
             batch.fit(dev(torch, X), dev(torch, X[:, 0]), dev(torch, np.ones(200)), 1e-6,
                       precision="mixed")
+
+
+# --------------------------------------------------------------------------- resident systems
+
+@pytest.mark.parametrize("d,degree", [(2, 2), (5, 2), (8, 2), (12, 1)])
+def test_resident_grams_solve_like_the_fp64_fit(torch_gpu, d, degree):
+    """grams() followed by solve_grams() gives LeafRidgeBatch.fit's FP64 answer."""
+    from art import LeafRidgeBatch
+    from art.leaf_ridge import solve_grams
+
+    torch, rng = torch_gpu, np.random.default_rng(70 + d)
+    sizes = np.array([300, 1, 17, 0, 129, 2055, 64])
+    offsets = np.concatenate([[0], np.cumsum(sizes)])
+    N = offsets[-1]
+    X = rng.uniform(-1, 1, size=(N, d))
+    y, w = np.sin(2 * X[:, 0]) + X[:, -1] ** 2, rng.uniform(0.05, 1.0, size=N)
+    w[offsets[2] + 4] = np.nan
+    Xd, yd, wd = dev(torch, X), dev(torch, y), dev(torch, w)
+    with LeafRidgeBatch(offsets, d, degree=degree) as batch:
+        c64, s64, l64 = (t.clone() for t in batch.fit(Xd, yd, wd, 1e-4))
+        grams, bad = batch.grams(Xd, yd, wd)
+        coef, status, lam = solve_grams(grams, bad, d, 1e-4, degree=degree)
+        torch.cuda.synchronize()
+    np.testing.assert_array_equal(status.cpu().numpy(), s64.cpu().numpy())
+    assert status.cpu().numpy()[2] == -1
+    ok = s64.cpu().numpy() == 0
+    np.testing.assert_allclose(coef.cpu().numpy()[ok], c64.cpu().numpy()[ok], rtol=1e-11,
+                               atol=1e-13)
+    np.testing.assert_allclose(lam.cpu().numpy(), l64.cpu().numpy(), rtol=1e-13)
+
+
+def test_resident_grams_of_disjoint_rows_add(torch_gpu):
+    """Rank-k refits: the systems of two disjoint row sets of each leaf sum to
+    the system of their union, and its solve matches the oracle on all rows."""
+    from art import LeafRidgeBatch
+    from art.leaf_ridge import solve_grams
+
+    torch, rng = torch_gpu, np.random.default_rng(71)
+    d, sizes_a, sizes_b = 3, np.array([200, 40, 1000, 7]), np.array([30, 500, 2, 90])
+    Xa = [rng.uniform(-1, 1, (n, d)) for n in sizes_a]
+    Xb = [rng.uniform(-1, 1, (n, d)) for n in sizes_b]
+    f = lambda X: np.exp(X[:, 0]) - X[:, 1] * X[:, 2]
+    wa = [rng.uniform(0.1, 1, n) for n in sizes_a]
+    wb = [rng.uniform(0.1, 1, n) for n in sizes_b]
+
+    def grams_of(Xs, ws):
+        offsets = np.concatenate([[0], np.cumsum([len(x) for x in Xs])])
+        X = np.vstack(Xs)
+        with LeafRidgeBatch(offsets, d) as batch:
+            g, b = batch.grams(dev(torch, X), dev(torch, f(X)), dev(torch, np.concatenate(ws)))
+            torch.cuda.synchronize()
+        return g, b
+
+    ga, ba = grams_of(Xa, wa)
+    gb, bb = grams_of(Xb, wb)
+    gu, bu = grams_of([np.vstack(p) for p in zip(Xa, Xb)],
+                      [np.concatenate(p) for p in zip(wa, wb)])
+    np.testing.assert_allclose((ga + gb).cpu().numpy(), gu.cpu().numpy(), rtol=1e-12,
+                               atol=1e-12)
+    coef, status, _ = solve_grams(ga + gb, ba | bb, d, 1e-6)
+    Xu = np.vstack([np.vstack(p) for p in zip(Xa, Xb)])
+    wu = np.concatenate([np.concatenate(p) for p in zip(wa, wb)])
+    offsets = np.concatenate([[0], np.cumsum(sizes_a + sizes_b)])
+    ref, st_ref, _ = R.leaf_fit(Xu, f(Xu), wu, offsets, 1e-6, 1e-12, 2, True, False)
+    np.testing.assert_array_equal(status.cpu().numpy(), st_ref)
+    assert _rel_err(coef.cpu().numpy(), ref).max() < 1e-9

@@ -504,3 +504,26 @@ def test_ragged_layout_of_equal_nodes_equals_the_block_layout(torch_gpu):
         if isinstance(u, SplitResult):
             np.testing.assert_array_equal(np.r_[u.w, u.z], np.r_[v.w, v.z])
             assert u.loss == v.loss and u.n_right == v.n_right
+
+
+def test_keyed_starts_do_not_depend_on_position(torch_gpu):
+    from art import AffineRidgeModel, SoftObliqueSplitter
+    from art.batched_optimizer import RaggedOptimizerWorkspace
+
+    rng = np.random.default_rng(89)
+    sizes = [150, 3000, 77, 9000]
+    X = rng.uniform(-1, 1, (sum(sizes), 3))
+    offsets = np.concatenate([[0], np.cumsum(sizes)])
+    s = SoftObliqueSplitter(AffineRidgeModel(1e-6), .3, max_iters=5)
+    keys = np.array([11, 2**63 + 5, 7, 12345678901234], dtype=np.uint64)
+    with RaggedOptimizerWorkspace(s, X, X[:, 0].copy(), offsets) as ws:
+        a = ws._device_starts([0, 1, 2, 3], keys=keys)
+        b = ws._device_starts([3, 1, 0], keys=keys[[3, 1, 0]])
+        c = ws._device_starts([0, 0], keys=keys[[0, 2]])
+    np.testing.assert_array_equal(b, a[[3, 1, 0]])
+    assert not np.array_equal(c[0], c[1])
+    np.testing.assert_array_equal(c[0], a[0])
+    for j in range(4):
+        np.testing.assert_allclose(a[j], O.device_start(X[offsets[j]:offsets[j + 1]], 0, 0, 0,
+                                                        key=int(keys[j])),
+                                   rtol=1e-12, atol=1e-13)
