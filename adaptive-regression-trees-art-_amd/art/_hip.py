@@ -106,6 +106,11 @@ _PROTOTYPES = {
     "art_gram_solve": (_i, [_p, _p, _i, Basis, ctypes.c_double, _p, _p, _p, _p]),
     "art_knn_median": (_i, [_p, _p, _p, _i, _i, _i, _p, _p]),
     "art_tree_leaf": (_i, [_p, _i64, _i, _p, _p, _p, _i, _p, _p]),
+    "art_gram_sse": (_i, [_p, _i, _i, _i, _p, _p, _p, _p]),
+    "art_gather_strided": (_i, [_p, _p, _p, _i, _i, _p, _p, _p, _i64, _p, _p, _p, _p]),
+    "art_partition_rows": (_i, [_p, _p, _p, _i, _i, _p, _p, _p, _i64, _p, _p, _p, _p, _p, _p,
+                                _p, _p, _p, _p]),
+    "art_partition_block_rows": (_i, []),
 }
 
 

@@ -538,15 +538,26 @@ class RaggedOptimizerWorkspace(BatchedOptimizerWorkspace):
     """
 
     def __init__(self, splitter, X, y, offsets, *, temperature=None):
-        X = np.ascontiguousarray(X, dtype=float)
-        y = np.ascontiguousarray(y, dtype=float).reshape(-1)
+        torch = _hip.torch_device()
+        on_device = isinstance(X, torch.Tensor)
+        if on_device:
+            # Device rows (already resident): float64, C-contiguous, finite by
+            # the caller's contract; they are used in place.
+            if (not isinstance(y, torch.Tensor) or not X.is_cuda or not y.is_cuda
+                    or X.dtype != torch.float64 or y.dtype != torch.float64
+                    or not X.is_contiguous() or not y.is_contiguous()):
+                raise ValueError("Device X and y must be contiguous float64 tensors.")
+            y = y.reshape(-1)
+        else:
+            X = np.ascontiguousarray(X, dtype=float)
+            y = np.ascontiguousarray(y, dtype=float).reshape(-1)
         offsets = np.ascontiguousarray(offsets, dtype=np.int64).reshape(-1)
         if X.ndim != 2 or X.shape[0] != y.shape[0] or offsets.size < 2:
             raise ValueError("X/y must have shapes (N,d)/(N,) and offsets (K+1,).")
         sizes = np.diff(offsets)
         if offsets[0] != 0 or offsets[-1] != X.shape[0] or np.any(sizes < 2):
             raise ValueError("offsets must run from 0 to N with at least two rows per node.")
-        if not np.isfinite(X).all() or not np.isfinite(y).all():
+        if not on_device and (not np.isfinite(X).all() or not np.isfinite(y).all()):
             raise ValueError("X and y must be finite.")
         K = sizes.size
         temp = np.broadcast_to(np.asarray(splitter.temperature if temperature is None
@@ -558,8 +569,8 @@ class RaggedOptimizerWorkspace(BatchedOptimizerWorkspace):
         self.required_k = [splitter._required_side_points(int(n)) for n in sizes]
         torch = self.torch
         with torch.cuda.stream(self.stream):
-            self.x = torch.from_numpy(X).to(self.device)
-            self.yd = torch.from_numpy(y).to(self.device)
+            self.x = X if on_device else torch.from_numpy(X).to(self.device)
+            self.yd = y if on_device else torch.from_numpy(y).to(self.device)
             self.offsets_dev = torch.from_numpy(offsets).to(self.device)
             self.temperature[:] = temp
             self.temp.copy_(torch.from_numpy(self.temperature.copy()))

@@ -1,0 +1,796 @@
+"""Serving loop: continuously batched tree fits and rank-k refits on one GPU.
+
+The reference builds one tree per process call (builder.py) and has no
+request loop.  ``ServingEngine`` accepts fit and refit requests from any
+thread and serves them from one scheduler thread:
+
+* **Fit** (``FitRequest``): a data-driven oblique regression tree on the
+  request's rows.  Every level, each frontier node's weighted ridge model
+  comes from its augmented system ``[Phi | y]^T W [Phi | y]`` (FP64 MFMA,
+  ``art_leaf_gram``) solved on the device (``art_gram_solve``); its weighted
+  SSE is read from the same system (``art_gram_sse``).  A node stays a leaf
+  at ``max_depth``, below ``2 * min_leaf`` rows or at a weighted MSE of at
+  most ``error_threshold``; otherwise the soft-oblique split optimizer
+  (``RaggedOptimizerWorkspace``) searches a strided sample of at most
+  ``split_rows`` of its rows from keyed random starts, and
+  ``art_partition_rows`` moves the rows to its children (right when
+  ``w.x - z >= 0``) in the next level buffer.
+* **Refit** (``RefitRequest``): new rows of a resident tree are routed to
+  their leaves (``art_tree_leaf``), their systems are added to the leaves'
+  resident systems (a rank-k update ``G += Phi_new^T W Phi_new``) and every
+  leaf is re-solved; the tree's structure is kept.
+
+Continuous batching: all requests of one basis (d, degree) share one level
+buffer, so one launch of each kernel serves every admitted request's
+frontier; requests that arrive while a level runs are admitted at the next
+level, as slots and HBM budget allow.  Every node's result depends only on
+its own rows and key, so a request's result is the same alone and in any
+mix (bit for bit).
+
+Leaf systems are exact FP64: they stay resident for refits, and a reduced
+precision system would carry its error into every later rank-k update.
+"""
+
+from __future__ import annotations
+
+import collections
+import itertools
+import threading
+import time
+from concurrent.futures import Future
+from dataclasses import dataclass, field
+from typing import Any, Optional
+
+import numpy as np
+
+from . import _hip
+
+_MASK64 = (1 << 64) - 1
+
+
+class AdmissionError(RuntimeError):
+    """A request that can never fit in the engine's HBM budget."""
+
+
+@dataclass
+class FitRequest:
+    """Fit a tree to rows X (N, d), y (N,), optional weights w (N,) >= 0.
+
+    X, y and w may be host arrays or float64 device tensors (used in place).
+    ``degree`` defaults to 2 for d <= 12 and 1 above (leaf basis
+    [1, x, x_i x_j] or [1, x]).  ``seed`` keys the split search's random
+    starts; ``tree_id`` names the resident tree (default: a fresh name).
+    """
+
+    X: Any
+    y: Any
+    w: Any = None
+    max_depth: int = 10
+    min_leaf: int = 64
+    degree: Optional[int] = None
+    error_threshold: float = 0.0
+    temperature: float = 0.05
+    seed: int = 0
+    tree_id: Optional[str] = None
+
+
+@dataclass
+class RefitRequest:
+    """Add rows X (k, d), y (k,), w (k,) to resident tree ``tree_id``."""
+
+    tree_id: str
+    X: Any
+    y: Any
+    w: Any = None
+
+
+@dataclass
+class RequestTiming:
+    submitted: float
+    admitted: float = float("nan")
+    finished: float = float("nan")
+
+    @property
+    def latency(self):
+        return self.finished - self.submitted
+
+    @property
+    def queued(self):
+        return self.admitted - self.submitted
+
+
+@dataclass
+class FitResult:
+    tree_id: str
+    tree: Any
+    coefficients: np.ndarray   # (leaves, p) in the flattened tree's leaf order
+    status: np.ndarray         # (leaves,) solve status per leaf (0 = ok)
+    n_rows: int
+    timing: RequestTiming
+
+
+@dataclass
+class RefitResult:
+    tree_id: str
+    tree: Any
+    coefficients: np.ndarray
+    status: np.ndarray
+    n_new_rows: int
+    timing: RequestTiming
+
+
+@dataclass
+class ServingConfig:
+    """Engine-wide settings (shared by every request of a launch)."""
+
+    hbm_budget_bytes: float = 200e9
+    max_active: int = 512          # fit requests on the device at once
+    ridge: float = 1e-6            # leaf ridge (trace-scaled, models.py:209-215)
+    weight_floor: float = 1e-12
+    split_rows: int = 4096         # split-search sample per node
+    split_ridge: float = 1e-4
+    n_restarts: int = 1
+    max_iters: int = 30
+    min_side_fraction: float = 0.05
+    quadratic_max_d: int = 12
+
+
+def _splitmix64(x):
+    x = (x + 0x9E3779B97F4A7C15) & _MASK64
+    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & _MASK64
+    x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & _MASK64
+    return x ^ (x >> 31)
+
+
+def start_key(seed, heap, restart):
+    """Random-start key of node ``heap`` (root 1, children 2h and 2h+1)."""
+    return _splitmix64((int(seed) & _MASK64) ^ _splitmix64((int(heap) << 8) | int(restart)))
+
+
+def _heap_path(heap):
+    """Node id of the reference's scheme ("root", "root/L", ...) for a heap index."""
+    bits = bin(heap)[3:]
+    return "/".join(["root"] + ["L" if b == "0" else "R" for b in bits])
+
+
+# --------------------------------------------------------------------------- device helpers
+
+def gram_sse(grams, beta, p, *, bias_col=0, stream=None):
+    """Weighted SSE and weight sum of each system's fit (device tensors (K,))."""
+    torch = _hip.torch_device()
+    lib = _hip.load()
+    K = int(grams.shape[0])
+    sse = torch.empty(K, dtype=torch.float64, device=grams.device)
+    wsum = torch.empty(K, dtype=torch.float64, device=grams.device)
+    stream = torch.cuda.current_stream() if stream is None else stream
+    _hip.check(lib.art_gram_sse(_hip.ptr(grams), K, int(p), int(bias_col), _hip.ptr(beta),
+                                _hip.ptr(sse), _hip.ptr(wsum), _hip.stream_handle(stream)))
+    return sse, wsum
+
+
+def gather_strided(X, y, w, begin, end, counts, *, stream=None):
+    """Strided row samples of ranges [begin[k], end[k]) with counts[k] rows each."""
+    torch = _hip.torch_device()
+    lib = _hip.load()
+    dev, d = X.device, int(X.shape[1])
+    counts = np.asarray(counts, dtype=np.int64)
+    dst = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    total = int(dst[-1])
+    Xo = torch.empty((total, d), dtype=torch.float64, device=dev)
+    yo = torch.empty(total, dtype=torch.float64, device=dev)
+    wo = None if w is None else torch.empty(total, dtype=torch.float64, device=dev)
+    b = torch.from_numpy(np.asarray(begin, dtype=np.int64)).to(dev)
+    e = torch.from_numpy(np.asarray(end, dtype=np.int64)).to(dev)
+    dd = torch.from_numpy(dst).to(dev)
+    stream = torch.cuda.current_stream() if stream is None else stream
+    _hip.check(lib.art_gather_strided(
+        _hip.ptr(X), _hip.ptr(y), 0 if w is None else _hip.ptr(w), d, len(counts), _hip.ptr(b),
+        _hip.ptr(e), _hip.ptr(dd), total, _hip.ptr(Xo), _hip.ptr(yo),
+        0 if wo is None else _hip.ptr(wo), _hip.stream_handle(stream)))
+    return Xo, yo, wo, dst
+
+
+def partition_rows(X, y, w, begin, end, node_w, node_z, out, out_base, *, stream=None):
+    """Stable partition of ranges [begin[k], end[k]) by hyperplanes (node_w, node_z)
+    into ``out`` = (Xo, yo, wo) at rows out_base[k] + ...; returns the right
+    counts (host int64 array, synchronizes)."""
+    torch = _hip.torch_device()
+    lib = _hip.load()
+    dev, d = X.device, int(X.shape[1])
+    begin = np.asarray(begin, dtype=np.int64)
+    end = np.asarray(end, dtype=np.int64)
+    K = begin.shape[0]
+    if K == 0:
+        return np.zeros(0, dtype=np.int64)
+    rows = lib.art_partition_block_rows()
+    nblk = (end - begin + rows - 1) // rows
+    blk = np.concatenate([[0], np.cumsum(nblk)]).astype(np.int64)
+    blocks = int(blk[-1])
+    table = torch.from_numpy(np.concatenate([begin, end, blk, np.asarray(out_base, np.int64)]))
+    table = table.to(dev)
+    tb, te = table[:K], table[K:2 * K]
+    tblk, tdst = table[2 * K:3 * K + 1], table[3 * K + 1:]
+    nw = torch.from_numpy(np.ascontiguousarray(node_w, dtype=np.float64)).to(dev)
+    nz = torch.from_numpy(np.ascontiguousarray(node_z, dtype=np.float64)).to(dev)
+    flags = torch.empty(int(X.shape[0]), dtype=torch.uint8, device=dev)
+    blk_right = torch.empty(max(blocks, 1), dtype=torch.int32, device=dev)
+    node_right = torch.empty(K, dtype=torch.int64, device=dev)
+    Xo, yo, wo = out
+    stream = torch.cuda.current_stream() if stream is None else stream
+    _hip.check(lib.art_partition_rows(
+        _hip.ptr(X), _hip.ptr(y), 0 if w is None else _hip.ptr(w), d, K, _hip.ptr(tb),
+        _hip.ptr(te), _hip.ptr(tblk), blocks, _hip.ptr(tdst), _hip.ptr(nw), _hip.ptr(nz),
+        _hip.ptr(flags), _hip.ptr(blk_right), _hip.ptr(node_right), _hip.ptr(Xo), _hip.ptr(yo),
+        0 if wo is None else _hip.ptr(wo), _hip.stream_handle(stream)))
+    return node_right.cpu().numpy()
+
+
+# --------------------------------------------------------------------------- engine state
+
+@dataclass
+class _Node:
+    heap: int
+    depth: int
+    n_rows: int
+    split: Optional[tuple] = None      # (w, z, metadata)
+    leaf: Optional[dict] = None        # coef, status, mse, n_rows
+    gram: Any = None                   # device (G,) resident system of a leaf
+    bad: int = 0
+    leaf_candidate: Optional[dict] = None   # the node's own fit while its split is open
+    gram_candidate: Any = None
+
+
+@dataclass
+class _FitJob:
+    req: FitRequest
+    future: Future
+    timing: RequestTiming
+    tree_id: str
+    d: int
+    degree: int
+    n_rows: int
+    est_bytes: float
+    X: Any = None
+    y: Any = None
+    w: Any = None
+    nodes: dict = field(default_factory=dict)
+    open: int = 1                       # nodes not yet decided
+
+
+@dataclass
+class _Refit:
+    req: RefitRequest
+    future: Future
+    timing: RequestTiming
+    X: Any = None
+    y: Any = None
+    w: Any = None
+
+
+@dataclass
+class _Resident:
+    tree_id: str
+    tree: Any
+    flat: Any
+    grams: Any
+    bad: Any
+    d: int
+    degree: int
+    p: int
+    bytes: float
+    last_used: float
+
+
+class _Group:
+    """Level buffer of every fit request with basis (d, degree)."""
+
+    def __init__(self, d, degree):
+        self.d, self.degree = d, degree
+        self.X = self.y = self.w = None
+        self.entries = []        # (job, node) in buffer order; node None = unused range
+        self.sizes = []
+        self.pending = None      # splits to apply: [(job, node, begin, end, w, z)]
+        self.new_jobs = []
+
+
+class ServingEngine:
+    """Request loop over one GPU; ``submit`` returns a ``Future``."""
+
+    def __init__(self, config: Optional[ServingConfig] = None, *, start=True):
+        self.config = ServingConfig() if config is None else config
+        torch = _hip.torch_device()
+        self.torch, self.lib = torch, _hip.load()
+        self.device = torch.device("cuda", torch.cuda.current_device())
+        self._cv = threading.Condition()
+        self._queue = collections.deque()
+        self._waiting = collections.deque()   # submitted, not yet admitted
+        self._groups = {}
+        self._refits = []
+        self._resident = collections.OrderedDict()
+        self._fitting = set()                 # tree ids of fits in flight
+        self._inflight_bytes = 0.0
+        self._resident_bytes = 0.0
+        self._active = 0
+        self._names = itertools.count()
+        self._closing = False
+        self._error = None
+        self.steps = 0
+        self.completed = 0
+        self._thread = None
+        self.stream = None
+        if start:
+            self.start()
+
+    # ----------------------------------------------------------------- public API
+
+    def start(self):
+        if self._thread is not None:
+            return
+        self.stream = self.torch.cuda.Stream(device=self.device)
+        self._thread = threading.Thread(target=self._run, name="art-serving", daemon=True)
+        self._thread.start()
+
+    def submit(self, request):
+        """Queue a FitRequest or RefitRequest; returns a Future of its result."""
+        if not isinstance(request, (FitRequest, RefitRequest)):
+            raise TypeError("submit takes a FitRequest or a RefitRequest.")
+        fut = Future()
+        with self._cv:
+            if self._closing:
+                raise RuntimeError("The engine is closed.")
+            if self._error is not None:
+                raise RuntimeError("The engine stopped after an error.") from self._error
+            self._queue.append((request, fut, RequestTiming(time.perf_counter())))
+            self._cv.notify()
+        return fut
+
+    def fit(self, request):
+        return self.submit(request).result()
+
+    def tree(self, tree_id):
+        with self._cv:
+            return self._resident[tree_id].tree
+
+    def evict(self, tree_id):
+        with self._cv:
+            r = self._resident.pop(tree_id)
+            self._resident_bytes -= r.bytes
+
+    def resident_ids(self):
+        with self._cv:
+            return list(self._resident)
+
+    def close(self, wait=True):
+        with self._cv:
+            self._closing = True
+            self._cv.notify()
+        if wait and self._thread is not None:
+            self._thread.join()
+        self._thread = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
+
+    # ----------------------------------------------------------------- scheduler
+
+    def _busy(self):
+        return self._active > 0 or bool(self._refits) or bool(self._waiting)
+
+    def _run(self):
+        torch = self.torch
+        torch.cuda.set_device(self.device)
+        try:
+            with torch.cuda.stream(self.stream):
+                while True:
+                    with self._cv:
+                        while not self._queue and not self._closing and not self._busy():
+                            self._cv.wait()
+                        incoming = list(self._queue)
+                        self._queue.clear()
+                        if self._closing and not incoming and not self._busy():
+                            break
+                    self._waiting.extend(incoming)
+                    self._admit()
+                    self._step()
+        except BaseException as exc:  # fail every open request loudly
+            self._error = exc
+            self._fail_all(exc)
+
+    def _fail_all(self, exc):
+        with self._cv:
+            pending = list(self._queue) + list(self._waiting)
+            self._queue.clear()
+            self._waiting.clear()
+        for _, fut, _ in pending:
+            if not fut.done():
+                fut.set_exception(exc)
+        for g in self._groups.values():
+            for job in {id(j): j for j, _ in g.entries if j is not None}.values():
+                if not job.future.done():
+                    job.future.set_exception(exc)
+            for job in g.new_jobs:
+                if not job.future.done():
+                    job.future.set_exception(exc)
+        for r in self._refits:
+            if not r.future.done():
+                r.future.set_exception(exc)
+
+    # ----------------------------------------------------------------- admission
+
+    def _basis(self, d, degree):
+        if degree is None:
+            degree = 2 if d <= self.config.quadratic_max_d else 1
+        if degree not in (1, 2):
+            raise ValueError("degree must be 1 or 2.")
+        p = (1 + d + d * (d + 1) // 2) if degree == 2 else d + 1
+        return degree, p
+
+    def _fit_bytes(self, n, d, p, max_depth, min_leaf):
+        gsize = self.lib.art_gram_size(p)
+        leaves = min(2 ** max_depth, max(1, n // max(min_leaf, 1)))
+        return 8.0 * (3 * n * (d + 2) + 2 * leaves * gsize + 2 * leaves * p)
+
+    def _to_device(self, a, shape, name):
+        torch = self.torch
+        if a is None:
+            return None
+        if isinstance(a, torch.Tensor):
+            if a.dtype != torch.float64 or tuple(a.shape) != shape or not a.is_cuda:
+                raise ValueError(f"{name} must be a float64 device tensor of shape {shape}.")
+            return a.contiguous()
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.shape != shape:
+            raise ValueError(f"{name} must have shape {shape}.")
+        if not np.isfinite(a).all():
+            raise ValueError(f"{name} must be finite.")
+        return torch.from_numpy(a).to(self.device)
+
+    def _validate_fit(self, req):
+        X = req.X
+        shape = tuple(X.shape)
+        if len(shape) != 2 or shape[0] < 1 or shape[1] < 1 or shape[1] > 31:
+            raise ValueError("X must have shape (N, d) with N >= 1 and 1 <= d <= 31.")
+        if not isinstance(req.max_depth, (int, np.integer)) or req.max_depth < 0 \
+                or req.max_depth > 60:
+            raise ValueError("max_depth must be an integer in [0, 60].")
+        if not isinstance(req.min_leaf, (int, np.integer)) or req.min_leaf < 1:
+            raise ValueError("min_leaf must be a positive integer.")
+        if not np.isfinite(req.temperature) or req.temperature <= 0:
+            raise ValueError("temperature must be finite and positive.")
+        if not np.isfinite(req.error_threshold) or req.error_threshold < 0:
+            raise ValueError("error_threshold must be finite and nonnegative.")
+        return shape
+
+    def _admit(self):
+        """Admit waiting requests in arrival order while slots and HBM allow
+        (a later request may pass one that does not fit yet)."""
+        cfg = self.config
+        keep = collections.deque()
+        refit_trees = {r.req.tree_id for r in self._refits}
+        while self._waiting:
+            req, fut, timing = self._waiting.popleft()
+            try:
+                if isinstance(req, RefitRequest):
+                    if req.tree_id in refit_trees or req.tree_id in self._fitting:
+                        keep.append((req, fut, timing))   # one refit per tree per level
+                        continue
+                    res = self._resident.get(req.tree_id)
+                    if res is None:
+                        raise KeyError(f"No resident tree {req.tree_id!r}.")
+                    k = int(req.X.shape[0]) if hasattr(req.X, "shape") else -1
+                    if len(tuple(req.X.shape)) != 2 or req.X.shape[1] != res.d or k < 1:
+                        raise ValueError(f"Refit rows must have shape (k, {res.d}), k >= 1.")
+                    refit = _Refit(req, fut, timing)
+                    refit.X = self._to_device(req.X, (k, res.d), "X")
+                    refit.y = self._to_device(req.y, (k,), "y")
+                    refit.w = self._to_device(req.w, (k,), "w")
+                    refit.timing.admitted = time.perf_counter()
+                    self._refits.append(refit)
+                    refit_trees.add(req.tree_id)
+                    continue
+                n, d = self._validate_fit(req)
+                degree, p = self._basis(d, req.degree)
+                est = self._fit_bytes(n, d, p, req.max_depth, req.min_leaf)
+                if est > cfg.hbm_budget_bytes:
+                    raise AdmissionError(f"Request needs ~{est / 1e9:.1f} GB, over the "
+                                         f"{cfg.hbm_budget_bytes / 1e9:.1f} GB budget.")
+                if self._active >= cfg.max_active or not self._reserve(est):
+                    keep.append((req, fut, timing))
+                    continue
+                tree_id = req.tree_id if req.tree_id is not None else f"tree-{next(self._names)}"
+                job = _FitJob(req, fut, timing, tree_id, d, degree, n, est)
+                job.X = self._to_device(req.X, (n, d), "X")
+                job.y = self._to_device(req.y, (n,), "y")
+                job.w = self._to_device(req.w, (n,), "w")
+                job.nodes[1] = _Node(1, 0, n)
+                job.timing.admitted = time.perf_counter()
+                self._inflight_bytes += est
+                self._active += 1
+                self._fitting.add(tree_id)
+                self._groups.setdefault((d, degree), _Group(d, degree)).new_jobs.append(job)
+            except Exception as exc:  # a bad request fails alone
+                fut.set_exception(exc)
+        self._waiting = keep
+
+    def _reserve(self, est):
+        """True when est more bytes fit the budget, evicting least recently
+        used resident trees that no admitted refit needs."""
+        budget = self.config.hbm_budget_bytes
+        if self._inflight_bytes + self._resident_bytes + est <= budget:
+            return True
+        busy = {r.req.tree_id for r in self._refits}
+        busy |= {req.tree_id for req, _, _ in self._waiting if isinstance(req, RefitRequest)}
+        with self._cv:
+            for tid in list(self._resident):
+                if self._inflight_bytes + self._resident_bytes + est <= budget:
+                    break
+                if tid in busy:
+                    continue
+                r = self._resident.pop(tid)
+                self._resident_bytes -= r.bytes
+        return self._inflight_bytes + self._resident_bytes + est <= budget
+
+    # ----------------------------------------------------------------- one level
+
+    def _step(self):
+        self.steps += 1
+        for key in list(self._groups):
+            g = self._groups[key]
+            self._build_level(g)
+            if g.entries:
+                self._level(g)
+            if not g.entries and not g.new_jobs and g.pending is None:
+                del self._groups[key]
+        if self._refits:
+            self._run_refits()
+
+    def _build_level(self, g):
+        """Next level buffer: children of last level's splits, then new roots."""
+        torch, d = self.torch, g.d
+        pend = g.pending or []
+        child_rows = sum(int(e - b) for _, _, b, e, _, _ in pend)
+        root_rows = sum(job.n_rows for job in g.new_jobs)
+        total = child_rows + root_rows
+        if total == 0:
+            g.entries, g.sizes, g.pending = [], [], None
+            g.X = g.y = g.w = None
+            return
+        f64 = dict(dtype=torch.float64, device=self.device)
+        X = torch.empty((total, d), **f64)
+        y = torch.empty(total, **f64)
+        w = torch.empty(total, **f64)
+        entries, sizes = [], []
+        if pend:
+            begin = [b for _, _, b, _, _, _ in pend]
+            end = [e for _, _, _, e, _, _ in pend]
+            base = np.concatenate([[0], np.cumsum(np.subtract(end, begin))[:-1]])
+            right = partition_rows(g.X, g.y, g.w, begin, end, np.stack([p[4] for p in pend]),
+                                   [p[5] for p in pend], (X, y, w), base, stream=self.stream)
+            for (job, node, b, e, _, _), nr in zip(pend, right):
+                n = int(e - b)
+                nl = n - int(nr)
+                for side, cnt in ((0, nl), (1, int(nr))):
+                    if nl == 0 or nr == 0:
+                        # Degenerate partition (sample and rows disagree at the
+                        # boundary): the node becomes a leaf, its rows unused.
+                        entries.append((None, None))
+                    else:
+                        child = _Node(2 * node.heap + side, node.depth + 1, cnt)
+                        job.nodes[child.heap] = child
+                        job.open += 1
+                        entries.append((job, child))
+                    sizes.append(cnt)
+                if nl == 0 or nr == 0:
+                    node.split = None
+                    node.leaf = node.leaf_candidate
+                    node.gram = node.gram_candidate
+                else:
+                    node.leaf_candidate = node.gram_candidate = None
+                job.open -= 1
+                self._maybe_finish(job)
+        off = child_rows
+        for job in g.new_jobs:
+            n = job.n_rows
+            X[off:off + n].copy_(job.X)
+            y[off:off + n].copy_(job.y)
+            if job.w is None:
+                w[off:off + n].fill_(1.0)
+            else:
+                w[off:off + n].copy_(job.w)
+            job.X = job.y = job.w = None   # the level buffer holds the rows now
+            entries.append((job, job.nodes[1]))
+            sizes.append(n)
+            off += n
+        g.new_jobs = []
+        g.X, g.y, g.w = X, y, w
+        g.entries, g.sizes, g.pending = entries, sizes, None
+
+    def _level(self, g):
+        from .leaf_ridge import LeafRidgeBatch, solve_grams
+
+        torch, cfg, d = self.torch, self.config, g.d
+        degree, p = self._basis(d, g.degree)
+        offsets = np.concatenate([[0], np.cumsum(g.sizes)]).astype(np.int64)
+        with LeafRidgeBatch(offsets, d, degree=degree, stream=self.stream) as lb:
+            grams, bad = lb.grams(g.X, g.y, g.w, cfg.weight_floor)
+            coef, status, _ = solve_grams(grams, bad, d, cfg.ridge, degree=degree,
+                                          stream=self.stream)
+            sse, wsum = gram_sse(grams, coef, p, bias_col=0, stream=self.stream)
+            self.stream.synchronize()
+        coef_h, status_h = coef.cpu().numpy(), status.cpu().numpy()
+        sse_h, wsum_h, bad_h = sse.cpu().numpy(), wsum.cpu().numpy(), bad.cpu().numpy()
+        split_idx = []
+        for i, (job, node) in enumerate(g.entries):
+            if job is None:
+                continue
+            req = job.req
+            mse = sse_h[i] / wsum_h[i] if wsum_h[i] > 0 else float("inf")
+            leaf = {"coef": coef_h[i], "status": int(status_h[i]), "mse": float(mse),
+                    "n_rows": node.n_rows}
+            stop = (node.depth >= req.max_depth or node.n_rows < 2 * req.min_leaf
+                    or node.n_rows < 2 * (d + 2) or mse <= req.error_threshold)
+            if stop:
+                node.leaf, node.gram, node.bad = leaf, grams[i].clone(), int(bad_h[i])
+                job.open -= 1
+                self._maybe_finish(job)
+            else:
+                node.leaf_candidate, node.gram_candidate = leaf, grams[i].clone()
+                node.bad = int(bad_h[i])
+                split_idx.append(i)
+        if split_idx:
+            self._split(g, offsets, split_idx)
+
+    def _split(self, g, offsets, idx):
+        from .batched_optimizer import RaggedOptimizerWorkspace
+        from .models import AffineRidgeModel
+        from .splitters import SoftObliqueSplitter
+
+        cfg, d = self.config, g.d
+        begin = offsets[idx]
+        end = offsets[np.asarray(idx) + 1]
+        counts = np.minimum(end - begin, cfg.split_rows)
+        Xs, ys, _, dst = gather_strided(g.X, g.y, None, begin, end, counts, stream=self.stream)
+        temps = np.array([g.entries[i][0].req.temperature for i in idx])
+        splitter = SoftObliqueSplitter(AffineRidgeModel(cfg.split_ridge), float(temps[0]),
+                                       max_iters=cfg.max_iters, min_side_points=d + 2,
+                                       min_side_fraction=cfg.min_side_fraction,
+                                       n_restarts=cfg.n_restarts, random_state=0)
+        R, count = cfg.n_restarts, len(idx)
+        self.stream.synchronize()
+        with RaggedOptimizerWorkspace(splitter, Xs, ys, dst, temperature=temps) as ws:
+            slots = [k for _ in range(R) for k in range(count)]
+            keys = [start_key(g.entries[i][0].req.seed, g.entries[i][1].heap, r)
+                    for r in range(R) for i in idx]
+            theta0 = ws._device_starts(slots, keys=keys).reshape(R, count, d + 1)
+            results = ws.optimize(theta0)
+        pending = []
+        for k, (i, res) in enumerate(zip(idx, results)):
+            job, node = g.entries[i]
+            if isinstance(res, Exception):
+                node.leaf, node.gram = node.leaf_candidate, node.gram_candidate
+                node.leaf["split_error"] = getattr(res, "reason", str(res))
+                node.leaf_candidate = node.gram_candidate = None
+                job.open -= 1
+                self._maybe_finish(job)
+                continue
+            node.split = (np.asarray(res.w, dtype=float), float(res.z),
+                          {"loss": float(res.loss), "sample_rows": int(counts[k])})
+            pending.append((job, node, int(begin[k]), int(end[k]), node.split[0], node.split[1]))
+        g.pending = pending or None
+
+    # ----------------------------------------------------------------- completion
+
+    def _maybe_finish(self, job):
+        if job.open > 0:
+            return
+        from .models import PolynomialRidgeModel
+        from .tree import LeafNode, RegressionTree, SplitNode
+
+        degree = job.degree
+        grams_of = {}
+
+        def build(heap):
+            node = job.nodes[heap]
+            nid = _heap_path(heap)
+            if node.split is not None:
+                w, z, meta = node.split
+                return SplitNode(w, z, build(2 * heap), build(2 * heap + 1), depth=node.depth,
+                                 node_id=nid, metadata=dict(meta))
+            m = PolynomialRidgeModel(degree, ridge=self.config.ridge)
+            m.coef_ = np.array(node.leaf["coef"], dtype=float)
+            leaf = LeafNode(m, depth=node.depth, node_id=nid,
+                            metadata={k: v for k, v in node.leaf.items() if k != "coef"})
+            grams_of[id(leaf)] = (node.gram, node.bad)
+            return leaf
+
+        tree = RegressionTree(build(1), metadata={"n_rows": job.n_rows, "d": job.d,
+                                                 "degree": degree})
+        flat = tree.flatten()
+        torch = self.torch
+        grams = torch.stack([grams_of[id(leaf)][0] for leaf in flat.leaf_nodes])
+        bad = torch.tensor([grams_of[id(leaf)][1] for leaf in flat.leaf_nodes],
+                           dtype=torch.int32, device=self.device)
+        coef = np.stack([leaf.model.coef_ for leaf in flat.leaf_nodes])
+        status = np.array([leaf.metadata["status"] for leaf in flat.leaf_nodes])
+        _, p = self._basis(job.d, degree)
+        nbytes = 8.0 * (grams.numel() + flat.coef.numel() + flat.w.numel() * 2 + len(status))
+        res = _Resident(job.tree_id, tree, flat, grams, bad, job.d, degree, p, nbytes,
+                        time.perf_counter())
+        with self._cv:
+            old = self._resident.pop(job.tree_id, None)
+            if old is not None:
+                self._resident_bytes -= old.bytes
+            self._resident[job.tree_id] = res
+            self._resident_bytes += nbytes
+            self._inflight_bytes -= job.est_bytes
+            self._active -= 1
+            self._fitting.discard(job.tree_id)
+        job.nodes.clear()
+        job.timing.finished = time.perf_counter()
+        self.completed += 1
+        job.future.set_result(FitResult(job.tree_id, tree, coef, status, job.n_rows, job.timing))
+
+    # ----------------------------------------------------------------- refits
+
+    def _run_refits(self):
+        """Every admitted refit in one grams launch per basis and one solve per
+        basis: route, sort by leaf, add the new systems, re-solve."""
+        from .leaf_ridge import LeafRidgeBatch, solve_grams
+
+        torch, cfg = self.torch, self.config
+        by_basis = collections.defaultdict(list)
+        for r in self._refits:
+            res = self._resident.get(r.req.tree_id)
+            if res is None:
+                r.future.set_exception(KeyError(f"No resident tree {r.req.tree_id!r}."))
+                continue
+            by_basis[(res.d, res.degree)].append((r, res))
+        self._refits = []
+        for (d, degree), items in by_basis.items():
+            Xs, ys, ws, offs = [], [], [], [0]
+            for r, res in items:
+                L = res.flat.n_leaves
+                ids = res.flat.leaves(r.X).to(torch.int64)
+                order = torch.sort(ids, stable=True).indices
+                counts = torch.bincount(ids, minlength=L).cpu().numpy()
+                Xs.append(r.X.index_select(0, order))
+                ys.append(r.y.index_select(0, order))
+                ws.append(torch.ones_like(r.y) if r.w is None else r.w.index_select(0, order))
+                offs.extend((offs[-1] + np.cumsum(counts)).tolist())
+            offsets = np.asarray(offs, dtype=np.int64)
+            X, y, w = torch.cat(Xs), torch.cat(ys), torch.cat(ws)
+            with LeafRidgeBatch(offsets, d, degree=degree, stream=self.stream) as lb:
+                new, bad = lb.grams(X, y, w, cfg.weight_floor)
+                start = 0
+                for r, res in items:
+                    L = res.flat.n_leaves
+                    res.grams += new[start:start + L]
+                    res.bad |= bad[start:start + L]
+                    start += L
+                allg = torch.cat([res.grams for _, res in items])
+                allb = torch.cat([res.bad for _, res in items])
+                coef, status, _ = solve_grams(allg, allb, d, cfg.ridge, degree=degree,
+                                              stream=self.stream)
+                self.stream.synchronize()
+            coef_h, status_h = coef.cpu().numpy(), status.cpu().numpy()
+            start = 0
+            for r, res in items:
+                L = res.flat.n_leaves
+                c, s = coef_h[start:start + L], status_h[start:start + L]
+                res.flat.coef.copy_(coef[start:start + L])
+                for leaf, row, st in zip(res.flat.leaf_nodes, c, s):
+                    leaf.model.coef_ = row.copy()
+                    leaf.metadata["status"] = int(st)
+                    leaf.metadata["n_rows"] = leaf.metadata.get("n_rows", 0)
+                start += L
+                res.last_used = time.perf_counter()
+                with self._cv:
+                    self._resident.move_to_end(res.tree_id)
+                r.timing.finished = time.perf_counter()
+                self.completed += 1
+                r.future.set_result(RefitResult(res.tree_id, res.tree, c.copy(), s.copy(),
+                                                int(r.X.shape[0]), r.timing))

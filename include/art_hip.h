@@ -340,6 +340,44 @@ int art_tree_leaf(const double* X, int64_t n, int d, const double* node_w, const
 int art_knn_median(const double* X, const int64_t* offsets, const int64_t* host_offsets, int sets,
                    int d, int mode, double* out, hipStream_t stream);
 
+/* ---- Serving loop: data-driven tree fits (no counterpart in the reference,
+ * whose builder queries an oracle; these move a request's own rows).  ---- */
+
+/* sse[k] = y'Wy - 2 beta'h + beta'G beta of system k of art_leaf_gram's
+ * layout (K systems of p coefficients) for coefficients beta (K, p);
+ * wsum[k] (optional) = G[bias_col][bias_col], the weight sum when bias_col
+ * is the constant column (-1: none, wsum = 0). */
+int art_gram_sse(const double* grams, int K, int p, int bias_col, const double* beta, double* sse,
+                 double* wsum, hipStream_t stream);
+
+/* Row sample of K row ranges: output row i of range k (rows
+ * [dst_offsets[k], dst_offsets[k+1])) copies input row
+ * src_begin[k] + floor((i - dst_offsets[k]) n_k / m_k), where
+ * n_k = src_end[k] - src_begin[k] and m_k <= n_k is the output range size.
+ * Offsets are device arrays; total = dst_offsets[K].  w and wo are both NULL
+ * or both set. */
+int art_gather_strided(const double* X, const double* y, const double* w, int d, int K,
+                       const int64_t* src_begin, const int64_t* src_end,
+                       const int64_t* dst_offsets, int64_t total, double* Xo, double* yo,
+                       double* wo, hipStream_t stream);
+
+/* Stable partition of K row ranges [src_begin[k], src_end[k]) of
+ * (X, y, w) by hyperplanes: a row goes right when node_w[k].x - node_z[k] >= 0
+ * (art_tree_leaf's rule).  Range k's left rows land at dst_base[k] + 0.. and
+ * its right rows after them, each side in input order; node_right[k] gets
+ * the right count.  Ranges are cut into blocks of art_partition_block_rows()
+ * rows: blk_offsets (K+1) gives range k blocks blk_offsets[k] ..
+ * blk_offsets[k+1] (ceil(n_k / rows) of them), blocks = blk_offsets[K].
+ * flags (one byte per input row) and blk_right (blocks ints) are scratch.
+ * All arrays are device arrays; outputs must not overlap inputs. */
+int art_partition_rows(const double* X, const double* y, const double* w, int d, int K,
+                       const int64_t* src_begin, const int64_t* src_end,
+                       const int64_t* blk_offsets, int64_t blocks,
+                       const int64_t* dst_base, const double* node_w, const double* node_z,
+                       unsigned char* flags, int* blk_right, int64_t* node_right, double* Xo,
+                       double* yo, double* wo, hipStream_t stream);
+int art_partition_block_rows(void);
+
 #ifdef __cplusplus
 }
 #endif
