@@ -342,6 +342,44 @@ class BatchedOptimizerWorkspace:
                                               int(ids[i]))
         return results
 
+    def hyperplanes(self, theta0, *, timing=None):
+        """Lean ``optimize`` for callers that need only each node's split.
+
+        theta0 (R, k, d+1): starts of every restart of the k loaded nodes.
+        Runs the same device launch (hard refit, no min-side fallback unless
+        the splitter asks for it) and keeps ``optimize``'s choice of restart,
+        but skips the parent-loss launch and the per-node result objects.
+        Returns host arrays theta (k, d+1), ok (k,) and reason (k,) -- the
+        stop reason code of a node without a split, else -1.
+        """
+        self._check_open()
+        starts = np.asarray(theta0, dtype=float)
+        R, count = starts.shape[0], self.k
+        if starts.shape != (R, count, self.d + 1) or not np.isfinite(starts).all():
+            raise ValueError("theta0 must be finite with shape (R, k, d+1).")
+        nodes = np.tile(self.all_ids[:count], R)
+        thetas = np.stack([project_unit_w(t) for t in starts.reshape(-1, self.d + 1)])
+        theta, _, hard, info, _, _ = self._launch(nodes, thetas, 0, self.s.min_side_fallback,
+                                                  True, timing=timing)
+        out = np.zeros((count, self.d + 1))
+        ok = np.zeros(count, dtype=bool)
+        reason = np.full(count, -1, dtype=np.int64)
+        best = [None] * count
+        for q in range(R * count):
+            i = q % count
+            if info[q, 6]:
+                if best[i] is None:
+                    reason[i] = int(info[q, 0])
+                continue
+            fb = bool(hard[q, 7])
+            b = best[i]
+            if b is None or (b[1] and not fb) or (b[1] == fb and hard[q, 0] < b[2]):
+                best[i] = (q, fb, hard[q, 0])
+        for i, b in enumerate(best):
+            if b is not None:
+                out[i], ok[i], reason[i] = theta[b[0]], True, -1
+        return out, ok, reason
+
     def anneal(self, theta0=None, *, nodes=None, t0=0.2, t1=0.01, iters=200, step=0.05,
                timing=None, batched=None):
         """Annealed fixed-step projected gradient from every start, then a hard refit.

@@ -37,7 +37,7 @@ import collections
 import itertools
 import threading
 import time
-from concurrent.futures import Future
+from concurrent.futures import Future, ThreadPoolExecutor
 from dataclasses import dataclass, field
 from typing import Any, Optional
 
@@ -234,7 +234,7 @@ class _Node:
     n_rows: int
     split: Optional[tuple] = None      # (w, z, metadata)
     leaf: Optional[dict] = None        # coef, status, mse, n_rows
-    gram: Any = None                   # device (G,) resident system of a leaf
+    gram: Any = None                   # (level systems tensor, row) of a leaf
     bad: int = 0
     leaf_candidate: Optional[dict] = None   # the node's own fit while its split is open
     gram_candidate: Any = None
@@ -291,6 +291,8 @@ class _Group:
         self.sizes = []
         self.pending = None      # splits to apply: [(job, node, begin, end, w, z)]
         self.new_jobs = []
+        self.offsets = None      # row offsets of entries in the level buffer
+        self.stream = None       # the group's split-search stream
 
 
 class ServingEngine:
@@ -316,8 +318,10 @@ class ServingEngine:
         self._error = None
         self.steps = 0
         self.completed = 0
+        self.phase_seconds = collections.defaultdict(float)
         self._thread = None
         self.stream = None
+        self._pool = ThreadPoolExecutor(max_workers=8, thread_name_prefix="art-search")
         if start:
             self.start()
 
@@ -367,6 +371,8 @@ class ServingEngine:
         if wait and self._thread is not None:
             self._thread.join()
         self._thread = None
+        if wait:
+            self._pool.shutdown(wait=True)
 
     def __enter__(self):
         return self
@@ -471,12 +477,16 @@ class ServingEngine:
         cfg = self.config
         keep = collections.deque()
         refit_trees = {r.req.tree_id for r in self._refits}
+        named = set()     # trees of fits still waiting ahead of a refit
         while self._waiting:
             req, fut, timing = self._waiting.popleft()
             try:
                 if isinstance(req, RefitRequest):
-                    if req.tree_id in refit_trees or req.tree_id in self._fitting:
-                        keep.append((req, fut, timing))   # one refit per tree per level
+                    if req.tree_id in refit_trees or req.tree_id in self._fitting \
+                            or req.tree_id in named:
+                        # One refit per tree per level, after the tree's fit.
+                        keep.append((req, fut, timing))
+                        refit_trees.add(req.tree_id)
                         continue
                     res = self._resident.get(req.tree_id)
                     if res is None:
@@ -498,8 +508,10 @@ class ServingEngine:
                 if est > cfg.hbm_budget_bytes:
                     raise AdmissionError(f"Request needs ~{est / 1e9:.1f} GB, over the "
                                          f"{cfg.hbm_budget_bytes / 1e9:.1f} GB budget.")
-                if self._active >= cfg.max_active or not self._reserve(est):
+                if self._active >= cfg.max_active or not self._reserve(est, keep):
                     keep.append((req, fut, timing))
+                    if req.tree_id is not None:
+                        named.add(req.tree_id)
                     continue
                 tree_id = req.tree_id if req.tree_id is not None else f"tree-{next(self._names)}"
                 job = _FitJob(req, fut, timing, tree_id, d, degree, n, est)
@@ -516,14 +528,15 @@ class ServingEngine:
                 fut.set_exception(exc)
         self._waiting = keep
 
-    def _reserve(self, est):
+    def _reserve(self, est, keep=()):
         """True when est more bytes fit the budget, evicting least recently
-        used resident trees that no admitted refit needs."""
+        used resident trees that no admitted or waiting refit needs."""
         budget = self.config.hbm_budget_bytes
         if self._inflight_bytes + self._resident_bytes + est <= budget:
             return True
         busy = {r.req.tree_id for r in self._refits}
-        busy |= {req.tree_id for req, _, _ in self._waiting if isinstance(req, RefitRequest)}
+        busy |= {req.tree_id for req, _, _ in itertools.chain(self._waiting, keep)
+                 if isinstance(req, RefitRequest)}
         with self._cv:
             for tid in list(self._resident):
                 if self._inflight_bytes + self._resident_bytes + est <= budget:
@@ -536,17 +549,45 @@ class ServingEngine:
 
     # ----------------------------------------------------------------- one level
 
+    def _clock(self, phase, t0):
+        t = time.perf_counter()
+        self.phase_seconds[phase] += t - t0
+        return t
+
     def _step(self):
+        """One level of every basis group; refits first (they finish in one level).
+
+        Node fits run group by group on the engine stream; the split searches
+        of all groups then run at once, each group on its own stream from its
+        own host thread (the kernels of different bases overlap on the device).
+        """
         self.steps += 1
+        if self._refits:
+            t = time.perf_counter()
+            self._run_refits()
+            self._clock("refit", t)
+        searches = []
         for key in list(self._groups):
             g = self._groups[key]
+            t = time.perf_counter()
             self._build_level(g)
+            t = self._clock("partition", t)
             if g.entries:
-                self._level(g)
+                idx = self._level(g)
+                if idx:
+                    searches.append((g, idx))
             if not g.entries and not g.new_jobs and g.pending is None:
                 del self._groups[key]
-        if self._refits:
-            self._run_refits()
+        if searches:
+            t = time.perf_counter()
+            if len(searches) == 1:
+                found = [self._search(*searches[0])]
+            else:
+                found = list(self._pool.map(lambda a: self._search(*a), searches))
+            t = self._clock("split", t)
+            for (g, idx), f in zip(searches, found):
+                self._apply_splits(g, idx, *f)
+            self._clock("decide", t)
 
     def _build_level(self, g):
         """Next level buffer: children of last level's splits, then new roots."""
@@ -614,7 +655,9 @@ class ServingEngine:
 
         torch, cfg, d = self.torch, self.config, g.d
         degree, p = self._basis(d, g.degree)
+        t = time.perf_counter()
         offsets = np.concatenate([[0], np.cumsum(g.sizes)]).astype(np.int64)
+        g.offsets = offsets
         with LeafRidgeBatch(offsets, d, degree=degree, stream=self.stream) as lb:
             grams, bad = lb.grams(g.X, g.y, g.w, cfg.weight_floor)
             coef, status, _ = solve_grams(grams, bad, d, cfg.ridge, degree=degree,
@@ -623,6 +666,7 @@ class ServingEngine:
             self.stream.synchronize()
         coef_h, status_h = coef.cpu().numpy(), status.cpu().numpy()
         sse_h, wsum_h, bad_h = sse.cpu().numpy(), wsum.cpu().numpy(), bad.cpu().numpy()
+        t = self._clock("node_fits", t)
         split_idx = []
         for i, (job, node) in enumerate(g.entries):
             if job is None:
@@ -634,51 +678,63 @@ class ServingEngine:
             stop = (node.depth >= req.max_depth or node.n_rows < 2 * req.min_leaf
                     or node.n_rows < 2 * (d + 2) or mse <= req.error_threshold)
             if stop:
-                node.leaf, node.gram, node.bad = leaf, grams[i].clone(), int(bad_h[i])
+                node.leaf, node.gram, node.bad = leaf, (grams, i), int(bad_h[i])
                 job.open -= 1
                 self._maybe_finish(job)
             else:
-                node.leaf_candidate, node.gram_candidate = leaf, grams[i].clone()
+                node.leaf_candidate, node.gram_candidate = leaf, (grams, i)
                 node.bad = int(bad_h[i])
                 split_idx.append(i)
-        if split_idx:
-            self._split(g, offsets, split_idx)
+        self._clock("decide", t)
+        return split_idx
 
-    def _split(self, g, offsets, idx):
+    def _search(self, g, idx):
+        """Split search of nodes idx of group g on the group's own stream:
+        strided row samples, keyed random starts, the optimizer launch."""
         from .batched_optimizer import RaggedOptimizerWorkspace
         from .models import AffineRidgeModel
         from .splitters import SoftObliqueSplitter
 
-        cfg, d = self.config, g.d
+        torch, cfg, d = self.torch, self.config, g.d
+        torch.cuda.set_device(self.device)
+        if g.stream is None:
+            g.stream = torch.cuda.Stream(device=self.device)
+        g.stream.wait_stream(self.stream)
+        offsets = g.offsets
         begin = offsets[idx]
         end = offsets[np.asarray(idx) + 1]
         counts = np.minimum(end - begin, cfg.split_rows)
-        Xs, ys, _, dst = gather_strided(g.X, g.y, None, begin, end, counts, stream=self.stream)
         temps = np.array([g.entries[i][0].req.temperature for i in idx])
         splitter = SoftObliqueSplitter(AffineRidgeModel(cfg.split_ridge), float(temps[0]),
                                        max_iters=cfg.max_iters, min_side_points=d + 2,
                                        min_side_fraction=cfg.min_side_fraction,
                                        n_restarts=cfg.n_restarts, random_state=0)
         R, count = cfg.n_restarts, len(idx)
-        self.stream.synchronize()
-        with RaggedOptimizerWorkspace(splitter, Xs, ys, dst, temperature=temps) as ws:
-            slots = [k for _ in range(R) for k in range(count)]
-            keys = [start_key(g.entries[i][0].req.seed, g.entries[i][1].heap, r)
-                    for r in range(R) for i in idx]
-            theta0 = ws._device_starts(slots, keys=keys).reshape(R, count, d + 1)
-            results = ws.optimize(theta0)
+        keys = [start_key(g.entries[i][0].req.seed, g.entries[i][1].heap, r)
+                for r in range(R) for i in idx]
+        with torch.cuda.stream(g.stream):
+            Xs, ys, _, dst = gather_strided(g.X, g.y, None, begin, end, counts, stream=g.stream)
+            with RaggedOptimizerWorkspace(splitter, Xs, ys, dst, temperature=temps) as ws:
+                slots = [k for _ in range(R) for k in range(count)]
+                theta0 = ws._device_starts(slots, keys=keys).reshape(R, count, d + 1)
+                theta, ok, reason = ws.hyperplanes(theta0)
+        return begin, end, counts, theta, ok, reason
+
+    def _apply_splits(self, g, idx, begin, end, counts, theta, ok, reason):
+        from .batched_optimizer import STOP_REASONS
+
         pending = []
-        for k, (i, res) in enumerate(zip(idx, results)):
+        for k, i in enumerate(idx):
             job, node = g.entries[i]
-            if isinstance(res, Exception):
+            if not ok[k]:
                 node.leaf, node.gram = node.leaf_candidate, node.gram_candidate
-                node.leaf["split_error"] = getattr(res, "reason", str(res))
+                node.leaf["split_error"] = STOP_REASONS[int(reason[k])]
                 node.leaf_candidate = node.gram_candidate = None
                 job.open -= 1
                 self._maybe_finish(job)
                 continue
-            node.split = (np.asarray(res.w, dtype=float), float(res.z),
-                          {"loss": float(res.loss), "sample_rows": int(counts[k])})
+            node.split = (theta[k, :-1].copy(), float(theta[k, -1]),
+                          {"sample_rows": int(counts[k])})
             pending.append((job, node, int(begin[k]), int(end[k]), node.split[0], node.split[1]))
         g.pending = pending or None
 
@@ -711,7 +767,17 @@ class ServingEngine:
                                                  "degree": degree})
         flat = tree.flatten()
         torch = self.torch
-        grams = torch.stack([grams_of[id(leaf)][0] for leaf in flat.leaf_nodes])
+        # Gather each leaf's system from the level tensor that holds it.
+        refs = [grams_of[id(leaf)][0] for leaf in flat.leaf_nodes]
+        grams = torch.empty((len(refs), refs[0][0].shape[1]), dtype=torch.float64,
+                            device=self.device)
+        by_level = collections.defaultdict(list)
+        for j, (t, i) in enumerate(refs):
+            by_level[id(t)].append((j, i, t))
+        for lst in by_level.values():
+            dst = torch.tensor([j for j, _, _ in lst], device=self.device)
+            src = torch.tensor([i for _, i, _ in lst], device=self.device)
+            grams.index_copy_(0, dst, lst[0][2].index_select(0, src))
         bad = torch.tensor([grams_of[id(leaf)][1] for leaf in flat.leaf_nodes],
                            dtype=torch.int32, device=self.device)
         coef = np.stack([leaf.model.coef_ for leaf in flat.leaf_nodes])

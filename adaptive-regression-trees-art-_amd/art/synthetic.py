@@ -209,3 +209,59 @@ def plain_fit_builder(seed=0, node_batch_size=32, error_threshold=1e-2, **overri
         node_batch_size=node_batch_size, random_state=seed)
     opts.update(overrides)
     return RegressionTreeBuilder(**opts)
+
+
+@dataclass
+class ServingEvent:
+    """One request of the serving stream: a fit of a new tree or a refit
+    adding rows to an earlier fit's tree.  ``rows`` and ``seed`` fix its data."""
+
+    arrival: float
+    kind: str            # "fit" or "refit"
+    d: int
+    rows: int
+    seed: int
+    target: BumpMixture
+    tree: int            # index of the fit this event creates or refits
+
+
+def serving_stream(n_requests, *, rate=200.0, dims=(2, 4, 8, 12, 16, 20), min_rows=1000,
+                   max_rows=2_000_000, refit_fraction=0.3, refit_rows=(0.01, 0.10),
+                   bumps=(4, 16), seed=5):
+    """Poisson arrivals at ``rate``/s; fits draw d uniformly from ``dims`` and
+    N log-uniformly from [min_rows, max_rows]; with probability
+    ``refit_fraction`` (once a fit exists) an event instead adds a uniform
+    1-10% of an earlier fit's N as new rows of the same target."""
+    rng = np.random.default_rng(seed)
+    t, events, fits = 0.0, [], []
+    for _ in range(n_requests):
+        t += rng.exponential(1.0 / rate)
+        if fits and rng.uniform() < refit_fraction:
+            f = events[fits[rng.integers(len(fits))]]
+            k = max(1, int(np.ceil(rng.uniform(*refit_rows) * f.rows)))
+            events.append(ServingEvent(t, "refit", f.d, k, int(rng.integers(2**31)), f.target,
+                                       f.tree))
+            continue
+        d = int(rng.choice(dims))
+        n = int(np.exp(rng.uniform(np.log(min_rows), np.log(max_rows))))
+        target = BumpMixture.random(rng, int(rng.integers(bumps[0], bumps[1] + 1)), d)
+        events.append(ServingEvent(t, "fit", d, n, int(rng.integers(2**31)), target, len(fits)))
+        fits.append(len(events) - 1)
+    return events
+
+
+def serving_rows(event, device):
+    """The event's rows on the device: X ~ U[-1,1]^d from its seed, y its
+    bump mixture (float64 tensors)."""
+    import torch
+
+    g = torch.Generator(device=device)
+    g.manual_seed(event.seed)
+    X = torch.rand((event.rows, event.d), generator=g, dtype=torch.float64, device=device)
+    X.mul_(2.0).sub_(1.0)
+    y = torch.zeros(event.rows, dtype=torch.float64, device=device)
+    m = event.target
+    for c, s, a in zip(m.centers, m.widths, m.amplitudes):
+        cd = torch.as_tensor(c, dtype=torch.float64, device=device)
+        y.add_(torch.exp(-((X - cd) ** 2).sum(dim=1) / (2.0 * s * s)), alpha=float(a))
+    return X, y

@@ -38,11 +38,15 @@ def parse():
     ap.add_argument("--leaves", type=int, default=4096)
     ap.add_argument("--chunk-rows", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--only", choices=("all", "ridge", "sampler", "split", "tree"), default="all",
+    ap.add_argument("--only", choices=("all", "ridge", "sampler", "split", "tree", "serving"),
+                    default="all",
                     help="run one leg (profiling); the JSON line keeps the ridge metric")
     ap.add_argument("--sampler-steps", type=int, default=3)
     ap.add_argument("--tree-steps", type=int, default=3)
     ap.add_argument("--polytopes", type=int, default=65536)
+    ap.add_argument("--requests", type=int, default=100,
+                    help="requests of the serving stream (its first requests, in order)")
+    ap.add_argument("--rate", type=float, default=200.0, help="serving arrivals per second")
     ap.add_argument("--split-nodes", type=int, default=2048,
                     help="nodes of the annealed split leg (64 restarts each)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -434,6 +438,114 @@ def bench_tree(args):
     return result
 
 
+def bench_serving(args):
+    """Serving leg: the first ``--requests`` requests of the seeded stream
+    (Poisson arrivals at ``--rate``/s; d in {2,4,8,12,16,20}; N log-uniform in
+    [1e3, 2e6]; 4-16 Gaussian bumps; 30% refits adding 1-10% new rows to an
+    earlier tree; quadratic leaves for d <= 12, linear above; depth <= 10,
+    >= 64 rows to split a node's half; 200 GB HBM budget).  Rows are generated
+    on the device before timing.  Leg 1 serves the stream as it arrives
+    (continuous batching); leg 2 serves the same requests one at a time, and
+    its latencies are those of a FIFO single server fed the same arrivals."""
+    import torch
+
+    from art.serving import FitRequest, RefitRequest, ServingConfig, ServingEngine
+    from art.synthetic import serving_rows, serving_stream
+
+    events = serving_stream(args.requests, rate=args.rate)
+    dev = torch.device("cuda", 0)
+    data = [serving_rows(e, dev) for e in events]
+    torch.cuda.synchronize()
+    rows = sum(e.rows for e in events)
+
+    def request(i, tag):
+        e = events[i]
+        X, y = data[i]
+        if e.kind == "fit":
+            return FitRequest(X, y, max_depth=10, min_leaf=64, seed=e.seed,
+                              tree_id=f"{tag}{e.tree}")
+        return RefitRequest(f"{tag}{e.tree}", X, y)
+
+    def warm(eng):
+        g = torch.Generator(device=dev)
+        g.manual_seed(0)
+        for d in (2, 4, 8, 12, 16, 20):
+            X = torch.rand((3000, d), generator=g, dtype=torch.float64, device=dev)
+            eng.fit(FitRequest(X, X[:, 0].contiguous(), max_depth=2, tree_id=f"warm{d}"))
+            eng.evict(f"warm{d}")
+
+    cfg = ServingConfig(hbm_budget_bytes=200e9)
+    with ServingEngine(cfg) as eng:
+        warm(eng)
+        t0 = time.perf_counter()
+        futs = []
+        for i, e in enumerate(events):
+            delay = t0 + e.arrival - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            futs.append(eng.submit(request(i, "b")))
+        res = []
+        for i, f in enumerate(futs):
+            res.append(f.result())
+            if (i + 1) % 20 == 0:
+                print(f"[bench] serving: {i + 1} results at {time.perf_counter() - t0:.1f} s",
+                      file=sys.stderr, flush=True)
+        makespan = time.perf_counter() - t0
+        steps = eng.steps
+        phases = {k: round(v, 4) for k, v in eng.phase_seconds.items()}
+    lat = np.array([r.timing.latency for r in res])
+    leaves = [len(r.coefficients) for r, e in zip(res, events) if e.kind == "fit"]
+    bad = sum(int((r.status != 0).sum()) for r in res)
+    print(f"[bench] serving {len(events)} requests in {makespan:.2f} s, p50 "
+          f"{np.percentile(lat, 50) * 1e3:.0f} ms p99 {np.percentile(lat, 99) * 1e3:.0f} ms",
+          file=sys.stderr, flush=True)
+    one = ServingConfig(hbm_budget_bytes=200e9, max_active=1)
+    service = []
+    with ServingEngine(one) as eng:
+        warm(eng)
+        for i in range(len(events)):
+            t = time.perf_counter()
+            eng.submit(request(i, "s")).result()
+            service.append(time.perf_counter() - t)
+            if (i + 1) % 20 == 0:
+                print(f"[bench] one at a time: {i + 1} done, {sum(service):.1f} s",
+                      file=sys.stderr, flush=True)
+    finish, lat1 = 0.0, []
+    for e, sv in zip(events, service):
+        finish = max(finish, e.arrival) + sv
+        lat1.append(finish - e.arrival)
+    lat1 = np.array(lat1)
+    print(f"[bench] one at a time: {sum(service):.2f} s of service, p50 "
+          f"{np.percentile(lat1, 50) * 1e3:.0f} ms p99 {np.percentile(lat1, 99) * 1e3:.0f} ms",
+          file=sys.stderr, flush=True)
+    del data
+    torch.cuda.empty_cache()
+    return {
+        "metric": "served requests/s and p50/p99 request latency",
+        "value": len(events) / makespan, "unit": "requests/s",
+        "latency_ms": {"p50": float(np.percentile(lat, 50)) * 1e3,
+                       "p99": float(np.percentile(lat, 99)) * 1e3,
+                       "max": float(lat.max()) * 1e3},
+        "makespan_s": makespan, "engine_levels": steps, "engine_phase_s": phases,
+        "requests": len(events), "fits": len(leaves),
+        "refits": sum(e.kind == "refit" for e in events), "rows": rows,
+        "mean_leaves_per_tree": float(np.mean(leaves)) if leaves else 0.0,
+        "failed_leaf_solves": bad,
+        "dtype": "f64",
+        "one_at_a_time": {"value": len(events) / sum(service), "unit": "requests/s",
+                          "service_ms": {"p50": float(np.percentile(service, 50)) * 1e3,
+                                         "p99": float(np.percentile(service, 99)) * 1e3},
+                          "latency_ms": {"p50": float(np.percentile(lat1, 50)) * 1e3,
+                                         "p99": float(np.percentile(lat1, 99)) * 1e3}},
+        "speedup_vs_one_at_a_time": sum(service) / makespan,
+        "config": {"workload": (f"serving stream, first {len(events)} requests of seed 5: "
+                                f"Poisson {args.rate:g}/s, d in {{2,4,8,12,16,20}}, N log-uniform "
+                                "[1e3, 2e6], 4-16 bumps, 30% refits of 1-10% new rows, quadratic "
+                                "leaves d <= 12 else linear, depth <= 10, min_leaf 64, split "
+                                "sample 4096 rows, 30 iterations, HBM budget 200 GB")},
+    }
+
+
 def rel_err(a, b):
     return np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-300)
 
@@ -460,6 +572,9 @@ def main():
         return
     if args.only == "tree":
         print(json.dumps(bench_tree(args)))
+        return
+    if args.only == "serving":
+        print(json.dumps(bench_serving(args)))
         return
     wl = leaf_ridge_workload(n_leaves=args.leaves, seed=2)
     dev = torch.device("cuda", 0)
@@ -558,6 +673,7 @@ def main():
     if args.only == "all":
         result["hit_and_run"] = bench_sampler(args)
         result["tree_builds"] = bench_tree(args)
+        result["serving"] = bench_serving(args)
     print(json.dumps(result))
 
 
