@@ -177,6 +177,124 @@ __device__ inline double rows_solve(const double* M, double* bc, int p, int i, d
   return x;
 }
 
+// ---------------------------------------------------------------------------
+// Wave-resident SPD systems (p <= PMAX <= 64): lane i holds row i of the lower
+// triangle in registers, a[j] = A[i][j] for j <= i, and zeros above the
+// diagonal and in rows i >= p.  Every loop is unrolled over PMAX, so the row
+// stays in registers; values move between lanes by v_readlane (a pivot or a
+// solved unknown) or a 64-double LDS column, and no barrier is needed.  All
+// 64 lanes must call these together.
+//
+// No lane masks its updates: the factorization updates the entries above the
+// diagonal too (they never feed the lower triangle) and zeroes them at the
+// end, and the substitutions then include them as zeros.  Masks per column
+// would be common subexpressions across the whole factorization and would
+// hold 2p scalar registers at once.
+
+// Lane `src`'s value of v (src uniform).
+__device__ inline double lane_value(double v, int src) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, src);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
+                                          (unsigned)lo));
+}
+
+// A copy of the lane index the compiler cannot relate to other copies, so
+// that `lane == k` tests of different phases are not merged (and kept alive)
+// across them.  A register move only.
+__device__ inline int opaque_lane(int lane) {
+  int r;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "v"(lane));
+  return r;
+}
+
+// Right-looking Cholesky, column by column: the pivot comes from lane k, each
+// lane scales its own entry of column k and stores it in `col` (64 doubles of
+// LDS), and every lane then reads the column back (broadcast reads) for its
+// trailing update a[j] -= l_ik l_jk.  Sets info to k + 1 for the first pivot
+// <= 0 (uniform), as rows_cholesky does, and diag to the lane's own l_ii.
+// Column K is its own instantiation so that every loop has a constant trip
+// count and unrolls, keeping a[] in registers (one loop nest over all columns
+// exceeds the unroller's budget).  Columns j >= p are updated too, with
+// zeros: a branch per column would cost more than the extra multiply-adds.
+template <int PMAX, int K>
+__device__ __attribute__((always_inline)) inline void wave_cholesky_col(double (&a)[PMAX], int p,
+                                                                        int me, int& info,
+                                                                        double& diag,
+                                                                        double* col) {
+  if constexpr (K < PMAX) {
+    if (K < p && info == 0) {
+      const double dkk = lane_value(a[K], K);
+      if (!(dkk > 0.0)) {
+        info = K + 1;
+      } else {
+        const double lkk = sqrt(dkk);
+        const double lik = me == K ? lkk : a[K] / lkk;
+        diag = me == K ? lkk : diag;
+        a[K] = lik;
+        // One wave writes and reads col: its LDS operations complete in
+        // order, and the fences keep the compiler from reordering them.
+        col[me] = lik;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll
+        for (int j = K + 1; j < PMAX; ++j) a[j] = fma(-lik, col[j], a[j]);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      }
+    }
+    wave_cholesky_col<PMAX, K + 1>(a, p, me, info, diag, col);
+  }
+}
+
+template <int PMAX>
+__device__ __attribute__((always_inline)) inline int wave_cholesky(double (&a)[PMAX], int p,
+                                                                   int lane, double& diag,
+                                                                   double* col) {
+  int info = 0;
+  diag = 1.0;
+  wave_cholesky_col<PMAX, 0>(a, p, opaque_lane(lane), info, diag, col);
+  const int me = opaque_lane(lane);
+#pragma unroll
+  for (int j = 0; j < PMAX; ++j) a[j] = j <= me ? a[j] : 0.0;
+  return info;
+}
+
+// Forward substitution L z = b: lane i holds row i of L (zeros above the
+// diagonal and in rows i >= p) and b_i; dinv = 1 / l_ii (1 for i >= p).  z_k is broadcast from lane k and
+// lanes below it subtract l_ik z_k (lanes above hold l_ik = 0).
+template <int PMAX>
+__device__ __attribute__((always_inline)) inline double wave_forward(const double (&row)[PMAX],
+                                                                     double dinv, int p, int lane,
+                                                                     double b) {
+  const int me = opaque_lane(lane);
+  // Steps k >= p change nothing (z_k = 0 and row[k] = 0 there), so they run
+  // unguarded: straight-line code.
+  double z = lane < p ? b : 0.0;
+#pragma unroll
+  for (int k = 0; k < PMAX; ++k) {
+    const double zk = lane_value(z * dinv, k);  // lane k scales, all read it
+    z = me == k ? zk : fma(-row[k], zk, z);
+  }
+  return z;
+}
+
+// Back substitution L^T x = z: lane i holds column i of L below the diagonal,
+// col[k] = l_ki for k > i and 0 otherwise.  x_k is broadcast from lane k and
+// lanes above it subtract l_ki x_k.
+template <int PMAX>
+__device__ __attribute__((always_inline)) inline double wave_backward(const double (&col)[PMAX],
+                                                                      double dinv, int p, int lane,
+                                                                      double z) {
+  const int me = opaque_lane(lane);
+  double x = z;
+#pragma unroll
+  for (int k = PMAX - 1; k >= 0; --k) {
+    const double xk = lane_value(x * dinv, k);
+    x = me == k ? xk : fma(-col[k], xk, x);
+  }
+  return x;
+}
+
 __device__ inline int block_or(int v, int* scratch) {
   v = wave_or(v);
   if (blockDim.x == 64) return v;

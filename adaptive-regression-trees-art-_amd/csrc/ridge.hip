@@ -604,6 +604,128 @@ __global__ __launch_bounds__(128) void solve_rows_kernel(
   }
 }
 
+// The same contract for p <= PMAX <= 64: four waves reduce the chunk partials
+// into LDS (many loads in flight: the reduction is latency-bound), then wave 0
+// loads its rows into registers and factors and solves them there
+// (wave_cholesky, wave_solve: no barrier per column).
+template <int TT>
+__global__ __launch_bounds__(256) void solve_wave_kernel(
+    const SystemDesc* __restrict__ systems, int nchunks, int T, int npairs, int p, double ridge,
+    const double* __restrict__ partials, const int* __restrict__ flags,
+    const double* __restrict__ tpart, const int* __restrict__ mask, double* __restrict__ factor,
+    double* __restrict__ beta, int* __restrict__ status, double* __restrict__ lambda) {
+  extern __shared__ double M[];  // packed lower (p+1) x (p+1): G and the row h
+  __shared__ double red[4];
+  __shared__ int ired[4];
+  const SystemDesc S = systems ? systems[blockIdx.x]
+                               : SystemDesc{(int)blockIdx.x, (int)blockIdx.x + 1,
+                                            (int)blockIdx.x, 0};
+  if (mask != nullptr && !mask[S.out]) return;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  int bad = 0;
+  for (int ch = S.chunk_begin + tid; ch < S.chunk_end; ch += nt)
+    bad |= flags[(size_t)S.side * nchunks + ch];
+  // Tile (t1, t2) element (a, b) is G[16 t1 + a][16 t2 + b], t1 <= t2.  With
+  // 256 threads, thread tid owns element (tid >> 4, tid & 15) of each of the
+  // NP tile pairs, so the adds need no atomics.  Four chunks' loads are issued
+  // together (latency-bound otherwise); the adds stay in chunk order.  The
+  // tiles cover every packed element of the (p+1) x (p+1) system once.
+  constexpr int PMAX = 16 * TT < 64 ? 16 * TT : 64, NP = TT * (TT + 1) / 2;
+  constexpr int total = NP * 256;
+  double acc[NP];
+#pragma unroll
+  for (int u = 0; u < NP; ++u) acc[u] = 0.0;
+  const double* base = partials + (size_t)S.side * nchunks * total + tid;
+  int ch = S.chunk_begin;
+  for (; ch + 4 <= S.chunk_end; ch += 4) {
+    double v[4][NP];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int u = 0; u < NP; ++u) v[q][u] = base[(size_t)(ch + q) * total + 256 * u];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int u = 0; u < NP; ++u) acc[u] += v[q][u];
+  }
+  for (; ch < S.chunk_end; ++ch) {
+#pragma unroll
+    for (int u = 0; u < NP; ++u) acc[u] += base[(size_t)ch * total + 256 * u];
+  }
+#pragma unroll
+  for (int u = 0; u < NP; ++u) {
+    int t1, t2;
+    pair_tiles(u, TT, t1, t2);
+    const int r = 16 * t1 + (tid >> 4), c = 16 * t2 + (tid & 15);
+    if (r <= c && c <= p) M[tri(c, r)] = acc[u];
+  }
+  double tpiece = 0.0;
+  if (tpart != nullptr)
+    for (int ch = S.chunk_begin + tid; ch < S.chunk_end; ch += nt) tpiece += tpart[ch];
+  bad = block_or(bad, ired);  // its barriers also publish M
+  if (tpart == nullptr && tid < p) tpiece = M[tri(tid, tid)];
+  const double tr = block_sum(tpiece, red);
+  if (tid >= 64) return;
+  const int lane = tid;
+  const double lam = ridge == 0.0 ? 0.0 : ridge * (tr / p);
+  int st = bad ? -1 : ((!isfinite(tr) || !isfinite(lam)) ? -2 : 0);
+  // Each loop below tests lane against every column; a fresh opaque copy of
+  // the lane index per loop keeps those tests from being shared (and held in
+  // scalar registers) across the factorization.
+  double a[PMAX];
+  {
+    const int me = opaque_lane(lane);
+#pragma unroll
+    for (int j = 0; j < PMAX; ++j) a[j] = (j <= me && me < p) ? M[tri(me, j)] : 0.0;
+  }
+  const double h = lane < p ? M[tri(p, lane)] : 0.0;
+  double x = 0.0;
+  if (st == 0) {
+#pragma unroll
+    for (int j = 0; j < PMAX; ++j)
+      if (j == lane) a[j] += lam;
+    double diag;
+    st = wave_cholesky<PMAX>(a, p, lane, diag, M);  // M's rows are in a[] now
+    if (st == 0) {
+      const int me = opaque_lane(lane);
+      int nf = me < p && !isfinite(h);
+#pragma unroll
+      for (int j = 0; j < PMAX; ++j) nf |= (j <= me && me < p && !isfinite(a[j]));
+      if (wave_or(nf)) st = -2;
+    }
+    if (st == 0) {
+      const double dinv = 1.0 / diag;
+      const double z = wave_forward<PMAX>(a, dinv, p, lane, h);
+      // Store the factor rows (the caller's copy, and the LDS copy from which
+      // every lane reads its column for the back substitution).
+      const int me = opaque_lane(lane);
+      double* row = factor != nullptr && me < p
+                        ? factor + (size_t)S.out * (p * (p + 1) / 2) + tri(me, 0)
+                        : nullptr;
+#pragma unroll
+      for (int j = 0; j < PMAX; ++j) {
+        if (j <= me && me < p) {
+          M[tri(me, j)] = a[j];
+          if (row) row[j] = a[j];
+        }
+      }
+      // One wave writes and reads M: LDS operations of a wave complete in
+      // order; the fence keeps the compiler from moving the reads up.
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      const int me2 = opaque_lane(lane);
+#pragma unroll
+      for (int k = 0; k < PMAX; ++k) a[k] = (k > me2 && k < p) ? M[tri(k, me2)] : 0.0;
+      x = wave_backward<PMAX>(a, dinv, p, lane, z);
+      if (wave_or(lane < p && !isfinite(x))) st = -3;
+    }
+  }
+  if (lane < p) beta[(size_t)S.out * p + lane] = st == 0 ? x : __builtin_nan("");
+  if (lane == 0) {
+    status[S.out] = st;
+    lambda[S.out] = lam;
+  }
+}
+
 // Resident systems: leaf j's augmented system is the sum of its chunks'
 // partial tiles (chunks goff[j] .. goff[j+1]); bad[j] ORs their weight flags.
 __global__ __launch_bounds__(256) void reduce_leaf_partials_kernel(
@@ -1069,6 +1191,54 @@ __global__ __launch_bounds__(128) void correct_rows_kernel(
   }
 }
 
+// correct_rows_kernel for p <= PMAX <= 64 on one wave: the stored factor's
+// rows are loaded into registers and the step is solved by wave_solve.
+template <int PMAX>
+__global__ __launch_bounds__(64) void correct_wave_kernel(
+    const SystemDesc* __restrict__ systems, const int* __restrict__ roff, int p, int rstride,
+    const double* __restrict__ rpart,
+    const double* __restrict__ factor, const double* __restrict__ lambda,
+    double* __restrict__ beta, const int* __restrict__ status, int* __restrict__ need,
+    double tau, const int* __restrict__ mask, double* __restrict__ dnorm) {
+  const SystemDesc S = systems[blockIdx.x];
+  const int leaf = S.out, lane = threadIdx.x;
+  if (mask && !mask[leaf]) return;
+  if (status[leaf] != 0) {
+    if (lane == 0) need[leaf] = 1;
+    return;
+  }
+  const double* F = factor + (size_t)leaf * (p * (p + 1) / 2);
+  double L[PMAX];
+#pragma unroll
+  for (int j = 0; j < PMAX; ++j) L[j] = (j <= lane && lane < p) ? F[tri(lane, j)] : 0.0;
+  const double dinv = lane < p ? 1.0 / F[tri(lane, lane)] : 1.0;
+  double r = 0.0, b0 = 0.0;
+  if (lane < p) {
+    for (int ch = roff[leaf]; ch < roff[leaf + 1]; ++ch) r += rpart[(size_t)ch * rstride + lane];
+    b0 = beta[(size_t)leaf * p + lane];
+    r -= lambda[leaf] * b0;
+  }
+  const double z = wave_forward<PMAX>(L, dinv, p, lane, r);
+#pragma unroll
+  for (int k = 0; k < PMAX; ++k) L[k] = (k > lane && k < p) ? F[tri(k, lane)] : 0.0;
+  const double delta = wave_backward<PMAX>(L, dinv, p, lane, z);
+  const double b1 = b0 + delta;
+  const double nd = wave_sum(lane < p ? delta * delta : 0.0);
+  const double nb = wave_sum(lane < p ? b1 * b1 : 0.0);
+  if (lane < p) beta[(size_t)leaf * p + lane] = b1;
+  if (lane == 0) {
+    if (!mask) {
+      need[leaf] = !(nd <= tau * tau * nb);
+    } else {
+      const double prev = dnorm[leaf];
+      const double rho = prev > 0.0 ? sqrt(nd / prev) : 1.0;
+      const double left = rho < 1.0 ? rho * sqrt(nd) / (1.0 - rho) : INFINITY;
+      need[leaf] = !(rho < 0.5 && left <= tau * tau * sqrt(nb));
+    }
+    dnorm[leaf] = nd;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Feature materialization and input checks.
 
@@ -1381,6 +1551,28 @@ int art_plan_info(const art_ridge_plan* P, int* p, int* nchunks, int* tiles) {
   return ART_OK;
 }
 
+// Row-per-lane solves: one wave in registers for p <= 64, the LDS form above.
+static void launch_rows_solve(const SystemDesc* sys, int nsys, int nchunks, int T, int npairs,
+                              int p, double ridge, const double* partials, const int* flags,
+                              const double* tpart, const int* mask, double* factor,
+                              double* beta, int* status, double* lambda, hipStream_t stream) {
+  const size_t lds = (size_t)(p + 1) * (p + 2) / 2 * sizeof(double);
+#define ART_WAVE(TT)                                                                        \
+  solve_wave_kernel<TT><<<dim3(nsys), dim3(256), lds, stream>>>(                            \
+      sys, nchunks, T, npairs, p, ridge, partials, flags, tpart, mask, factor, beta, status, \
+      lambda)
+  if (T == 1) ART_WAVE(1);
+  else if (T == 2) ART_WAVE(2);
+  else if (T == 3) ART_WAVE(3);
+  else if (T == 4) ART_WAVE(4);
+  else if (p <= 64) ART_WAVE(5);
+  else
+    solve_rows_kernel<<<dim3(nsys), dim3(64 * ((p + 63) / 64)), lds, stream>>>(
+        sys, nchunks, T, npairs, p, ridge, partials, flags, tpart, mask, factor, beta, status,
+        lambda);
+#undef ART_WAVE
+}
+
 static int launch_solve(const art_ridge_plan* P, int nsys, int nchunks_stride, double ridge,
                         double* beta, int* status, double* lambda, hipStream_t stream,
                         const int* mask = nullptr, double* factor = nullptr,
@@ -1388,10 +1580,9 @@ static int launch_solve(const art_ridge_plan* P, int nsys, int nchunks_stride, d
   const int P1 = P->p + 1;
   const size_t bytes = (size_t)P1 * (P1 + 1) / 2 * sizeof(double);
   if (P->p <= kRowsMaxP) {
-    const size_t lds = (size_t)(P->p + 1) * (P->p + 2) / 2 * sizeof(double);
-    solve_rows_kernel<<<dim3(nsys), dim3(64 * ((P->p + 63) / 64)), lds, stream>>>(
-        P->d_sys, nchunks_stride, P->T, P->npairs, P->p, ridge, P->d_partials, P->d_flags, tpart,
-        mask, factor, beta, status, lambda);
+    launch_rows_solve(P->d_sys, nsys, nchunks_stride, P->T, P->npairs, P->p, ridge,
+                      P->d_partials, P->d_flags, tpart, mask, factor, beta, status, lambda,
+                      stream);
   } else if (bytes <= ART_SOLVE_LDS_BYTES) {
     solve_f64_kernel<true><<<dim3(nsys), dim3(64), bytes, stream>>>(
         P->d_sys, nchunks_stride, P->T, P->npairs, P->p, ridge, P->d_partials, P->d_flags,
@@ -1497,10 +1688,19 @@ static int leaf_fit_mixed(art_ridge_plan* P, const double* X, const double* y, c
       }
     }
     ART_HIP(hipGetLastError());
-    correct_rows_kernel<<<sgrid, dim3(64 * ((p + 63) / 64)),
-                          (size_t)p * (p + 1) / 2 * sizeof(double), stream>>>(
-        P->d_sys, quad ? P->d_roff : P->d_goff, p, quad ? p : 16 * T, P->d_rpart, P->d_factor,
-        lambda, beta, status, P->d_need, P->refine_tol, mask, P->d_dnorm);
+    {
+      const int* ro = quad ? P->d_roff : P->d_goff;
+      const int rs = quad ? p : 16 * T;
+#define ART_CORRECT(PM)                                                                     \
+  correct_wave_kernel<PM><<<sgrid, dim3(64), 0, stream>>>(                                  \
+      P->d_sys, ro, p, rs, P->d_rpart, P->d_factor, lambda, beta, status, P->d_need,        \
+      P->refine_tol, mask, P->d_dnorm)
+      if (p <= 16) ART_CORRECT(16);
+      else if (p <= 32) ART_CORRECT(32);
+      else if (p <= 48) ART_CORRECT(48);
+      else ART_CORRECT(64);
+#undef ART_CORRECT
+    }
     ART_HIP(hipGetLastError());
   }
   // Exact FP64 path for the flagged leaves only.
@@ -1636,10 +1836,8 @@ int art_gram_solve(const double* grams, const int* bad, int K, art_basis basis, 
     return fail(ART_ERR_ARG, "art_gram_solve: ridge must be finite and >= 0");
   if (K == 0) return ART_OK;
   const int T = tiles_for(p), npairs = T * (T + 1) / 2;
-  const size_t lds = (size_t)(p + 1) * (p + 2) / 2 * sizeof(double);
-  solve_rows_kernel<<<dim3(K), dim3(64 * ((p + 63) / 64)), lds, stream>>>(
-      nullptr, K, T, npairs, p, ridge, grams, bad, nullptr, nullptr, nullptr, beta, status,
-      lambda);
+  launch_rows_solve(nullptr, K, K, T, npairs, p, ridge, grams, bad, nullptr, nullptr, nullptr,
+                    beta, status, lambda, stream);
   ART_HIP(hipGetLastError());
   return ART_OK;
 }
