@@ -614,7 +614,9 @@ __global__ __launch_bounds__(256) void solve_wave_kernel(
     const double* __restrict__ partials, const int* __restrict__ flags,
     const double* __restrict__ tpart, const int* __restrict__ mask, double* __restrict__ factor,
     double* __restrict__ beta, int* __restrict__ status, double* __restrict__ lambda) {
-  extern __shared__ double M[];  // packed lower (p+1) x (p+1): G and the row h
+  // Packed lower (p+1) x (p+1): G and the row h; at least 64 doubles, as the
+  // factorization's column buffer reuses it.
+  extern __shared__ double M[];
   __shared__ double red[4];
   __shared__ int ired[4];
   const SystemDesc S = systems ? systems[blockIdx.x]
@@ -1557,8 +1559,10 @@ static void launch_rows_solve(const SystemDesc* sys, int nsys, int nchunks, int 
                               const double* tpart, const int* mask, double* factor,
                               double* beta, int* status, double* lambda, hipStream_t stream) {
   const size_t lds = (size_t)(p + 1) * (p + 2) / 2 * sizeof(double);
+  // The wave kernel also uses the start of its LDS as a 64-double column.
+  const size_t wlds = std::max(lds, (size_t)64 * sizeof(double));
 #define ART_WAVE(TT)                                                                        \
-  solve_wave_kernel<TT><<<dim3(nsys), dim3(256), lds, stream>>>(                            \
+  solve_wave_kernel<TT><<<dim3(nsys), dim3(256), wlds, stream>>>(                           \
       sys, nchunks, T, npairs, p, ridge, partials, flags, tpart, mask, factor, beta, status, \
       lambda)
   if (T == 1) ART_WAVE(1);
