@@ -102,21 +102,31 @@ class RequestTiming:
 @dataclass
 class FitResult:
     tree_id: str
-    tree: Any
     coefficients: np.ndarray   # (leaves, p) in the flattened tree's leaf order
     status: np.ndarray         # (leaves,) solve status per leaf (0 = ok)
     n_rows: int
     timing: RequestTiming
+    resident: Any = field(default=None, repr=False)
+
+    @property
+    def tree(self):
+        """The fitted ``RegressionTree`` (built on first access; a refit
+        updates its leaf models in place)."""
+        return self.resident.tree
 
 
 @dataclass
 class RefitResult:
     tree_id: str
-    tree: Any
     coefficients: np.ndarray
     status: np.ndarray
     n_new_rows: int
     timing: RequestTiming
+    resident: Any = field(default=None, repr=False)
+
+    @property
+    def tree(self):
+        return self.resident.tree
 
 
 @dataclass
@@ -267,18 +277,64 @@ class _Refit:
     w: Any = None
 
 
-@dataclass
 class _Resident:
-    tree_id: str
-    tree: Any
-    flat: Any
-    grams: Any
-    bad: Any
-    d: int
-    degree: int
-    p: int
-    bytes: float
-    last_used: float
+    """A fitted tree kept on the device: flattened split and leaf tables
+    (breadth-first: heap order), the leaves' FP64 systems for refits, and the
+    host records from which the ``RegressionTree`` is built on demand."""
+
+    def __init__(self, tree_id, flat, grams, bad, d, degree, p, ridge, splits, leaves, n_rows):
+        self.tree_id, self.flat, self.grams, self.bad = tree_id, flat, grams, bad
+        self.d, self.degree, self.p, self.ridge = d, degree, p, ridge
+        self.splits = splits      # [(heap, depth, w, z, metadata)] in heap order
+        self.leaves = leaves      # [(heap, depth, metadata)] in heap order (= flat leaf order)
+        self.coef = None          # host (L, p), set by the owner
+        self.n_rows = n_rows
+        self.bytes = 0.0
+        self.last_used = time.perf_counter()
+        self._tree = None
+        self._lock = threading.Lock()
+
+    @property
+    def tree(self):
+        with self._lock:
+            if self._tree is None:
+                self._tree = self._build()
+            return self._tree
+
+    def _build(self):
+        from .models import PolynomialRidgeModel
+        from .tree import LeafNode, RegressionTree, SplitNode
+
+        split_of = {h: (depth, w, z, meta) for h, depth, w, z, meta in self.splits}
+        leaf_of = {h: (j, depth, meta) for j, (h, depth, meta) in enumerate(self.leaves)}
+        self._models = [None] * len(self.leaves)
+
+        def build(h):
+            if h in split_of:
+                depth, w, z, meta = split_of[h]
+                return SplitNode(w, z, build(2 * h), build(2 * h + 1), depth=depth,
+                                 node_id=_heap_path(h), metadata=dict(meta))
+            j, depth, meta = leaf_of[h]
+            m = PolynomialRidgeModel(self.degree, ridge=self.ridge)
+            m.coef_ = self.coef[j].copy()
+            self._models[j] = m
+            # The leaf shares its metadata dict with the record: refit
+            # statuses show through.
+            return LeafNode(m, depth=depth, node_id=_heap_path(h), metadata=meta)
+
+        return RegressionTree(build(1), metadata={"n_rows": self.n_rows, "d": self.d,
+                                                  "degree": self.degree})
+
+    def set_coefficients(self, coef, status):
+        """Leaf coefficients after a refit (host copies; the device table is
+        updated by the caller)."""
+        with self._lock:
+            self.coef = coef
+            for j, (_, _, meta) in enumerate(self.leaves):
+                meta["status"] = int(status[j])
+            if self._tree is not None:
+                for j, m in enumerate(self._models):
+                    m.coef_ = coef[j].copy()
 
 
 class _Group:
@@ -353,7 +409,8 @@ class ServingEngine:
 
     def tree(self, tree_id):
         with self._cv:
-            return self._resident[tree_id].tree
+            res = self._resident[tree_id]
+        return res.tree
 
     def evict(self, tree_id):
         with self._cv:
@@ -743,32 +800,47 @@ class ServingEngine:
     def _maybe_finish(self, job):
         if job.open > 0:
             return
-        from .models import PolynomialRidgeModel
-        from .tree import LeafNode, RegressionTree, SplitNode
+        t0 = time.perf_counter()
+        self._finish(job)
+        self.phase_seconds["assemble"] += time.perf_counter() - t0
 
-        degree = job.degree
-        grams_of = {}
+    def _finish(self, job):
+        """Flatten a finished fit straight from its node records: splits and
+        leaves in heap order (the breadth-first order of ``FlatTree``), leaf
+        systems gathered from the level tensors that hold them."""
+        from .tree import FlatTree
 
-        def build(heap):
-            node = job.nodes[heap]
-            nid = _heap_path(heap)
-            if node.split is not None:
-                w, z, meta = node.split
-                return SplitNode(w, z, build(2 * heap), build(2 * heap + 1), depth=node.depth,
-                                 node_id=nid, metadata=dict(meta))
-            m = PolynomialRidgeModel(degree, ridge=self.config.ridge)
-            m.coef_ = np.array(node.leaf["coef"], dtype=float)
-            leaf = LeafNode(m, depth=node.depth, node_id=nid,
-                            metadata={k: v for k, v in node.leaf.items() if k != "coef"})
-            grams_of[id(leaf)] = (node.gram, node.bad)
-            return leaf
+        torch, d, degree = self.torch, job.d, job.degree
+        _, p = self._basis(d, degree)
+        heaps = sorted(job.nodes)
+        split_h = [h for h in heaps if job.nodes[h].split is not None]
+        leaf_h = [h for h in heaps if job.nodes[h].split is None]
+        sidx = {h: i for i, h in enumerate(split_h)}
+        lidx = {h: j for j, h in enumerate(leaf_h)}
 
-        tree = RegressionTree(build(1), metadata={"n_rows": job.n_rows, "d": job.d,
-                                                 "degree": degree})
-        flat = tree.flatten()
-        torch = self.torch
+        def link(h):
+            return sidx[h] if h in sidx else -1 - lidx[h]
+
+        S = max(len(split_h), 1)
+        w = np.zeros((S, d))
+        z = np.zeros(S)
+        child = np.zeros((S, 2), dtype=np.int32)
+        splits = []
+        for i, h in enumerate(split_h):
+            node = job.nodes[h]
+            w[i], z[i] = node.split[0], node.split[1]
+            child[i] = link(2 * h), link(2 * h + 1)
+            splits.append((h, node.depth, node.split[0], node.split[1], node.split[2]))
+        leaves, refs, bads = [], [], []
+        coef = np.empty((len(leaf_h), p))
+        for j, h in enumerate(leaf_h):
+            node = job.nodes[h]
+            coef[j] = node.leaf["coef"]
+            leaves.append((h, node.depth, {k: v for k, v in node.leaf.items() if k != "coef"}))
+            refs.append(node.gram)
+            bads.append(node.bad)
+        flat = FlatTree.from_tables(w, z, child, link(1), coef, d, (degree, True, False))
         # Gather each leaf's system from the level tensor that holds it.
-        refs = [grams_of[id(leaf)][0] for leaf in flat.leaf_nodes]
         grams = torch.empty((len(refs), refs[0][0].shape[1]), dtype=torch.float64,
                             device=self.device)
         by_level = collections.defaultdict(list)
@@ -778,27 +850,26 @@ class ServingEngine:
             dst = torch.tensor([j for j, _, _ in lst], device=self.device)
             src = torch.tensor([i for _, i, _ in lst], device=self.device)
             grams.index_copy_(0, dst, lst[0][2].index_select(0, src))
-        bad = torch.tensor([grams_of[id(leaf)][1] for leaf in flat.leaf_nodes],
-                           dtype=torch.int32, device=self.device)
-        coef = np.stack([leaf.model.coef_ for leaf in flat.leaf_nodes])
-        status = np.array([leaf.metadata["status"] for leaf in flat.leaf_nodes])
-        _, p = self._basis(job.d, degree)
-        nbytes = 8.0 * (grams.numel() + flat.coef.numel() + flat.w.numel() * 2 + len(status))
-        res = _Resident(job.tree_id, tree, flat, grams, bad, job.d, degree, p, nbytes,
-                        time.perf_counter())
+        bad = torch.tensor(bads, dtype=torch.int32, device=self.device)
+        status = np.array([meta["status"] for _, _, meta in leaves])
+        res = _Resident(job.tree_id, flat, grams, bad, d, degree, p, self.config.ridge, splits,
+                        leaves, job.n_rows)
+        res.coef = coef
+        res.bytes = 8.0 * (grams.numel() + flat.coef.numel() + flat.w.numel() * 2 + len(status))
         with self._cv:
             old = self._resident.pop(job.tree_id, None)
             if old is not None:
                 self._resident_bytes -= old.bytes
             self._resident[job.tree_id] = res
-            self._resident_bytes += nbytes
+            self._resident_bytes += res.bytes
             self._inflight_bytes -= job.est_bytes
             self._active -= 1
             self._fitting.discard(job.tree_id)
         job.nodes.clear()
         job.timing.finished = time.perf_counter()
         self.completed += 1
-        job.future.set_result(FitResult(job.tree_id, tree, coef, status, job.n_rows, job.timing))
+        job.future.set_result(FitResult(job.tree_id, coef.copy(), status, job.n_rows, job.timing,
+                                        res))
 
     # ----------------------------------------------------------------- refits
 
@@ -848,15 +919,12 @@ class ServingEngine:
                 L = res.flat.n_leaves
                 c, s = coef_h[start:start + L], status_h[start:start + L]
                 res.flat.coef.copy_(coef[start:start + L])
-                for leaf, row, st in zip(res.flat.leaf_nodes, c, s):
-                    leaf.model.coef_ = row.copy()
-                    leaf.metadata["status"] = int(st)
-                    leaf.metadata["n_rows"] = leaf.metadata.get("n_rows", 0)
+                res.set_coefficients(c.copy(), s)
                 start += L
                 res.last_used = time.perf_counter()
                 with self._cv:
                     self._resident.move_to_end(res.tree_id)
                 r.timing.finished = time.perf_counter()
                 self.completed += 1
-                r.future.set_result(RefitResult(res.tree_id, res.tree, c.copy(), s.copy(),
-                                                int(r.X.shape[0]), r.timing))
+                r.future.set_result(RefitResult(res.tree_id, c.copy(), s.copy(),
+                                                int(r.X.shape[0]), r.timing, res))

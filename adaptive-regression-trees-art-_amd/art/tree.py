@@ -95,10 +95,6 @@ class FlatTree:
     """
 
     def __init__(self, root, d, basis):
-        import torch
-
-        from . import _hip
-
         splits, leaves, links = [], [], {}
         queue = [root]
         while queue:
@@ -124,16 +120,38 @@ class FlatTree:
         coef = np.stack([np.asarray(leaf.model.coef_, dtype=float).reshape(-1) for leaf in leaves])
         if coef.shape[1] != p:
             raise ValueError("Leaf models have different coefficient counts.")
+        self._upload(w, z, child, links[id(root)], coef, d, basis, len(splits))
+
+    @classmethod
+    def from_tables(cls, w, z, child, root, coef, d, basis):
+        """Device tables from host arrays already in this layout: w (S, d),
+        z (S,), child links (S, 2) int32, root link, leaf coefficients (L, p)
+        and basis (degree, bias, bias_last); S >= 1 rows (a placeholder row
+        when the tree is one leaf)."""
+        self = cls.__new__(cls)
+        self.leaf_nodes = None
+        n_splits = int(np.asarray(child).shape[0]) if root >= 0 else 0
+        self._upload(np.asarray(w, dtype=float), np.asarray(z, dtype=float),
+                     np.asarray(child, dtype=np.int32), int(root), np.asarray(coef, dtype=float),
+                     int(d), basis, n_splits)
+        return self
+
+    def _upload(self, w, z, child, root, coef, d, basis, n_splits):
+        import torch
+
+        from . import _hip
+
+        degree, bias, bias_last = basis
         self.torch, self.lib = _hip.torch_device(), _hip.load()
         dev = torch.device("cuda", torch.cuda.current_device())
         self.w = torch.from_numpy(np.ascontiguousarray(w)).to(dev)
-        self.z = torch.from_numpy(z).to(dev)
+        self.z = torch.from_numpy(np.ascontiguousarray(z)).to(dev)
         self.child = torch.from_numpy(np.ascontiguousarray(child)).to(dev)
         self.coef = torch.from_numpy(np.ascontiguousarray(coef)).to(dev)
-        self.root = links[id(root)]
+        self.root = root
         self.d, self.device = d, dev
-        self.basis = _hip.Basis(d, p, degree, int(bias), int(bias_last))
-        self.n_splits, self.n_leaves = len(splits), len(leaves)
+        self.basis = _hip.Basis(d, coef.shape[1], degree, int(bias), int(bias_last))
+        self.n_splits, self.n_leaves = n_splits, coef.shape[0]
 
     def leaves(self, X, out=None):
         """Leaf index (breadth-first leaf numbering) of device rows X (n, d)."""

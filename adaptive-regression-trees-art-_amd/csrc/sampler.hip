@@ -349,9 +349,8 @@ constexpr int kChainLdsBytes = 64 * 1024;
 template <int DM, bool EXACT>
 void launch_chain(const art_har_args& a, int max_rows, hipStream_t stream) {
   const int E = chain_entry(DM, max_rows);
+  // At most kChainLdsBytes (64 KB): within the default dynamic LDS limit.
   const size_t bytes = sizeof(double) * (size_t)kChainBlock * E;
-  hipFuncSetAttribute((const void*)har_chain_kernel<DM, EXACT>,
-                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   har_chain_kernel<DM, EXACT><<<a.P, 64, bytes, stream>>>(a, E);
 }
 
