@@ -224,14 +224,15 @@ size_t leaf_stage_smem_bytes(int d) {
 }
 
 // The mixed Gram kernel stages rows through registers (one row per thread)
-// into an FP32 table [x_0..x_{d-1}, 1, y, 0] (stride d + 3 floats) plus FP32
-// weights; it has no FP64 LDS buffers.
+// into an FP32 table of two halves per row, [x_0..x_{d-1}, 1, y, 0] and the
+// same scaled by sqrt(w) (each half d + 3 floats); it has no FP64 LDS
+// buffers.
 constexpr int kMixedRows = 256;
 constexpr int kMixedMaxD = 16;
 constexpr int kMixedFlush = 1;  // 32-row steps per FP32 accumulation
 __host__ __device__ inline int rows32_stride(int d) { return d + 3; }
 size_t mixed_stage_smem_bytes(int d) {
-  const size_t f32 = (size_t)kMixedRows * (rows32_stride(d) + 1) * sizeof(float);
+  const size_t f32 = (size_t)kMixedRows * 2 * rows32_stride(d) * sizeof(float);
   return f32 > 1024 * sizeof(double) ? f32 : 1024 * sizeof(double);
 }
 
@@ -794,7 +795,8 @@ typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 // the exact FP64 trace of the Gram (which fixes lambda) from
 // |phi|^2 = bias + s1 [+ (s1^2 + s2)/2 for the quadratic basis],
 // s1 = sum x_i^2, s2 = sum x_i^4, checks the weight, and writes the row to
-// an FP32 table [x, 1, y, 0] with the square root of its floored weight.
+// an FP32 table: [x, 1, y, 0] and sqrt(w_floored) [x, 1, y, 0], so that a
+// feature value sqrt(w) x_a x_b is one product of a scaled and a plain entry.
 // Wave w then takes the 32-row steps s = w, w+4, ...: lane (col, kq) holds
 // rows 32 s + 4 e + kq (e < 8; the four row groups of one LDS read are
 // adjacent rows) of features 16 t + col as v = sqrt(w) phi, split
@@ -814,9 +816,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DC > 0 ? 3 
   extern __shared__ double smem[];
   __shared__ double tred[4];
   const int d = DC > 0 ? DC : B.d;
-  const int S32 = rows32_stride(d);
+  const int S32 = rows32_stride(d), RS = 2 * S32;  // half and row strides
   float* rows32 = reinterpret_cast<float*>(smem);
-  float* w32 = rows32 + kMixedRows * S32;
   const int c = blockIdx.x;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int col = lane & 15, kq = lane >> 4;
@@ -837,8 +838,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DC > 0 ? 3 
       fa[t] = f == B.p ? d + 1 : d + 2;
       fb[t] = d;
     }
-    fa[t] += kq * S32;
-    fb[t] += kq * S32;
+    fa[t] += kq * RS + S32;  // scaled half
+    fb[t] += kq * RS;        // plain half
   }
   f64x4 acc[NP];
   f32x4 z[NP];
@@ -867,11 +868,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DC > 0 ? 3 
     const int nrows = (int)(end - r0 < kMixedRows ? end - r0 : kMixedRows);
     __syncthreads();  // the previous block's MFMA phase is done with the table
     {
-      float* row = rows32 + tid * S32;
+      float* row = rows32 + tid * RS;
+      float* srow = row + S32;
       if (tid < nrows) {
         const bool ok = (wr >= 0.0) & !isinf(wr);
         bad |= !ok;
         const double wt = ok ? fmax(wr, floor) : 0.0;
+        const double sw = sqrt(wt);
         double s1 = 0.0, s2 = 0.0;
 #pragma unroll
         for (int i = 0; i < MD; ++i)
@@ -880,36 +883,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DC > 0 ? 3 
             s1 += x2;
             s2 += x2 * x2;
             row[i] = (float)xr[i];
+            srow[i] = (float)(sw * xr[i]);
           }
         tr += wt * (bias + s1 + quad * (s1 * s1 + s2));
         row[d] = 1.0f;
         row[d + 1] = (float)yr;
         row[d + 2] = 0.0f;
-        w32[tid] = (float)sqrt(wt);
+        srow[d] = (float)sw;
+        srow[d + 1] = (float)(sw * yr);
+        srow[d + 2] = 0.0f;
       } else {
-        for (int i = 0; i < S32; ++i) row[i] = 0.0f;
-        w32[tid] = 0.0f;
+        for (int i = 0; i < RS; ++i) row[i] = 0.0f;
       }
     }
     load_row(r0 + kMixedRows + tid);  // next block, in flight during the MFMA phase
     __syncthreads();
     const int steps = (nrows + 31) >> 5;
     for (int s = wave; s < steps; s += 4) {
-      const float* base = rows32 + 32 * s * S32;
-      const float* wb = w32 + 32 * s + kq;
+      const float* base = rows32 + 32 * s * RS;
       // Both operands are sqrt(w) phi (G = sum w phi phi^T is symmetric), so
       // each value is split once.  Rows 8k + kq and 8k + 4 + kq share one
       // packed word.
       u32x4 ahw[T], amw[T], alw[T];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const float w0 = wb[8 * k], w1 = wb[8 * k + 4];
 #pragma unroll
         for (int t = 0; t < T; ++t) {
-          const float p0 = base[fa[t] + 8 * k * S32] * base[fb[t] + 8 * k * S32];
-          const float p1 = base[fa[t] + (8 * k + 4) * S32] * base[fb[t] + (8 * k + 4) * S32];
+          const float p0 = base[fa[t] + 8 * k * RS] * base[fb[t] + 8 * k * RS];
+          const float p1 = base[fa[t] + (8 * k + 4) * RS] * base[fb[t] + (8 * k + 4) * RS];
           unsigned hi, mid, lo;
-          split3_pair(w0 * p0, w1 * p1, hi, mid, lo);
+          split3_pair(p0, p1, hi, mid, lo);
           ahw[t][k] = hi;
           amw[t][k] = mid;
           alw[t][k] = lo;
