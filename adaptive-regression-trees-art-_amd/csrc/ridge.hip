@@ -610,66 +610,66 @@ __global__ __launch_bounds__(128) void solve_rows_kernel(
 // loads its rows into registers and factors and solves them there
 // (wave_cholesky, wave_solve: no barrier per column).
 template <int TT>
-__global__ __launch_bounds__(256) void solve_wave_kernel(
+__global__ __launch_bounds__(64) void solve_wave_kernel(
     const SystemDesc* __restrict__ systems, int nchunks, int T, int npairs, int p, double ridge,
     const double* __restrict__ partials, const int* __restrict__ flags,
     const double* __restrict__ tpart, const int* __restrict__ mask, double* __restrict__ factor,
     double* __restrict__ beta, int* __restrict__ status, double* __restrict__ lambda) {
   // Packed lower (p+1) x (p+1): G and the row h; at least 64 doubles, as the
-  // factorization's column buffer reuses it.
+  // factorization's column buffer reuses it.  One wave: LDS written by one
+  // lane and read by another needs no barrier (a wave's LDS operations
+  // complete in order).
   extern __shared__ double M[];
-  __shared__ double red[4];
-  __shared__ int ired[4];
   const SystemDesc S = systems ? systems[blockIdx.x]
                                : SystemDesc{(int)blockIdx.x, (int)blockIdx.x + 1,
                                             (int)blockIdx.x, 0};
   if (mask != nullptr && !mask[S.out]) return;
-  const int tid = threadIdx.x, nt = blockDim.x;
+  const int lane = threadIdx.x;
   int bad = 0;
-  for (int ch = S.chunk_begin + tid; ch < S.chunk_end; ch += nt)
+  for (int ch = S.chunk_begin + lane; ch < S.chunk_end; ch += 64)
     bad |= flags[(size_t)S.side * nchunks + ch];
-  // Tile (t1, t2) element (a, b) is G[16 t1 + a][16 t2 + b], t1 <= t2.  With
-  // 256 threads, thread tid owns element (tid >> 4, tid & 15) of each of the
-  // NP tile pairs, so the adds need no atomics.  Four chunks' loads are issued
-  // together (latency-bound otherwise); the adds stay in chunk order.  The
-  // tiles cover every packed element of the (p+1) x (p+1) system once.
-  constexpr int PMAX = 16 * TT < 64 ? 16 * TT : 64, NP = TT * (TT + 1) / 2;
+  bad = wave_or(bad);
+  // Tile (t1, t2) element (a, b) is G[16 t1 + a][16 t2 + b], t1 <= t2.  Lane
+  // l owns elements l + 64 v (v < 4 NP) of a chunk's NP tile pairs, so the
+  // adds need no atomics.  Two chunks' loads are issued together
+  // (latency-bound otherwise); the adds stay in chunk order.  The tiles cover
+  // every packed element of the (p+1) x (p+1) system once.
+  constexpr int PMAX = 16 * TT < 64 ? 16 * TT : 64, NP = TT * (TT + 1) / 2, NE = 4 * NP;
   constexpr int total = NP * 256;
-  double acc[NP];
+  double acc[NE];
 #pragma unroll
-  for (int u = 0; u < NP; ++u) acc[u] = 0.0;
-  const double* base = partials + (size_t)S.side * nchunks * total + tid;
+  for (int v = 0; v < NE; ++v) acc[v] = 0.0;
+  const double* base = partials + (size_t)S.side * nchunks * total + lane;
   int ch = S.chunk_begin;
-  for (; ch + 4 <= S.chunk_end; ch += 4) {
-    double v[4][NP];
+  for (; ch + 2 <= S.chunk_end; ch += 2) {
+    double v0[NE], v1[NE];
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+    for (int v = 0; v < NE; ++v) {
+      v0[v] = base[(size_t)ch * total + 64 * v];
+      v1[v] = base[(size_t)(ch + 1) * total + 64 * v];
+    }
 #pragma unroll
-      for (int u = 0; u < NP; ++u) v[q][u] = base[(size_t)(ch + q) * total + 256 * u];
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-#pragma unroll
-      for (int u = 0; u < NP; ++u) acc[u] += v[q][u];
+    for (int v = 0; v < NE; ++v) acc[v] = (acc[v] + v0[v]) + v1[v];
   }
-  for (; ch < S.chunk_end; ++ch) {
+  if (ch < S.chunk_end) {
 #pragma unroll
-    for (int u = 0; u < NP; ++u) acc[u] += base[(size_t)ch * total + 256 * u];
+    for (int v = 0; v < NE; ++v) acc[v] += base[(size_t)ch * total + 64 * v];
   }
 #pragma unroll
-  for (int u = 0; u < NP; ++u) {
+  for (int v = 0; v < NE; ++v) {
     int t1, t2;
-    pair_tiles(u, TT, t1, t2);
-    const int r = 16 * t1 + (tid >> 4), c = 16 * t2 + (tid & 15);
-    if (r <= c && c <= p) M[tri(c, r)] = acc[u];
+    pair_tiles(v >> 2, TT, t1, t2);
+    const int e = ((v & 3) << 6) + lane;  // element within the tile
+    const int r = 16 * t1 + (e >> 4), c = 16 * t2 + (e & 15);
+    if (r <= c && c <= p) M[tri(c, r)] = acc[v];
   }
   double tpiece = 0.0;
-  if (tpart != nullptr)
-    for (int ch = S.chunk_begin + tid; ch < S.chunk_end; ch += nt) tpiece += tpart[ch];
-  bad = block_or(bad, ired);  // its barriers also publish M
-  if (tpart == nullptr && tid < p) tpiece = M[tri(tid, tid)];
-  const double tr = block_sum(tpiece, red);
-  if (tid >= 64) return;
-  const int lane = tid;
+  if (tpart != nullptr) {
+    for (int ch2 = S.chunk_begin + lane; ch2 < S.chunk_end; ch2 += 64) tpiece += tpart[ch2];
+  } else if (lane < p) {
+    tpiece = M[tri(lane, lane)];
+  }
+  const double tr = wave_sum(tpiece);
   const double lam = ridge == 0.0 ? 0.0 : ridge * (tr / p);
   int st = bad ? -1 : ((!isfinite(tr) || !isfinite(lam)) ? -2 : 0);
   // Each loop below tests lane against every column; a fresh opaque copy of
@@ -1565,7 +1565,7 @@ static void launch_rows_solve(const SystemDesc* sys, int nsys, int nchunks, int 
   // The wave kernel also uses the start of its LDS as a 64-double column.
   const size_t wlds = std::max(lds, (size_t)64 * sizeof(double));
 #define ART_WAVE(TT)                                                                        \
-  solve_wave_kernel<TT><<<dim3(nsys), dim3(256), wlds, stream>>>(                           \
+  solve_wave_kernel<TT><<<dim3(nsys), dim3(64), wlds, stream>>>(                            \
       sys, nchunks, T, npairs, p, ridge, partials, flags, tpart, mask, factor, beta, status, \
       lambda)
   if (T == 1) ART_WAVE(1);
