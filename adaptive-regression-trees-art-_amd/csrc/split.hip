@@ -109,24 +109,30 @@ __device__ inline void init_basis_tables(const Basis& B, SplitShared& S) {
 }
 
 // Row staging: a 32-lane group per row, 256 threads -> 8 rows per sweep and
-// kStage sweeps per chunk.  All kStage global loads of a thread issue before
-// its LDS stores, so a chunk costs one memory round trip; the values are live
-// only while staging, and the second resident workgroup of the CU computes
-// meanwhile.  S.xs holds x_0..x_{d-1}, y per row with stride d + 1.
+// kStage sweeps per chunk.  A thread's kStage global loads for the next
+// chunk are issued before the current chunk is processed (register double
+// buffering: the barriers wait on LDS only, not on these loads), so the
+// memory round trip hides behind a chunk of compute.  S.xs holds
+// x_0..x_{d-1}, y per row with stride kXStride.
 constexpr int kStage = kChunk / 8;
 
-__device__ inline void stage_rows(const SplitCtx& C, SplitShared& S, int base, int rows) {
+__device__ inline void load_rows(const SplitCtx& C, int base, int rows, double (&v)[kStage]) {
   const int g = threadIdx.x >> 5, j = threadIdx.x & 31;
   const bool isx = j < C.d, isy = j == C.d;
   const double* src = isx ? C.X + ((size_t)base + g) * C.d + j : C.y + base + g;
   const size_t stride = isx ? 8 * (size_t)C.d : 8;
   const int lim = (isx || isy) ? rows : 0;
-  double v[kStage];
 #pragma unroll
   for (int i = 0; i < kStage; ++i) {
     v[i] = (g + 8 * i < lim) ? *src : 0.0;
     src += stride;
   }
+}
+
+__device__ inline void store_rows(const SplitCtx& C, SplitShared& S, int rows,
+                                  const double (&v)[kStage]) {
+  const int g = threadIdx.x >> 5, j = threadIdx.x & 31;
+  const int lim = j <= C.d ? rows : 0;
   double* dst = S.xs + g * kXStride + j;
 #pragma unroll
   for (int i = 0; i < kStage; ++i)
@@ -134,13 +140,17 @@ __device__ inline void stage_rows(const SplitCtx& C, SplitShared& S, int base, i
 }
 
 // Iterates the node's rows in staged chunks: body(rows) runs with chunk rows
-// [base, base + rows) in S.xs.
+// [base, base + rows) in S.xs while the next chunk's loads are in flight.
 template <class Body>
 __device__ inline void for_each_chunk(const SplitCtx& C, SplitShared& S, Body body) {
+  double v[kStage];
+  load_rows(C, 0, min(kChunk, C.n), v);
   for (int base = 0; base < C.n; base += kChunk) {
     const int rows = min(kChunk, C.n - base);
     __syncthreads();  // the previous chunk is consumed
-    stage_rows(C, S, base, rows);
+    store_rows(C, S, rows, v);
+    const int next = base + kChunk;
+    if (next < C.n) load_rows(C, next, min(kChunk, C.n - next), v);
     __syncthreads();
     body(rows);
   }
