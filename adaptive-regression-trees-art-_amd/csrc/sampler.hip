@@ -272,6 +272,15 @@ __global__ __launch_bounds__(64) void har_chain_kernel(art_har_args a, int E) {
 #pragma unroll
   for (int j = 0; j < DM; ++j) x[j] = (EXACT || j < d) ? x0[j] : 0.0;
   double* out = a.samples + (size_t)poly * a.n * d;
+  // Box bounds in registers for the whole chain, and a countdown to the next
+  // recorded move instead of a division per move.
+  double blo[DM], bhi[DM];
+#pragma unroll
+  for (int j = 0; j < DM; ++j) {
+    blo[j] = (box_lo && (EXACT || j < d)) ? box_lo[j] : 0.0;
+    bhi[j] = (box_lo && (EXACT || j < d)) ? box_hi[j] : 0.0;
+  }
+  int until_record = a.first, recorded = 0;
 
   for (int s0 = 0; s0 < moves; s0 += kChainBlock) {
     const int cnt = min(kChainBlock, moves - s0);
@@ -305,8 +314,8 @@ __global__ __launch_bounds__(64) void har_chain_kernel(art_har_args a, int E) {
         for (int j = 0; j < DM; ++j)
           if (EXACT || j < d) {
             const double dj = e[j], r = e[DM + j];
-            const double rl = (box_lo[j] - x[j]) * r;
-            const double rh = (box_hi[j] - x[j]) * r;
+            const double rl = (blo[j] - x[j]) * r;
+            const double rh = (bhi[j] - x[j]) * r;
             const bool order = rl < rh;
             const double lo_c = order ? rl : rh;
             const double hi_c = order ? rh : rl;
@@ -330,12 +339,15 @@ __global__ __launch_bounds__(64) void har_chain_kernel(art_har_args a, int E) {
       const double t = (thi - tlo) * e[2 * DM] + tlo;
 #pragma unroll
       for (int j = 0; j < DM; ++j) x[j] += e[j] * t;
-      const int done = s0 + k + 1;
-      if (lane == 0 && done >= a.first && (done - a.first) % a.stride == 0) {
-        double* o = out + (size_t)((done - a.first) / a.stride) * d;
+      if (--until_record == 0) {  // moves first, first + stride, ...
+        if (lane == 0) {
+          double* o = out + (size_t)recorded * d;
 #pragma unroll
-        for (int j = 0; j < DM; ++j)
-          if (EXACT || j < d) o[j] = x[j];
+          for (int j = 0; j < DM; ++j)
+            if (EXACT || j < d) o[j] = x[j];
+        }
+        ++recorded;
+        until_record = a.stride;
       }
     }
     __syncthreads();
