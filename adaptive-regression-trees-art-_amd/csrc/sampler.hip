@@ -248,9 +248,14 @@ __global__ __launch_bounds__(64) void har_kernel(art_har_args a) {
 // LDS entry of a move: dir[DM], 1/dir[DM], u, A_i.d[m], 1/(A_i.d)[m].
 constexpr int kChainBlock = 64;
 
-template <int DM, bool EXACT>
-__global__ __launch_bounds__(64) void har_chain_kernel(art_har_args a, int E) {
-  extern __shared__ double chain_lds[];
+// One chain per workgroup, M >= 0: exactly M general rows, held in registers
+// with b for the whole chain (the replay then reads no constraint data per
+// move; a scalar load on every row of every move otherwise); M < 0: any row
+// count, read per move.  The rows are scanned in the same order either way,
+// so both forms give the same chain.
+template <int DM, bool EXACT, int M>
+__device__ __attribute__((always_inline)) inline void chain_run(const art_har_args& a, int E,
+                                                               double* chain_lds) {
   const int d = EXACT ? DM : a.d;
   const int poly = blockIdx.x, lane = (int)threadIdx.x;
   ci64* offs = (ci64*)a.row_offsets;
@@ -281,6 +286,16 @@ __global__ __launch_bounds__(64) void har_chain_kernel(art_har_args a, int E) {
     bhi[j] = (box_lo && (EXACT || j < d)) ? box_hi[j] : 0.0;
   }
   int until_record = a.first, recorded = 0;
+  constexpr int MR = M > 0 ? M : 1;
+  double Ar[MR][DM], br[MR];
+  if constexpr (M > 0) {
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+      br[i] = b[i];
+#pragma unroll
+      for (int j = 0; j < DM; ++j) Ar[i][j] = (EXACT || j < d) ? A[i * d + j] : 0.0;
+    }
+  }
 
   for (int s0 = 0; s0 < moves; s0 += kChainBlock) {
     const int cnt = min(kChainBlock, moves - s0);
@@ -325,16 +340,30 @@ __global__ __launch_bounds__(64) void har_chain_kernel(art_har_args a, int E) {
           }
       }
       const double* ea = e + 2 * DM + 1;
-#pragma unroll 2
-      for (int i = 0; i < m; ++i) {
-        double ax = 0.0;
+      if constexpr (M >= 0) {
 #pragma unroll
-        for (int j = 0; j < DM; ++j)
-          if (EXACT || j < d) ax = fma(A[i * d + j], x[j], ax);
-        const double ad = ea[i];
-        const double ratio = (b[i] - ax) * ea[m + i];
-        tlo = (ad < 0.0 && ratio > tlo) ? ratio : tlo;
-        thi = (ad > 0.0 && ratio < thi) ? ratio : thi;
+        for (int i = 0; i < M; ++i) {
+          double ax = 0.0;
+#pragma unroll
+          for (int j = 0; j < DM; ++j)
+            if (EXACT || j < d) ax = fma(Ar[i][j], x[j], ax);
+          const double ad = ea[i];
+          const double ratio = (br[i] - ax) * ea[M + i];
+          tlo = (ad < 0.0 && ratio > tlo) ? ratio : tlo;
+          thi = (ad > 0.0 && ratio < thi) ? ratio : thi;
+        }
+      } else {
+#pragma unroll 2
+        for (int i = 0; i < m; ++i) {
+          double ax = 0.0;
+#pragma unroll
+          for (int j = 0; j < DM; ++j)
+            if (EXACT || j < d) ax = fma(A[i * d + j], x[j], ax);
+          const double ad = ea[i];
+          const double ratio = (b[i] - ax) * ea[m + i];
+          tlo = (ad < 0.0 && ratio > tlo) ? ratio : tlo;
+          thi = (ad > 0.0 && ratio < thi) ? ratio : thi;
+        }
       }
       const double t = (thi - tlo) * e[2 * DM] + tlo;
 #pragma unroll
@@ -352,6 +381,31 @@ __global__ __launch_bounds__(64) void har_chain_kernel(art_har_args a, int E) {
     }
     __syncthreads();
   }
+}
+
+
+// Single-chain kernel: polytopes of at most 8 general rows in d <= 4 (the
+// builder's nodes: a box and the splits above them) run with their rows in
+// registers.
+template <int DM, bool EXACT>
+__global__ __launch_bounds__(64) void har_chain_kernel(art_har_args a, int E) {
+  extern __shared__ double chain_lds[];
+  if constexpr (DM <= 4) {
+    ci64* offs = (ci64*)a.row_offsets;
+    const int m = (int)(offs[blockIdx.x + 1] - offs[blockIdx.x]);
+    switch (m) {
+#define ART_ROWS(MM) \
+  case MM:           \
+    chain_run<DM, EXACT, MM>(a, E, chain_lds); \
+    return;
+      ART_ROWS(0) ART_ROWS(1) ART_ROWS(2) ART_ROWS(3) ART_ROWS(4) ART_ROWS(5) ART_ROWS(6)
+      ART_ROWS(7) ART_ROWS(8)
+#undef ART_ROWS
+      default:
+        break;
+    }
+  }
+  chain_run<DM, EXACT, -1>(a, E, chain_lds);
 }
 
 // LDS doubles per move of har_chain_kernel for dimension-bucket DM and m rows.
