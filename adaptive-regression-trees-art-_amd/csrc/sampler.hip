@@ -303,14 +303,20 @@ __device__ __attribute__((always_inline)) inline void chain_run(const art_har_ar
       double dir[DM], u;
       move_randoms<DM, EXACT>(a, shape, d, W, poly, stream_poly, 0, s0 + lane, k0, k1, dir, u);
       double* e = chain_lds + (size_t)lane * E;
+      // Per coordinate: the box face the chord leaves through on each side
+      // is fixed by the direction's sign, so the move stores it (or +-inf
+      // with a unit scale when the coordinate does not move).
 #pragma unroll
       for (int j = 0; j < DM; ++j)
         if (EXACT || j < d) {
+          const bool nz = dir[j] != 0.0, up = dir[j] > 0.0;
           e[j] = dir[j];
-          e[DM + j] = rcp2(dir[j]);
+          e[DM + j] = nz ? rcp2(dir[j]) : 1.0;
+          e[2 * DM + 1 + j] = nz ? (up ? blo[j] : bhi[j]) : -INFINITY;
+          e[3 * DM + 1 + j] = nz ? (up ? bhi[j] : blo[j]) : INFINITY;
         }
       e[2 * DM] = u;
-      double* ea = e + 2 * DM + 1;
+      double* ea = e + 4 * DM + 1;
       for (int i = 0; i < m; ++i) {
         double ad = 0.0;
 #pragma unroll
@@ -325,21 +331,19 @@ __device__ __attribute__((always_inline)) inline void chain_run(const art_har_ar
       const double* e = chain_lds + (size_t)k * E;
       double tlo = -INFINITY, thi = INFINITY;
       if (box_lo) {
+        // (face - x) * (1 / d_j): the same products as ordering the two face
+        // ratios by value, the order taken from the direction's sign.
 #pragma unroll
         for (int j = 0; j < DM; ++j)
           if (EXACT || j < d) {
-            const double dj = e[j], r = e[DM + j];
-            const double rl = (blo[j] - x[j]) * r;
-            const double rh = (bhi[j] - x[j]) * r;
-            const bool order = rl < rh;
-            const double lo_c = order ? rl : rh;
-            const double hi_c = order ? rh : rl;
-            const bool nz = dj != 0.0;
-            tlo = (nz && lo_c > tlo) ? lo_c : tlo;
-            thi = (nz && hi_c < thi) ? hi_c : thi;
+            const double r = e[DM + j];
+            const double lo_c = (e[2 * DM + 1 + j] - x[j]) * r;
+            const double hi_c = (e[3 * DM + 1 + j] - x[j]) * r;
+            tlo = lo_c > tlo ? lo_c : tlo;
+            thi = hi_c < thi ? hi_c : thi;
           }
       }
-      const double* ea = e + 2 * DM + 1;
+      const double* ea = e + 4 * DM + 1;
       if constexpr (M >= 0) {
 #pragma unroll
         for (int i = 0; i < M; ++i) {
@@ -409,7 +413,7 @@ __global__ __launch_bounds__(64) void har_chain_kernel(art_har_args a, int E) {
 }
 
 // LDS doubles per move of har_chain_kernel for dimension-bucket DM and m rows.
-inline int chain_entry(int DM, int m) { return 2 * DM + 1 + 2 * m; }
+inline int chain_entry(int DM, int m) { return 4 * DM + 1 + 2 * m; }
 constexpr int kChainLdsBytes = 64 * 1024;
 
 template <int DM, bool EXACT>
