@@ -514,7 +514,6 @@ __global__ __launch_bounds__(64) void solve_f64_kernel(const SystemDesc* __restr
 // loops are not unrolled over p.
 constexpr int kRowsMaxP = 120;
 
-__device__ inline int rows_threads(int p) { return 64 * ((p + 63) / 64); }
 
 // Reduce + regularize + factor + solve, same contract as solve_f64_kernel;
 // optionally stores the factor rows (packed) and skips unmasked systems.
@@ -760,7 +759,7 @@ __global__ __launch_bounds__(256) void reduce_leaf_partials_kernel(
 //    lambda from the exact trace.
 // 3. residual_leaf_kernel: FP64 data-space residual Phi^T W (y - Phi beta0)
 //    per chunk (a second pass over X, y, w).
-// 4. correct_rows_kernel: beta1 = beta0 + (L L^T)^-1 (r - lambda beta0); a
+// 4. correct_wave_kernel: beta1 = beta0 + (L L^T)^-1 (r - lambda beta0); a
 //    leaf whose correction exceeds tau * |beta1| (or whose mixed solve
 //    failed) is flagged for the exact FP64 path, which then runs only on the
 //    flagged leaves' chunks.
@@ -1151,53 +1150,9 @@ __global__ __launch_bounds__(256) void residual_quad_kernel(
 // left is ~ tau^2 |beta|.  Later steps run on the flagged leaves (mask) and
 // accept when the step contracts (rho = |delta_k| / |delta_{k-1}| < 1/2) and
 // the error left, ~ rho |delta_k| / (1 - rho), is at most tau^2 |beta|.
-// dnorm holds |delta|^2 of the last step per leaf.
-__global__ __launch_bounds__(128) void correct_rows_kernel(
-    const SystemDesc* __restrict__ systems, const int* __restrict__ roff, int p, int rstride,
-    const double* __restrict__ rpart,
-    const double* __restrict__ factor, const double* __restrict__ lambda,
-    double* __restrict__ beta, const int* __restrict__ status, int* __restrict__ need,
-    double tau, const int* __restrict__ mask, double* __restrict__ dnorm) {
-  extern __shared__ double M[];
-  __shared__ double bc[2];
-  __shared__ double red[2];
-  const SystemDesc S = systems[blockIdx.x];
-  const int leaf = S.out, i = threadIdx.x, nt = blockDim.x;
-  if (mask && !mask[leaf]) return;
-  if (status[leaf] != 0) {
-    if (i == 0) need[leaf] = 1;
-    return;
-  }
-  const int nel = p * (p + 1) / 2;
-  const double* L = factor + (size_t)leaf * nel;
-  for (int e = i; e < nel; e += nt) M[e] = L[e];
-  double r = 0.0, b0 = 0.0;
-  if (i < p) {
-    for (int ch = roff[leaf]; ch < roff[leaf + 1]; ++ch) r += rpart[(size_t)ch * rstride + i];
-    b0 = beta[(size_t)leaf * p + i];
-    r -= lambda[leaf] * b0;
-  }
-  __syncthreads();
-  const double delta = rows_solve(M, bc, p, i, r);
-  const double b1 = b0 + delta;
-  const double nd = block_sum(i < p ? delta * delta : 0.0, red);
-  const double nb = block_sum(i < p ? b1 * b1 : 0.0, red);
-  if (i < p) beta[(size_t)leaf * p + i] = b1;
-  if (i == 0) {
-    if (!mask) {
-      need[leaf] = !(nd <= tau * tau * nb);
-    } else {
-      const double prev = dnorm[leaf];
-      const double rho = prev > 0.0 ? sqrt(nd / prev) : 1.0;
-      const double left = rho < 1.0 ? rho * sqrt(nd) / (1.0 - rho) : INFINITY;
-      need[leaf] = !(rho < 0.5 && left <= tau * tau * sqrt(nb));
-    }
-    dnorm[leaf] = nd;
-  }
-}
-
-// correct_rows_kernel for p <= PMAX <= 64 on one wave: the stored factor's
-// rows are loaded into registers and the step is solved by wave_solve.
+// dnorm holds |delta|^2 of the last step per leaf.  One wave per leaf
+// (p <= PMAX <= 64): the stored factor's rows, then its columns, are loaded
+// into registers for the two substitutions.
 template <int PMAX>
 __global__ __launch_bounds__(64) void correct_wave_kernel(
     const SystemDesc* __restrict__ systems, const int* __restrict__ roff, int p, int rstride,
