@@ -108,42 +108,50 @@ __device__ inline void init_basis_tables(const Basis& B, SplitShared& S) {
   __syncthreads();
 }
 
-// Row staging: a 32-lane group per row, 256 threads -> 8 rows per sweep and
-// kStage sweeps per chunk.  A thread's kStage global loads for the next
-// chunk are issued before the current chunk is processed (register double
-// buffering: the barriers wait on LDS only, not on these loads), so the
-// memory round trip hides behind a chunk of compute.  S.xs holds
-// x_0..x_{d-1}, y per row with stride kXStride.
-constexpr int kStage = kChunk / 8;
+// Row staging: a chunk's x values are one contiguous block of rows * d
+// doubles (rows are contiguous in X), loaded element-wise by the 256 threads
+// (coalesced; ceil(rows d / 256) loads per thread, one for d = 2), and its
+// y values one per thread.  A thread's loads for the next chunk are issued
+// before the current chunk is processed (register double buffering: the
+// barriers wait on LDS only, not on these loads), so the memory round trip
+// hides behind a chunk of compute.  S.xs holds x_0..x_{d-1}, y per row with
+// stride kXStride.
+constexpr int kStage = (kChunk * kSplitMaxD + 255) / 256;
 
-__device__ inline void load_rows(const SplitCtx& C, int base, int rows, double (&v)[kStage]) {
-  const int g = threadIdx.x >> 5, j = threadIdx.x & 31;
-  const bool isx = j < C.d, isy = j == C.d;
-  const double* src = isx ? C.X + ((size_t)base + g) * C.d + j : C.y + base + g;
-  const size_t stride = isx ? 8 * (size_t)C.d : 8;
-  const int lim = (isx || isy) ? rows : 0;
+struct Staged {
+  double x[kStage];
+  double y;
+};
+
+__device__ inline void load_rows(const SplitCtx& C, int base, int rows, Staged& v) {
+  const int tid = threadIdx.x, nx = rows * C.d;
+  const double* src = C.X + (size_t)base * C.d;
 #pragma unroll
   for (int i = 0; i < kStage; ++i) {
-    v[i] = (g + 8 * i < lim) ? *src : 0.0;
-    src += stride;
+    const int e = tid + 256 * i;
+    v.x[i] = e < nx ? src[e] : 0.0;
   }
+  v.y = tid < rows ? C.y[base + tid] : 0.0;
 }
 
-__device__ inline void store_rows(const SplitCtx& C, SplitShared& S, int rows,
-                                  const double (&v)[kStage]) {
-  const int g = threadIdx.x >> 5, j = threadIdx.x & 31;
-  const int lim = j <= C.d ? rows : 0;
-  double* dst = S.xs + g * kXStride + j;
+__device__ inline void store_rows(const SplitCtx& C, SplitShared& S, int rows, const Staged& v) {
+  const int tid = threadIdx.x, d = C.d, nx = rows * d;
 #pragma unroll
-  for (int i = 0; i < kStage; ++i)
-    if (g + 8 * i < lim) dst[8 * kXStride * i] = v[i];
+  for (int i = 0; i < kStage; ++i) {
+    const int e = tid + 256 * i;
+    if (e < nx) {
+      const int r = e / d;
+      S.xs[r * kXStride + (e - r * d)] = v.x[i];
+    }
+  }
+  if (tid < rows) S.xs[tid * kXStride + d] = v.y;
 }
 
 // Iterates the node's rows in staged chunks: body(rows) runs with chunk rows
 // [base, base + rows) in S.xs while the next chunk's loads are in flight.
 template <class Body>
 __device__ inline void for_each_chunk(const SplitCtx& C, SplitShared& S, Body body) {
-  double v[kStage];
+  Staged v;
   load_rows(C, 0, min(kChunk, C.n), v);
   for (int base = 0; base < C.n; base += kChunk) {
     const int rows = min(kChunk, C.n - base);
