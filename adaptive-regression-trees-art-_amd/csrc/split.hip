@@ -53,9 +53,17 @@ __device__ inline double gate(double score) {
   return 1.0 / (1.0 + exp(-fmin(50.0, fmax(-50.0, score))));
 }
 
-constexpr int kChunk = 128;             // rows staged in LDS per pass step
-constexpr int kXStride = kSplitMaxD + 2;  // odd stride in doubles: 33
+// Rows staged in LDS per pass step and their stride (odd, in doubles).  With
+// one feature tile (at most 16 non-constant child features, so d <= 16) a
+// chunk is 256 rows of stride 17: one row per thread in the per-row phases;
+// two tiles (d <= 31) keep 128 rows of stride 33 within the LDS budget.
+template <int T> struct ChunkShape;
+template <> struct ChunkShape<1> { static constexpr int rows = 256, stride = 17, maxd = 16; };
+template <> struct ChunkShape<2> { static constexpr int rows = 128, stride = 33, maxd = 31; };
+template <int T> constexpr int kChunkT = ChunkShape<T>::rows;
+template <int T> constexpr int kXStrideT = ChunkShape<T>::stride;
 
+template <int T>
 struct SplitShared {
   double theta[kSplitMaxD + 1];
   double pdir[kSplitMaxD + 1];
@@ -72,8 +80,8 @@ struct SplitShared {
   int ired[4];
   double dred[4 * 64];
   double tot[64];
-  double xs[kChunk * kXStride];  // staged rows: x_0..x_{d-1}, y; stride kXStride
-  double qs[2][kChunk];          // per-row side weights (fit) or gradient terms (stats)
+  double xs[kChunkT<T> * kXStrideT<T>];  // staged rows: x_0..x_{d-1}, y
+  double qs[2][kChunkT<T>];  // per-row side weights (fit) or gradient terms (stats)
   signed char fa[kSplitMaxP];    // feature f = x[fa] * x[fb]; -1 stands for 1
   signed char fb[kSplitMaxP];
   signed char core[kSplitMaxP];  // basis column of non-constant feature k
@@ -91,7 +99,8 @@ struct SplitCtx {
 };
 
 // Feature tables of the basis (once per kernel, before any pass).
-__device__ inline void init_basis_tables(const Basis& B, SplitShared& S) {
+template <int T>
+__device__ inline void init_basis_tables(const Basis& B, SplitShared<T>& S) {
   if (threadIdx.x == 0) {
     int nf = 0, cb = -1;
     for (int f = 0; f < B.p; ++f) {
@@ -116,42 +125,47 @@ __device__ inline void init_basis_tables(const Basis& B, SplitShared& S) {
 // barriers wait on LDS only, not on these loads), so the memory round trip
 // hides behind a chunk of compute.  S.xs holds x_0..x_{d-1}, y per row with
 // stride kXStride.
-constexpr int kStage = (kChunk * kSplitMaxD + 255) / 256;
+template <int T> constexpr int kStageT = (kChunkT<T> * ChunkShape<T>::maxd + 255) / 256;
 
+template <int T>
 struct Staged {
-  double x[kStage];
+  double x[kStageT<T>];
   double y;
 };
 
-__device__ inline void load_rows(const SplitCtx& C, int base, int rows, Staged& v) {
+template <int T>
+__device__ inline void load_rows(const SplitCtx& C, int base, int rows, Staged<T>& v) {
   const int tid = threadIdx.x, nx = rows * C.d;
   const double* src = C.X + (size_t)base * C.d;
 #pragma unroll
-  for (int i = 0; i < kStage; ++i) {
+  for (int i = 0; i < kStageT<T>; ++i) {
     const int e = tid + 256 * i;
     v.x[i] = e < nx ? src[e] : 0.0;
   }
   v.y = tid < rows ? C.y[base + tid] : 0.0;
 }
 
-__device__ inline void store_rows(const SplitCtx& C, SplitShared& S, int rows, const Staged& v) {
+template <int T>
+__device__ inline void store_rows(const SplitCtx& C, SplitShared<T>& S, int rows,
+                                  const Staged<T>& v) {
   const int tid = threadIdx.x, d = C.d, nx = rows * d;
 #pragma unroll
-  for (int i = 0; i < kStage; ++i) {
+  for (int i = 0; i < kStageT<T>; ++i) {
     const int e = tid + 256 * i;
     if (e < nx) {
       const int r = e / d;
-      S.xs[r * kXStride + (e - r * d)] = v.x[i];
+      S.xs[r * kXStrideT<T> + (e - r * d)] = v.x[i];
     }
   }
-  if (tid < rows) S.xs[tid * kXStride + d] = v.y;
+  if (tid < rows) S.xs[tid * kXStrideT<T> + d] = v.y;
 }
 
 // Iterates the node's rows in staged chunks: body(rows) runs with chunk rows
 // [base, base + rows) in S.xs while the next chunk's loads are in flight.
-template <class Body>
-__device__ inline void for_each_chunk(const SplitCtx& C, SplitShared& S, Body body) {
-  Staged v;
+template <int T, class Body>
+__device__ inline void for_each_chunk(const SplitCtx& C, SplitShared<T>& S, Body body) {
+  constexpr int kChunk = kChunkT<T>;
+  Staged<T> v;
   load_rows(C, 0, min(kChunk, C.n), v);
   for (int base = 0; base < C.n; base += kChunk) {
     const int rows = min(kChunk, C.n - base);
@@ -164,7 +178,8 @@ __device__ inline void for_each_chunk(const SplitCtx& C, SplitShared& S, Body bo
   }
 }
 
-__device__ inline double staged_feature(const SplitShared& S, const double* xr, int f) {
+template <int T>
+__device__ inline double staged_feature(const SplitShared<T>& S, const double* xr, int f) {
   const int a = S.fa[f], b = S.fb[f];
   double v = a >= 0 ? xr[a] : 1.0;
   if (b >= 0) v *= xr[b];
@@ -172,8 +187,8 @@ __device__ inline double staged_feature(const SplitShared& S, const double* xr, 
 }
 
 // Sums of K values over the block; result in S.tot[0..K).  All threads call.
-template <int K>
-__device__ inline void block_sums(const double (&v)[K], SplitShared& S, int count = K) {
+template <int K, int T>
+__device__ inline void block_sums(const double (&v)[K], SplitShared<T>& S, int count = K) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   __syncthreads();
 #pragma unroll
@@ -201,12 +216,12 @@ __device__ inline void block_sums(const double (&v)[K], SplitShared& S, int coun
 // of row 4s + kq), and the same lanes accumulate the feature sums against
 // q and q y (the constant and y columns of the augmented system).
 template <int T>
-__device__ void fit_sides(const SplitCtx& C, SplitShared& S, bool hard, double floor,
+__device__ void fit_sides(const SplitCtx& C, SplitShared<T>& S, bool hard, double floor,
                           bool half = false) {
   constexpr int NP = T * (T + 1) / 2;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int col = lane & 15, kq = lane >> 4;
-  const int d = C.d, p = C.p, n = C.n, sd = kXStride, nf = S.nf;
+  const int d = C.d, p = C.p, n = C.n, sd = kXStrideT<T>, nf = S.nf;
   int fa[T], fb[T];
   bool live[T];
 #pragma unroll
@@ -230,7 +245,7 @@ __device__ void fit_sides(const SplitCtx& C, SplitShared& S, bool hard, double f
   int bad = 0;
   const double z = S.theta[d];
   for_each_chunk(C, S, [&](int rows) {
-    if (tid < kChunk) {
+    if (tid < kChunkT<T>) {
       double q0 = 0.0, q1 = 0.0;
       if (tid < rows) {
         const double* xr = S.xs + tid * sd;
@@ -261,7 +276,7 @@ __device__ void fit_sides(const SplitCtx& C, SplitShared& S, bool hard, double f
       S.qs[1][tid] = q1;
     }
     __syncthreads();
-    for (int st = wave; st < kChunk / 4; st += 4) {
+    for (int st = wave; st < kChunkT<T> / 4; st += 4) {
       const int r = 4 * st + kq;
       const bool valid = r < rows;
       const double* xr = S.xs + r * sd;
@@ -395,7 +410,8 @@ __device__ void fit_sides(const SplitCtx& C, SplitShared& S, bool hard, double f
 }
 
 // Predictions of both sides for one staged row.
-__device__ inline void side_predictions(const SplitShared& S, const double* xr, int p, double& pl,
+template <int T>
+__device__ inline void side_predictions(const SplitShared<T>& S, const double* xr, int p, double& pl,
                                         double& pr) {
   pl = 0.0;
   pr = 0.0;
@@ -411,16 +427,17 @@ __device__ inline void side_predictions(const SplitShared& S, const double* xr, 
 // packet = [loss, |projected grad|, slope, n_right, lambda_L, lambda_R, bad].
 // The gradient X^T c is formed per chunk from the per-row terms c in LDS,
 // a (column, row group) thread grid over the staged rows.
-__device__ void stats_and_direction(const SplitCtx& C, SplitShared& S) {
+template <int T>
+__device__ void stats_and_direction(const SplitCtx& C, SplitShared<T>& S) {
   const int tid = threadIdx.x;
-  const int d = C.d, p = C.p, n = C.n, sd = kXStride;
+  const int d = C.d, p = C.p, n = C.n, sd = kXStrideT<T>;
   const double z = S.theta[d];
   const double inv = 1.0 / (n * C.T);
   const int gj = tid & 31, gg = tid >> 5, groups = (int)(blockDim.x >> 5);
   double sums[3] = {0.0, 0.0, 0.0};  // loss, -sum c, count(pi >= 1/2)
   double g = 0.0;                   // partial of (X^T c)[gj]
   for_each_chunk(C, S, [&](int rows) {
-    if (tid < kChunk) {
+    if (tid < kChunkT<T>) {
       double c = 0.0;
       if (tid < rows) {
         const double* xr = S.xs + tid * sd;
@@ -494,10 +511,11 @@ __device__ void stats_and_direction(const SplitCtx& C, SplitShared& S) {
 }
 
 // Frozen-model trial losses of `count` (<= kSplitMaxH) candidates alphas[0..count).
-__device__ void trial_losses(const SplitCtx& C, SplitShared& S, const double* alphas, int count,
+template <int T>
+__device__ void trial_losses(const SplitCtx& C, SplitShared<T>& S, const double* alphas, int count,
                              double* out) {
   const int tid = threadIdx.x;
-  const int d = C.d, p = C.p, n = C.n, sd = kXStride;
+  const int d = C.d, p = C.p, n = C.n, sd = kXStrideT<T>;
   const double z = S.theta[d], temp = C.T;
   const double g0 = S.geometry[0], g1 = S.geometry[1], g2 = S.geometry[2], g3 = S.geometry[3];
   double f[kSplitMaxH];
@@ -544,10 +562,11 @@ __device__ void trial_losses(const SplitCtx& C, SplitShared& S, const double* al
 
 // Hard partition statistics after a hard fit (art/_cuda_batched_optimizer.cu:92-120):
 // [MSE, MSE_L, MSE_R, n_right, lambda_L, lambda_R, bad].
-__device__ void hard_stats(const SplitCtx& C, SplitShared& S, bool refit, double* out,
+template <int T>
+__device__ void hard_stats(const SplitCtx& C, SplitShared<T>& S, bool refit, double* out,
                            bool all_right = false) {
   const int tid = threadIdx.x;
-  const int d = C.d, p = C.p, n = C.n, sd = kXStride;
+  const int d = C.d, p = C.p, n = C.n, sd = kXStrideT<T>;
   const double z = S.theta[d];
   double sums[3] = {0.0, 0.0, 0.0};  // SSE left, SSE right, count right
   int bad = 0;
@@ -597,7 +616,7 @@ __device__ inline double clip_alpha(const art_split_args* a, double v) {
 
 template <int T>
 __global__ __launch_bounds__(256) void split_optimize_kernel(art_split_args args) {
-  __shared__ SplitShared S;
+  __shared__ SplitShared<T> S;
   __shared__ double alphas[kSplitMaxH];
   __shared__ double losses[kSplitMaxH];
   __shared__ int sflag[8];
