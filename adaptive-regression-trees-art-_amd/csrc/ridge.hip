@@ -229,7 +229,6 @@ size_t leaf_stage_smem_bytes(int d) {
 // buffers.
 constexpr int kMixedRows = 256;
 constexpr int kMixedMaxD = 16;
-constexpr int kMixedFlush = 1;  // 32-row steps per FP32 accumulation
 __host__ __device__ inline int rows32_stride(int d) { return d + 3; }
 size_t mixed_stage_smem_bytes(int d) {
   const size_t f32 = (size_t)kMixedRows * 2 * rows32_stride(d) * sizeof(float);
@@ -767,23 +766,47 @@ __global__ __launch_bounds__(256) void reduce_leaf_partials_kernel(
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
-// Round two floats to bf16 (RNE) with one v_cvt_pk_bf16_f32; the asm barrier
-// keeps the compiler from fusing an earlier double->float conversion into a
-// double->bf16 one (which it emulates with a round-to-odd sequence).
+// Round two floats to bf16 (RNE) with one v_cvt_pk_bf16_f32.  The operands
+// are FP32 values (products of FP32 table entries), so no double->bf16
+// conversion (emulated by a round-to-odd sequence) can form; no asm barrier
+// is used, since a volatile one orders every LDS read behind the previous
+// pair's conversions.
 __device__ inline unsigned pack_bf16(float a, float b) {
-  asm volatile("" : "+v"(a), "+v"(b));
   const f32x2 v = {a, b};
   return __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2));
+}
+
+// x minus the low (c = (-1, 0)) or high (c = (0, -1)) bf16 of the packed
+// word u, in one v_dot2_f32_bf16: both products are exact and the remainders
+// taken here are exactly representable in FP32, so the result is exact.
+__device__ inline float sub_bf16(unsigned u, unsigned c, float x) {
+  return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, u),
+                                         __builtin_bit_cast(bf16x2, c), x, false);
+}
+
+// The bf16 pairs (-1, 0) and (0, -1) for sub_bf16, held in scalar
+// registers: this compiler encodes the (-1, 0) pair as the inline constant
+// -1.0, which v_dot2c_f32_bf16 reads as the FP32 bit pattern 0xbf800000, i.e.
+// the pair (0, -1).  The asm (not volatile, so free to be hoisted) hides the
+// value from constant folding.
+struct NegPairs {
+  unsigned low, high;
+};
+__device__ inline NegPairs neg_pairs() {
+  unsigned lo = 0x0000bf80u, hi = 0xbf800000u;
+  asm("" : "+s"(lo), "+s"(hi));
+  return NegPairs{lo, hi};
 }
 
 // Split a pair of floats into packed bf16 hi, mid and lo words:
 // v = hi + mid + lo + O(2^-27 v).  Each remainder v - hi (then r - mid) is
 // exact in FP32: it has at most 16 significant bits.
-__device__ inline void split3_pair(float a, float b, unsigned& hi, unsigned& mid, unsigned& lo) {
+__device__ inline void split3_pair(float a, float b, const NegPairs& c, unsigned& hi,
+                                   unsigned& mid, unsigned& lo) {
   hi = pack_bf16(a, b);
-  const float ra = a - __uint_as_float(hi << 16), rb = b - __uint_as_float(hi & 0xffff0000u);
+  const float ra = sub_bf16(hi, c.low, a), rb = sub_bf16(hi, c.high, b);
   mid = pack_bf16(ra, rb);
-  lo = pack_bf16(ra - __uint_as_float(mid << 16), rb - __uint_as_float(mid & 0xffff0000u));
+  lo = pack_bf16(sub_bf16(mid, c.low, ra), sub_bf16(mid, c.high, rb));
 }
 
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
@@ -797,13 +820,12 @@ typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 // an FP32 table: [x, 1, y, 0] and sqrt(w_floored) [x, 1, y, 0], so that a
 // feature value sqrt(w) x_a x_b is one product of a scaled and a plain entry.
 // Wave w then takes the 32-row steps s = w, w+4, ...: lane (col, kq) holds
-// rows 32 s + 4 e + kq (e < 8; the four row groups of one LDS read are
-// adjacent rows) of features 16 t + col as v = sqrt(w) phi, split
+// rows 32 s + 8 kq + e (e < 8) of features 16 t + col as v = sqrt(w) phi, split
 // v = hi + mid + lo in bf16, in the operand layout of
 // v_mfma_f32_16x16x32_bf16; both operands of every tile pair are these v.
 // A tile pair accumulates the six products down to 2^-16 (hi*lo, lo*hi,
 // mid*mid, hi*mid, mid*hi, hi*hi: Gram entries to ~1e-8 relative) in FP32
-// and is added into FP64 after every kMixedFlush steps.  The two-term split
+// and is added into FP64 after every 32-row step.  The two-term split
 // (~1e-5) sent five times as many leaves to the FP64 path.
 template <int T, int DC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DC > 0 ? 3 : 2))) void gram_leaf_mixed_kernel(
@@ -837,8 +859,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DC > 0 ? 3 
       fa[t] = f == B.p ? d + 1 : d + 2;
       fb[t] = d;
     }
-    fa[t] += kq * RS + S32;  // scaled half
-    fb[t] += kq * RS;        // plain half
+    fa[t] += S32;  // scaled half
   }
   f64x4 acc[NP];
   f32x4 z[NP];
@@ -859,8 +880,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DC > 0 ? 3 
       wr = w[row];
     }
   };
-  int bad = 0, pending = 0;
+  int bad = 0;
   double tr = 0.0;
+  const NegPairs neg = neg_pairs();
   int64_t r0 = begin;
   load_row(r0 + tid);
   for (; r0 < end; r0 += kMixedRows) {
@@ -899,19 +921,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DC > 0 ? 3 
     __syncthreads();
     const int steps = (nrows + 31) >> 5;
     for (int s = wave; s < steps; s += 4) {
-      const float* base = rows32 + 32 * s * RS;
+      // This lane's eight rows 32 s + 8 kq + e are adjacent, so all of its
+      // reads of a step are immediate offsets (< 8 RS dwords) from one
+      // address per factor.
+      const float* base = rows32 + (32 * s + 8 * kq) * RS;
       // Both operands are sqrt(w) phi (G = sum w phi phi^T is symmetric), so
-      // each value is split once.  Rows 8k + kq and 8k + 4 + kq share one
+      // each value is split once.  Rows 2k and 2k + 1 of the eight share one
       // packed word.
       u32x4 ahw[T], amw[T], alw[T];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
 #pragma unroll
         for (int t = 0; t < T; ++t) {
-          const float p0 = base[fa[t] + 8 * k * RS] * base[fb[t] + 8 * k * RS];
-          const float p1 = base[fa[t] + (8 * k + 4) * RS] * base[fb[t] + (8 * k + 4) * RS];
+          const float p0 = base[fa[t] + 2 * k * RS] * base[fb[t] + 2 * k * RS];
+          const float p1 = base[fa[t] + (2 * k + 1) * RS] * base[fb[t] + (2 * k + 1) * RS];
           unsigned hi, mid, lo;
-          split3_pair(p0, p1, hi, mid, lo);
+          split3_pair(p0, p1, neg, hi, mid, lo);
           ahw[t][k] = hi;
           amw[t][k] = mid;
           alw[t][k] = lo;
@@ -941,22 +966,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DC > 0 ? 3 
                                                              0, 0, 0);
         }
       }
-      if (++pending == kMixedFlush) {
+      // Into FP64 after every step: the FP32 sums are dead during the next
+      // step's operand construction, which keeps the kernel within the
+      // register budget of three waves per SIMD.
 #pragma unroll
-        for (int q = 0; q < NP; ++q)
+      for (int q = 0; q < NP; ++q)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            acc[q][r] += (double)z[q][r];
-            z[q][r] = 0.f;
-          }
-        pending = 0;
-      }
+        for (int r = 0; r < 4; ++r) {
+          acc[q][r] += (double)z[q][r];
+          z[q][r] = 0.f;
+        }
     }
   }
-#pragma unroll
-  for (int q = 0; q < NP; ++q)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) acc[q][r] += (double)z[q][r];
 
   double* red = smem;
   const size_t pbase = (size_t)c * NP;

@@ -340,7 +340,9 @@ def test_mixed_leaf_fits_within_1e6_of_fp64(torch_gpu, n_leaves, seed):
         np.testing.assert_allclose(lm.cpu().numpy(), l64.cpu().numpy(), rtol=1e-12)
         err = _rel_err(cm.cpu().numpy(), c64.cpu().numpy())
         assert err.max() <= 1e-6, err.max()
-        assert 0 <= batch.fallback_count() < n_leaves
+        # The split-bf16 Gram is accurate to ~1e-8, so only the worst-conditioned
+        # leaves (~1 %) need the FP64 path; a broken split sends most of them there.
+        assert 0 <= batch.fallback_count() <= n_leaves // 8, batch.fallback_count()
     ref, st_ref, _ = R.leaf_fit(wl.X, wl.y, wl.w, wl.offsets, wl.ridge)
     np.testing.assert_array_equal(sm.cpu().numpy(), st_ref)
     assert _rel_err(cm.cpu().numpy(), ref).max() <= 1e-6
