@@ -79,12 +79,14 @@ __device__ inline double readlane_f64(double v, int src) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Row range of one Gram work item (a chunk of one leaf's rows).
+// Row range of one Gram work item (a chunk of one leaf's rows).  The table is
+// launched largest chunk first; `slot` is the chunk's place in leaf order,
+// where its partial results go (a leaf's chunks have adjacent slots).
 struct LeafChunk {
   int64_t begin;
   int64_t end;
   int leaf;
-  int pad;
+  int slot;
 };
 
 // One ridge system: its chunks [chunk_begin, chunk_end), output slot and side.

@@ -108,6 +108,7 @@ struct TableChunks {  // varlen leaves: a host-built table
     begin = table[c].begin;
     end = table[c].end;
   }
+  __device__ int slot(int c) const { return table[c].slot; }
 };
 
 struct EqualChunks {  // equal-size nodes: active ids, fixed chunks per node
@@ -121,6 +122,7 @@ struct EqualChunks {  // equal-size nodes: active ids, fixed chunks per node
     begin = (int64_t)node * n + off;
     end = (int64_t)node * n + (off + rows < n ? off + rows : n);
   }
+  __device__ int slot(int c) const { return c; }
 };
 
 // ---------------------------------------------------------------------------
@@ -180,7 +182,8 @@ __global__ __launch_bounds__(256) void gram_f64_kernel(Src src, Chunks chunks, i
     }
   }
 
-  const size_t base = ((size_t)side * nchunks + c) * npairs;
+  const int slot = chunks.slot(c);
+  const size_t base = ((size_t)side * nchunks + slot) * npairs;
 #pragma unroll
   for (int q = 0; q < kGramMaxQ; ++q) {
     if (q < nq) {
@@ -194,7 +197,7 @@ __global__ __launch_bounds__(256) void gram_f64_kernel(Src src, Chunks chunks, i
     }
   }
   bad = __syncthreads_or(bad);
-  if (threadIdx.x == 0 && blockIdx.y == 0) flags[(size_t)side * nchunks + c] = bad;
+  if (threadIdx.x == 0 && blockIdx.y == 0) flags[(size_t)side * nchunks + slot] = bad;
 }
 
 // ---------------------------------------------------------------------------
@@ -372,7 +375,7 @@ __global__ __launch_bounds__(256) void gram_leaf_f64_kernel(
   }
 
   double* red = smem;
-  const size_t base = (size_t)c * NP;
+  const size_t base = (size_t)chunks[c].slot * NP;
 #pragma unroll
   for (int q = 0; q < NP; ++q) {
     __syncthreads();
@@ -383,7 +386,7 @@ __global__ __launch_bounds__(256) void gram_leaf_f64_kernel(
     partials[(base + q) * 256 + tid] = v;
   }
   bad = __syncthreads_or(bad);
-  if (tid == 0) flags[c] = bad;
+  if (tid == 0) flags[chunks[c].slot] = bad;
 }
 
 // ---------------------------------------------------------------------------
@@ -603,12 +606,16 @@ __global__ __launch_bounds__(128) void solve_rows_kernel(
   }
 }
 
-// The same contract for p <= PMAX <= 64: four waves reduce the chunk partials
-// into LDS (many loads in flight: the reduction is latency-bound), then wave 0
-// loads its rows into registers and factors and solves them there
-// (wave_cholesky, wave_solve: no barrier per column).
+// The same contract for p <= PMAX <= 64, one wave per system: the lanes
+// reduce the chunk partials into the packed system in LDS (two chunks' loads
+// in flight: the reduction is latency-bound), then load their rows into
+// registers and factor and solve them there (wave_cholesky, wave_forward,
+// wave_backward: no barrier per column).  Up to three tiles
+// (p <= 48) the kernel is held to 128 VGPRs, four waves per SIMD: one round
+// for 4096 leaves on 1024 SIMDs.  The few registers it then spills cost less
+// than a second round (solve of 4096 leaves at p = 45: 110 -> 83 us).
 template <int TT>
-__global__ __launch_bounds__(64) void solve_wave_kernel(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TT <= 3 ? 4 : 1))) void solve_wave_kernel(
     const SystemDesc* __restrict__ systems, int nchunks, int T, int npairs, int p, double ridge,
     const double* __restrict__ partials, const int* __restrict__ flags,
     const double* __restrict__ tpart, const int* __restrict__ mask, double* __restrict__ factor,
@@ -980,7 +987,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DC > 0 ? 3 
   }
 
   double* red = smem;
-  const size_t pbase = (size_t)c * NP;
+  const int slot = chunks[c].slot;
+  const size_t pbase = (size_t)slot * NP;
 #pragma unroll
   for (int q = 0; q < NP; ++q) {
     __syncthreads();
@@ -994,8 +1002,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DC > 0 ? 3 
   if (lane == 0) tred[wave] = tr;
   bad = __syncthreads_or(bad);
   if (tid == 0) {
-    flags[c] = bad;
-    tpart[c] = ((tred[0] + tred[1]) + tred[2]) + tred[3];
+    flags[slot] = bad;
+    tpart[slot] = ((tred[0] + tred[1]) + tred[2]) + tred[3];
   }
 }
 
@@ -1068,7 +1076,8 @@ __global__ __launch_bounds__(256) void residual_leaf_kernel(
   }
   __syncthreads();
   if (tid < 16 * T)
-    rpart[(size_t)c * 16 * T + tid] = ((rred[0][tid] + rred[1][tid]) + rred[2][tid]) + rred[3][tid];
+    rpart[(size_t)chunks[c].slot * 16 * T + tid] =
+        ((rred[0][tid] + rred[1][tid]) + rred[2][tid]) + rred[3][tid];
 }
 
 // Quadratic-basis residual, one row per thread.  With the input dimension D
@@ -1076,7 +1085,10 @@ __global__ __launch_bounds__(256) void residual_leaf_kernel(
 //   phi . beta = b0 + sum_i x_i (b_i + sum_{j>=i} b_ij x_j),
 //   r += e [1, x_i, x_i x_j]   (e = w (y - phi . beta)),
 // cost about (D + 1)(D + 2) FP64 FMAs per row, with no cross-lane traffic
-// until the chunk ends.  Chunks here are up to kResidualChunkRows long.
+// until the chunk ends.  Chunks here are up to kResidualChunkRows long.  The
+// coefficients are uniform over the workgroup and come in by scalar loads
+// (scalar registers), and each thread's next row is loaded while the current
+// one is processed, so that a wave keeps two rows per lane in flight.
 constexpr int kResidualChunkRows = 8192;
 
 // Wave reduce-scatter of N <= 64 values per lane (entries >= N count as 0):
@@ -1117,35 +1129,43 @@ __global__ __launch_bounds__(256) void residual_quad_kernel(
   const int c = blockIdx.x;
   const int leaf = rchunks[c].leaf;
   if (status[leaf] != 0 || (mask && !mask[leaf])) return;
-  __shared__ double bs[NF];
   __shared__ double red[4][64];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int bias = B.bias, p = B.p;
-  if (tid < NF) {
-    const int f = tid - 1 + bias;  // coefficient index in the leaf's order
-    bs[tid] = (tid == 0 && !bias) ? 0.0 : beta[(size_t)leaf * p + f];
-  }
-  __syncthreads();
+  // bl[i] (linear) and bl[D + k] (quadratic) follow the optional bias.
+  const double* bl = beta + (size_t)leaf * p + bias;
+  const double b0 = bias ? bl[-1] : 0.0;
   double acc[NF];
 #pragma unroll
   for (int k = 0; k < NF; ++k) acc[k] = 0.0;
   const int64_t begin = rchunks[c].begin, end = rchunks[c].end;
+  double xn[D], wn = 0.0, yn = 0.0;
+  auto load_row = [&](int64_t row) {
+    if (row < end) {
+#pragma unroll
+      for (int i = 0; i < D; ++i) xn[i] = X[row * D + i];
+      wn = w[row];
+      yn = y[row];
+    }
+  };
+  load_row(begin + tid);
   for (int64_t row = begin + tid; row < end; row += 256) {
     double x[D];
 #pragma unroll
-    for (int i = 0; i < D; ++i) x[i] = X[row * D + i];
-    const double wv = w[row];
+    for (int i = 0; i < D; ++i) x[i] = xn[i];
+    const double wv = wn, yv = yn;
+    load_row(row + 256);
     const double wt = ((wv >= 0.0) & !isinf(wv)) ? fmax(wv, floor) : 0.0;
-    double dot = bs[0];
+    double dot = b0;
     int k = 0;
 #pragma unroll
     for (int i = 0; i < D; ++i) {
-      double u = bs[1 + i];
+      double u = bl[i];
 #pragma unroll
-      for (int j = i; j < D; ++j, ++k) u += bs[1 + D + k] * x[j];
+      for (int j = i; j < D; ++j, ++k) u += bl[D + k] * x[j];
       dot += x[i] * u;
     }
-    const double e = wt * (y[row] - dot);
+    const double e = wt * (yv - dot);
     acc[0] += e;
     k = 0;
 #pragma unroll
@@ -1161,7 +1181,7 @@ __global__ __launch_bounds__(256) void residual_quad_kernel(
   __syncthreads();
   if (tid < NF) {
     const double v = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
-    if (tid > 0 || bias) rpart[(size_t)c * NF + tid - 1 + bias] = v;
+    if (tid > 0 || bias) rpart[(size_t)rchunks[c].slot * NF + tid - 1 + bias] = v;
   }
 }
 
@@ -1409,6 +1429,14 @@ static int plan_alloc_common(art_ridge_plan* P, int sides) {
   return ART_OK;
 }
 
+// Sort a chunk table (built in leaf order, slot = index) into its launch
+// order: longest first, ties in slot order.
+static void launch_order(std::vector<LeafChunk>& table) {
+  std::stable_sort(table.begin(), table.end(), [](const LeafChunk& a, const LeafChunk& b) {
+    return a.end - a.begin > b.end - b.begin;
+  });
+}
+
 int art_leaf_plan_create(const int64_t* offsets, int K, int d, int degree, int include_bias,
                          int bias_last, int chunk_rows, art_ridge_plan** out) {
   if (out == nullptr || offsets == nullptr || K < 1 || d < 1)
@@ -1437,7 +1465,8 @@ int art_leaf_plan_create(const int64_t* offsets, int K, int d, int degree, int i
     }
     sys[j].chunk_begin = (int)chunks.size();
     for (int64_t r = b; r < e; r += chunk_rows)
-      chunks.push_back(LeafChunk{r, r + chunk_rows < e ? r + chunk_rows : e, j, 0});
+      chunks.push_back(
+          LeafChunk{r, r + chunk_rows < e ? r + chunk_rows : e, j, (int)chunks.size()});
     sys[j].chunk_end = (int)chunks.size();
     sys[j].out = j;
     sys[j].side = 0;
@@ -1453,12 +1482,16 @@ int art_leaf_plan_create(const int64_t* offsets, int K, int d, int degree, int i
     for (int64_t r = offsets[j]; r < offsets[j + 1]; r += kResidualChunkRows)
       rchunks.push_back(LeafChunk{r, r + kResidualChunkRows < offsets[j + 1] ? r + kResidualChunkRows
                                                                           : offsets[j + 1],
-                                  j, 0});
+                                  j, (int)rchunks.size()});
   }
   goff[K] = sys[K - 1].chunk_end;
   roff[K] = (int)rchunks.size();
   if (rchunks.empty()) rchunks.push_back(LeafChunk{0, 0, 0, 0});
   P->nrchunks = (int)rchunks.size();
+  // Launch order: largest chunk first, so that the long chunks do not start
+  // last and leave a tail (rows differ by up to 256x between leaves).
+  launch_order(chunks);
+  launch_order(rchunks);
   if (hipMalloc(&P->d_goff, (K + 1) * sizeof(int)) != hipSuccess ||
       hipMalloc(&P->d_roff, (K + 1) * sizeof(int)) != hipSuccess ||
       hipMalloc(&P->d_rchunks, rchunks.size() * sizeof(LeafChunk)) != hipSuccess ||
