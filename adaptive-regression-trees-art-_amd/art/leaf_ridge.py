@@ -19,6 +19,7 @@ from . import _hip
 from .features import _columns
 
 PRECISIONS = {"fp64": _hip.PRECISION_FP64, "mixed": _hip.PRECISION_MIXED}
+DEFAULT_CHUNK_ROWS = 4096  # the library's default for leaf plans (chunk_rows=0)
 
 
 class LeafRidgeBatch:

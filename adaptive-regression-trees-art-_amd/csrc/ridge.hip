@@ -1444,7 +1444,7 @@ int art_leaf_plan_create(const int64_t* offsets, int K, int d, int degree, int i
   *out = nullptr;
   if ((degree != 1 && degree != 2) || (bias_last && degree != 1))
     return fail(ART_ERR_ARG, "art_leaf_plan_create: degree must be 1 or 2");
-  if (chunk_rows < 16 || chunk_rows % 16) chunk_rows = ART_DEFAULT_CHUNK_ROWS;
+  if (chunk_rows < 16 || chunk_rows % 16) chunk_rows = ART_DEFAULT_LEAF_CHUNK_ROWS;
   art_ridge_plan* P = new (std::nothrow) art_ridge_plan();
   if (!P) return fail(ART_ERR_ALLOC, "art_leaf_plan_create: out of host memory");
   P->kind = 0;

@@ -558,7 +558,7 @@ def main():
         sys.exit(2)
     import torch
 
-    from art import LeafRidgeBatch
+    from art import LeafRidgeBatch, leaf_ridge
     from art.synthetic import leaf_ridge_workload
 
     if not torch.cuda.is_available():
@@ -640,7 +640,7 @@ def main():
             "workload": ("batched weighted ridge: 4096 leaves, d=8, quadratic p=45, "
                          "log-uniform 64..16384 rows, ridge 1e-6, seed 2"),
             "leaves": K, "rows": N, "p": p, "precision": args.precision,
-            "chunk_rows": args.chunk_rows or 2048, "chunks": batch.n_chunks,
+            "chunk_rows": args.chunk_rows or leaf_ridge.DEFAULT_CHUNK_ROWS, "chunks": batch.n_chunks,
         },
         "latency_ms": {"p50": float(np.percentile(run["step_ms"], 50)),
                        "p99": float(np.percentile(run["step_ms"], 99)),

@@ -30,7 +30,8 @@ extern "C" {
 #define ART_PRECISION_FP64 0  /* FP64 Gram + FP64 Cholesky ("plain") */
 #define ART_PRECISION_MIXED 1 /* low-precision MFMA Gram + FP64 refinement */
 
-#define ART_DEFAULT_CHUNK_ROWS 2048
+#define ART_DEFAULT_CHUNK_ROWS 2048      /* equal-size node plans */
+#define ART_DEFAULT_LEAF_CHUNK_ROWS 4096 /* uneven leaf plans (launched largest chunk first) */
 #define ART_MIXED_REFINE_TOL 1e-4 /* accept |delta| <= tol*|beta| after one refinement */
 #define ART_MIXED_REFINE_STEPS 1  /* further refinement steps on still-flagged leaves */
 #define ART_SOLVE_LDS_BYTES 65536
