@@ -902,7 +902,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DC > 0 ? 3 
         const bool ok = (wr >= 0.0) & !isinf(wr);
         bad |= !ok;
         const double wt = ok ? fmax(wr, floor) : 0.0;
-        const double sw = sqrt(wt);
+        // The scaled half only feeds FP32 values: sqrt(w) by v_sqrt_f32 (1 ulp)
+        // and FP32 products; the trace stays FP64.
+        const float sw = __builtin_amdgcn_sqrtf((float)wt);
         double s1 = 0.0, s2 = 0.0;
 #pragma unroll
         for (int i = 0; i < MD; ++i)
@@ -910,15 +912,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DC > 0 ? 3 
             const double x2 = xr[i] * xr[i];
             s1 += x2;
             s2 += x2 * x2;
-            row[i] = (float)xr[i];
-            srow[i] = (float)(sw * xr[i]);
+            const float xf = (float)xr[i];
+            row[i] = xf;
+            srow[i] = sw * xf;
           }
         tr += wt * (bias + s1 + quad * (s1 * s1 + s2));
+        const float yf = (float)yr;
         row[d] = 1.0f;
-        row[d + 1] = (float)yr;
+        row[d + 1] = yf;
         row[d + 2] = 0.0f;
-        srow[d] = (float)sw;
-        srow[d + 1] = (float)(sw * yr);
+        srow[d] = sw;
+        srow[d + 1] = sw * yf;
         srow[d + 2] = 0.0f;
       } else {
         for (int i = 0; i < RS; ++i) row[i] = 0.0f;
