@@ -74,7 +74,8 @@ int art_check_finite(const double* a, const double* y, int64_t groups, int64_t r
 
 /* Variable-length leaves.  offsets (K+1, host memory) delimit leaf j's rows
  * [offsets[j], offsets[j+1]).  Allocates device buffers (synchronous).
- * chunk_rows: rows per Gram work item (multiple of 16; 0 = default). */
+ * chunk_rows: rows per Gram work item (multiple of 16; 0 =
+ * ART_DEFAULT_LEAF_CHUNK_ROWS).  Work items are launched largest first. */
 int art_leaf_plan_create(const int64_t* offsets, int K, int d, int degree, int include_bias,
                          int bias_last, int chunk_rows, art_ridge_plan** plan);
 
