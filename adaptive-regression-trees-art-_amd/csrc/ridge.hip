@@ -1212,11 +1212,27 @@ __global__ __launch_bounds__(64) void correct_wave_kernel(
     if (lane == 0) need[leaf] = 1;
     return;
   }
-  const double* F = factor + (size_t)leaf * (p * (p + 1) / 2);
+  // The packed factor comes into LDS in one pass of coalesced loads (all in
+  // flight together); its rows and then its columns are read from there.
+  extern __shared__ double Fs[];
+  {
+    const int nf = p * (p + 1) / 2;
+    const double* F = factor + (size_t)leaf * nf;
+    constexpr int NV = (PMAX * (PMAX + 1) / 2 + 63) / 64;
+    double v[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] = lane + 64 * i < nf ? F[lane + 64 * i] : 0.0;
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+      if (lane + 64 * i < nf) Fs[lane + 64 * i] = v[i];
+    // One wave writes and reads Fs: its LDS operations complete in order; the
+    // fence keeps the compiler from moving the reads up.
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
   double L[PMAX];
 #pragma unroll
-  for (int j = 0; j < PMAX; ++j) L[j] = (j <= lane && lane < p) ? F[tri(lane, j)] : 0.0;
-  const double dinv = lane < p ? 1.0 / F[tri(lane, lane)] : 1.0;
+  for (int j = 0; j < PMAX; ++j) L[j] = (j <= lane && lane < p) ? Fs[tri(lane, j)] : 0.0;
+  const double dinv = lane < p ? 1.0 / Fs[tri(lane, lane)] : 1.0;
   double r = 0.0, b0 = 0.0;
   if (lane < p) {
     for (int ch = roff[leaf]; ch < roff[leaf + 1]; ++ch) r += rpart[(size_t)ch * rstride + lane];
@@ -1225,7 +1241,7 @@ __global__ __launch_bounds__(64) void correct_wave_kernel(
   }
   const double z = wave_forward<PMAX>(L, dinv, p, lane, r);
 #pragma unroll
-  for (int k = 0; k < PMAX; ++k) L[k] = (k > lane && k < p) ? F[tri(k, lane)] : 0.0;
+  for (int k = 0; k < PMAX; ++k) L[k] = (k > lane && k < p) ? Fs[tri(k, lane)] : 0.0;
   const double delta = wave_backward<PMAX>(L, dinv, p, lane, z);
   const double b1 = b0 + delta;
   const double nd = wave_sum(lane < p ? delta * delta : 0.0);
@@ -1712,7 +1728,8 @@ static int leaf_fit_mixed(art_ridge_plan* P, const double* X, const double* y, c
       const int* ro = quad ? P->d_roff : P->d_goff;
       const int rs = quad ? p : 16 * T;
 #define ART_CORRECT(PM)                                                                     \
-  correct_wave_kernel<PM><<<sgrid, dim3(64), 0, stream>>>(                                  \
+  correct_wave_kernel<PM><<<sgrid, dim3(64), (size_t)p * (p + 1) / 2 * sizeof(double),        \
+                            stream>>>(                                                          \
       P->d_sys, ro, p, rs, P->d_rpart, P->d_factor, lambda, beta, status, P->d_need,        \
       P->refine_tol, mask, P->d_dnorm)
       if (p <= 16) ART_CORRECT(16);
