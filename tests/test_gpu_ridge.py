@@ -348,6 +348,29 @@ def test_mixed_leaf_fits_within_1e6_of_fp64(torch_gpu, n_leaves, seed):
     assert _rel_err(cm.cpu().numpy(), ref).max() <= 1e-6
 
 
+@pytest.mark.parametrize("d", [5, 9])
+def test_mixed_fits_other_basis_sizes(torch_gpu, d):
+    """Quadratic bases of p = 21 and p = 55 (the 32- and 64-column solve and
+    correction kernels): mixed within 1e-6 of FP64 on every leaf."""
+    from art import LeafRidgeBatch
+
+    torch, rng = torch_gpu, np.random.default_rng(30 + d)
+    sizes = rng.integers(300, 3000, size=48)
+    offsets = np.concatenate([[0], np.cumsum(sizes)])
+    N = offsets[-1]
+    X = rng.uniform(-1, 1, size=(N, d))
+    y = np.sin(X @ rng.normal(size=d)) + 0.01 * rng.normal(size=N)
+    w = rng.uniform(0.2, 1.0, size=N)
+    Xd, yd, wd = dev(torch, X), dev(torch, y), dev(torch, w)
+    with LeafRidgeBatch(offsets, d) as batch:
+        c64, s64, _ = (t.clone() for t in batch.fit(Xd, yd, wd, 1e-6, precision="fp64"))
+        cm, sm, _ = batch.fit(Xd, yd, wd, 1e-6, precision="mixed")
+        np.testing.assert_array_equal(sm.cpu().numpy(), s64.cpu().numpy())
+        assert (s64.cpu().numpy() == 0).all()
+        err = _rel_err(cm.cpu().numpy(), c64.cpu().numpy())
+        assert err.max() <= 1e-6, err.max()
+
+
 def test_mixed_edge_leaves_match_fp64(torch_gpu):
     """Empty, one-row, flagged and tiny-weight leaves take the exact path."""
     from art import LeafRidgeBatch
