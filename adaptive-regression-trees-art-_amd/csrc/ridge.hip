@@ -1183,9 +1183,11 @@ __global__ __launch_bounds__(256) void residual_quad_kernel(
   const double total = reduce_scatter<NF>(acc, lane);
   red[wave][lane] = total;
   __syncthreads();
+  // Slot stride p (the basis width, NF - 1 + bias): the correction kernel
+  // reads rpart[chunk * p + feature].
   if (tid < NF) {
     const double v = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
-    if (tid > 0 || bias) rpart[(size_t)rchunks[c].slot * NF + tid - 1 + bias] = v;
+    if (tid > 0 || bias) rpart[(size_t)rchunks[c].slot * p + tid - 1 + bias] = v;
   }
 }
 

@@ -150,15 +150,12 @@ __device__ __forceinline__ double rcp2(double v) {
   return r;
 }
 
-// DM = the compile-time dimension; EXACT: d == DM (no per-column predicates).
+// Chain c of polytope poly, one lane: DM = the compile-time dimension;
+// EXACT: d == DM (no per-column predicates).
 template <int DM, bool EXACT>
-__global__ __launch_bounds__(64) void har_kernel(art_har_args a) {
+__device__ __attribute__((always_inline)) inline void har_chain_lane(const art_har_args& a, int poly, int c) {
   const int d = EXACT ? DM : a.d;
   const int C = a.chains;
-  const int wpp = (C + 63) / 64;
-  const int poly = blockIdx.x / wpp;
-  const int c = (blockIdx.x - poly * wpp) * 64 + (int)threadIdx.x;
-  if (c >= C) return;
   ci64* offs = (ci64*)a.row_offsets;
   const int64_t r0 = offs[poly];
   const int m = (int)(offs[poly + 1] - r0);
@@ -237,6 +234,15 @@ __global__ __launch_bounds__(64) void har_kernel(art_har_args a) {
         if (EXACT || j < d) o[j] = x[j];
     }
   }
+}
+
+// A lane per chain, a wave per 64 chains of one polytope.
+template <int DM, bool EXACT>
+__global__ __launch_bounds__(64) void har_kernel(art_har_args a) {
+  const int wpp = (a.chains + 63) / 64;
+  const int poly = blockIdx.x / wpp;
+  const int c = (blockIdx.x - poly * wpp) * 64 + (int)threadIdx.x;
+  if (c < a.chains) har_chain_lane<DM, EXACT>(a, poly, c);
 }
 
 // One chain per polytope (the scalar sampler): the wave splits each block
@@ -396,9 +402,16 @@ __device__ __attribute__((always_inline)) inline void chain_run(const art_har_ar
 template <int DM, bool EXACT>
 __global__ __launch_bounds__(64) void har_chain_kernel(art_har_args a, int E) {
   extern __shared__ double chain_lds[];
+  ci64* offs = (ci64*)a.row_offsets;
+  const int m = (int)(offs[blockIdx.x + 1] - offs[blockIdx.x]);
+  // A move's LDS entry holds 2 (E - 4 DM - 1) / 2 row values: a polytope
+  // with more rows than the launch's row bound replays its chain on one lane
+  // by the per-chain code (the same chain), instead of overrunning LDS.
+  if (m > (E - 4 * DM - 1) / 2) {
+    if (threadIdx.x == 0) har_chain_lane<DM, EXACT>(a, (int)blockIdx.x, 0);
+    return;
+  }
   if constexpr (DM <= 4) {
-    ci64* offs = (ci64*)a.row_offsets;
-    const int m = (int)(offs[blockIdx.x + 1] - offs[blockIdx.x]);
     switch (m) {
 #define ART_ROWS(MM) \
   case MM:           \

@@ -371,6 +371,46 @@ def test_mixed_fits_other_basis_sizes(torch_gpu, d):
         assert err.max() <= 1e-6, err.max()
 
 
+@pytest.mark.parametrize("d,degree,bias,last,chunk", [
+    (3, 2, False, False, 0), (8, 2, False, False, 0), (8, 2, False, False, 16384),
+    (3, 2, True, False, 16384),
+    (3, 1, True, False, 0), (12, 1, True, False, 0), (16, 1, True, False, 0),
+    (3, 1, True, True, 0), (12, 1, True, True, 0), (16, 1, True, True, 0),
+    (3, 1, False, False, 0), (12, 1, False, False, 0), (16, 1, False, False, 0),
+])
+def test_mixed_fits_every_basis_layout(torch_gpu, d, degree, bias, last, chunk):
+    """Mixed within 1e-6 of FP64 for quadratic bases without a bias (the
+    register residual kernel, whose partials have stride p) and every linear
+    layout (bias first, bias last, none: the feature-layout residual kernel),
+    with one leaf above 8192 rows (several residual chunks) and, in two
+    cases, 16384-row Gram chunks."""
+    from art import LeafRidgeBatch
+
+    torch, rng = torch_gpu, np.random.default_rng(400 + 7 * d + degree + 2 * bias + last)
+    sizes = np.concatenate([rng.integers(200, 3000, size=20), [9000, 17000, 64]])
+    offsets = np.concatenate([[0], np.cumsum(sizes)])
+    N = offsets[-1]
+    X = rng.uniform(-1, 1, size=(N, d))
+    y = np.sin(X @ rng.normal(size=d)) + 0.3 * X[:, 0] ** 2 + 0.01 * rng.normal(size=N)
+    w = rng.uniform(0.2, 1.0, size=N)
+    Xd, yd, wd = dev(torch, X), dev(torch, y), dev(torch, w)
+    with LeafRidgeBatch(offsets, d, degree=degree, include_bias=bias, bias_last=last,
+                        chunk_rows=chunk) as batch:
+        c64, s64, l64 = (t.clone() for t in batch.fit(Xd, yd, wd, 1e-6, precision="fp64"))
+        cm, sm, lm = batch.fit(Xd, yd, wd, 1e-6, precision="mixed")
+        np.testing.assert_array_equal(sm.cpu().numpy(), s64.cpu().numpy())
+        assert (s64.cpu().numpy() == 0).all()
+        np.testing.assert_allclose(lm.cpu().numpy(), l64.cpu().numpy(), rtol=1e-12)
+        err = _rel_err(cm.cpu().numpy(), c64.cpu().numpy())
+        assert err.max() <= 1e-6, err.max()
+        # Well-conditioned leaves: a wrong residual sends most of them to the
+        # FP64 path (or, unflagged, past the bound above).
+        assert batch.fallback_count() <= len(sizes) // 4, batch.fallback_count()
+    ref, st_ref, _ = R.leaf_fit(X, y, w, offsets, 1e-6, 1e-12, degree, bias, last)
+    np.testing.assert_array_equal(sm.cpu().numpy(), st_ref)
+    assert _rel_err(cm.cpu().numpy(), ref).max() <= 1e-6
+
+
 def test_mixed_edge_leaves_match_fp64(torch_gpu):
     """Empty, one-row, flagged and tiny-weight leaves take the exact path."""
     from art import LeafRidgeBatch

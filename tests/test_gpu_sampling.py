@@ -271,3 +271,26 @@ def test_single_chain_with_cuts_matches_the_scalar_oracle(torch_gpu):
     for p, (A, b, star) in enumerate(regs):
         exp = S.scalar_sample(A, b, star, n, burn, thin, dirs[p, :, 0], us[p, :, 0])
         np.testing.assert_allclose(got[p], exp, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("d,k", [(2, 10), (12, 30)])
+def test_single_chain_understated_row_bound_replays_safely(torch_gpu, d, k):
+    """art_har_sample's row_bound sizes the single-chain kernel's LDS entries;
+    a polytope with more rows than the bound replays its chain on the
+    lane-per-chain code, so the samples equal those of the exact bound."""
+    from art.sampling import PolytopeBatch
+
+    rng = np.random.default_rng(500 + d)
+    regs = [_cut_box(rng, d, kk) for kk in (1, k, 3)]
+    batch = PolytopeBatch(([r[0] for r in regs], [r[1] for r in regs]))
+    starts = np.stack([r[2] for r in regs])[:, None]
+    exact = batch.sample(starts, 40, first=3, stride=4, seed=9).cpu().numpy()
+    true_rows = batch.max_rows
+    batch.max_rows = 2  # the kernel is told at most 2 general rows
+    try:
+        low = batch.sample(starts, 40, first=3, stride=4, seed=9).cpu().numpy()
+    finally:
+        batch.max_rows = true_rows
+    np.testing.assert_array_equal(low, exact)
+    for (A, b, _), pts in zip(regs, low):
+        assert np.all(pts @ A.T <= b + 1e-9)
