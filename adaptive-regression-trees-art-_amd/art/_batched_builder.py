@@ -1,12 +1,14 @@
 """FIFO cohort scheduling of a tree build (art/_batched_builder.py:20-62).
 
-Jobs leave the queue oldest first.  Up to node_batch_size splittable nodes
-form a cohort: the queue hands the builder as many jobs at a time as there
-are free cohort slots, and the builder prepares them together (sampling in
-queue order, then one batched GPU model fit); terminal nodes are attached at
-once and free their slot again.  A cohort is split by ``split_batch`` (one
-GPU launch for the batched optimizer) or node by node, and its results are
-attached in input order, each split's children queued left then right.
+A ``_Frontier`` holds the queue of node jobs, oldest first, and the place
+every finished node goes (the root slot, or its parent's child field).  A
+cohort is filled with up to node_batch_size jobs that need a split: the
+frontier hands the builder as many jobs at a time as there are free cohort
+places, and the builder prepares them together (sampling in queue order,
+then one batched GPU model fit); jobs that end as leaves are placed at once
+and free their place again.  The cohort is then split by ``split_batch``
+(one GPU launch per attempt round) or node by node, and its nodes are placed
+in input order, each split's children queued left then right.
 """
 
 from __future__ import annotations
@@ -20,38 +22,53 @@ if TYPE_CHECKING:
     from .builder import RegressionTreeBuilder, _NodeWork
 
 
-def build_bfs(builder: RegressionTreeBuilder, work: _NodeWork,
-              split_batch: Callable[[list[_NodeWork]], list[TreeNode]] | None = None) -> TreeNode:
-    """Build breadth-first from the root job and return the root node."""
-    pending = deque([work])
-    slot = {"root": None}
+class _Frontier:
+    """The build's job queue and the slots its finished nodes fill."""
 
-    def attach(job, node):
+    def __init__(self, root_job):
+        self.queue = deque([root_job])
+        self.root = None
+
+    def place(self, job, node):
         if job.parent is None:
-            slot["root"] = node
+            self.root = node
         else:
             setattr(job.parent, job.side, node)
 
-    while pending:
-        ready = []
-        while pending and len(ready) < builder.node_batch_size:
-            take = min(builder.node_batch_size - len(ready), len(pending))
-            jobs = [pending.popleft() for _ in range(take)]
-            for job, leaf in zip(jobs, builder._prepare_cohort(jobs)):
+    def next_cohort(self, builder) -> list:
+        """Jobs (oldest first) that still need a split, at most
+        node_batch_size; prepared in batches of the free places."""
+        limit = builder.node_batch_size
+        cohort = []
+        while self.queue and len(cohort) < limit:
+            take = min(limit - len(cohort), len(self.queue))
+            batch = [self.queue.popleft() for _ in range(take)]
+            for job, leaf in zip(batch, builder._prepare_cohort(batch)):
                 if leaf is None:
-                    ready.append(job)
+                    cohort.append(job)
                 else:
-                    attach(job, leaf)
-        if not ready:
+                    self.place(job, leaf)
+        return cohort
+
+
+def build_bfs(builder: RegressionTreeBuilder, work: _NodeWork,
+              split_batch: Callable[[list[_NodeWork]], list[TreeNode]] | None = None) -> TreeNode:
+    """Build breadth-first from the root job and return the root node."""
+    frontier = _Frontier(work)
+    if split_batch is None:
+        def split_batch(jobs):
+            return [builder._split_node(job) for job in jobs]
+    while frontier.queue:
+        cohort = frontier.next_cohort(builder)
+        if not cohort:
             continue
-        nodes = ([builder._split_node(job) for job in ready] if split_batch is None
-                 else split_batch(ready))
-        if len(nodes) != len(ready):
+        nodes = split_batch(cohort)
+        if len(nodes) != len(cohort):
             raise RuntimeError("Split dispatch must return one node per ready job.")
-        for job, node in zip(ready, nodes):
-            attach(job, node)
+        for job, node in zip(cohort, nodes):
+            frontier.place(job, node)
             if isinstance(node, SplitNode):
-                pending.extend(builder._children(job, node))
-    if slot["root"] is None:
+                frontier.queue.extend(builder._children(job, node))
+    if frontier.root is None:
         raise RuntimeError("The build queue finished without a root node.")
-    return slot["root"]
+    return frontier.root

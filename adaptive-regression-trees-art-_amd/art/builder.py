@@ -95,30 +95,51 @@ class _NodeWork:
     prepared: Any = None
 
 
+# Outcomes that earn another attempt while the node's retry budget lasts:
+# split failures by reason, accepted results by their rejection reason.
+_RETRY_FAILURES = frozenset({"min_side_points"})
+_RETRY_REJECTIONS = frozenset({"insufficient_split_gain", "insufficient_relative_split_gain"})
+
+
 @dataclass
-class _SplitState:
-    """Temperature and retry bookkeeping of one node's split (art/builder.py:86-113)."""
+class _SplitLedger:
+    """One node's split attempts (art/builder.py:86-113 keeps the same facts).
+
+    ``budget`` counts the retries still allowed; the min-side fallback is
+    switched on for the attempt made when none remain.  Every attempt's
+    failure or rejection reason is logged in order.
+    """
 
     temperature: float | None
     context: dict[str, object]
-    limit: int
-    min_side_fallback: bool
-    retries: int = 0
-    reasons: list[str] = field(default_factory=list)
+    budget: int
+    fallback_enabled: bool
+    spent: int = 0
+    log: list[str] = field(default_factory=list)
 
     @property
     def fallback(self):
-        return self.min_side_fallback and self.retries >= self.limit
+        return self.fallback_enabled and self.budget <= 0
 
-    @property
-    def metadata(self):
-        return {"restarts_on_failure": self.retries,
-                "split_attempt_failure_reasons": tuple(self.reasons)}
+    def settle(self, reason, retryable):
+        """Log an attempt's reason; True when it is retried."""
+        if reason is not None:
+            self.log.append(reason)
+        if not retryable or self.budget <= 0:
+            return False
+        self.budget -= 1
+        self.spent += 1
+        return True
 
-    def error(self, exc):
+    def summary(self):
+        return {"restarts_on_failure": self.spent,
+                "split_attempt_failure_reasons": tuple(self.log)}
+
+    def failure(self, exc):
+        """The node's final SplitNotFoundError, carrying the whole ledger."""
         return SplitNotFoundError(exc.reason, str(exc), diagnostics=exc.diagnostics,
-                                  restarts_on_failure=self.retries,
-                                  failure_reasons=tuple(self.reasons), context=self.context)
+                                  restarts_on_failure=self.spent,
+                                  failure_reasons=tuple(self.log), context=self.context)
 
 
 @dataclass
@@ -385,23 +406,25 @@ class RegressionTreeBuilder:
         return out
 
     def _gpu_fits(self):
-        return isinstance(self.model_template, (AffineRidgeModel, PolynomialRidgeModel))
+        """The GPU ridge models themselves (not subclasses, whose own fit is
+        called) are fitted a cohort at a time."""
+        return type(self.model_template) in (AffineRidgeModel, PolynomialRidgeModel)
 
     def _fit_requests(self, requests: list[_FitRequest]):
         """(model, predictions on eval_X) per request; GPU ridge templates fit
-        every request in one launch."""
+        every request in one launch.  Node fits are not timed as the split
+        optimizer's model refits (they count as other build time)."""
         if not requests:
             return []
-        with self._timing_context("model_refit"):
-            if self._gpu_fits():
-                models, _ = fit_models(self.model_template, [(r.X, r.y) for r in requests])
-                return [(m, m.predict(r.eval_X)) for m, r in zip(models, requests)]
-            out = []
-            for r in requests:
-                m = self.model_template.clone()
-                m.fit(r.X, r.y)
-                out.append((m, np.asarray(m.predict(r.eval_X), dtype=float).reshape(-1)))
-            return out
+        if self._gpu_fits():
+            models, _ = fit_models(self.model_template, [(r.X, r.y) for r in requests])
+            return [(m, m.predict(r.eval_X)) for m, r in zip(models, requests)]
+        out = []
+        for r in requests:
+            m = self.model_template.clone()
+            m.fit(r.X, r.y)
+            out.append((m, np.asarray(m.predict(r.eval_X), dtype=float).reshape(-1)))
+        return out
 
     def _fit_node_model(self, X, y, prepared_design=None):
         """One node fit: (model, predictions on X) (art/builder.py:812-829)."""
@@ -419,13 +442,17 @@ class RegressionTreeBuilder:
     # ------------------------------------------------------------ splitting
     def _split_node(self, work: _NodeWork) -> TreeNode:
         """Split one node with its retries, then finish it."""
+        X, y = work.samples.X, work.samples.y
         try:
-            result, temperature, retry = self._fit_split(
-                work.samples.X, work.samples.y, work.model, work.parent_mse, work.depth,
-                work.prepared)
+            ledger = self._open_ledger(X, y, work.depth)
         except SplitNotFoundError as exc:
             return self._finish_node(work, exc)
-        return self._finish_node(work, result, {**temperature, **retry})
+        outcome = self._attempt_split(ledger, X, y, work.model, work.parent_mse)
+        while self._settle_attempt(ledger, outcome):
+            outcome = self._attempt_split(ledger, X, y, work.model, work.parent_mse)
+        if isinstance(outcome, SplitNotFoundError):
+            return self._finish_node(work, ledger.failure(outcome))
+        return self._finish_node(work, outcome, {**ledger.context, **ledger.summary()})
 
     def _finish_node(self, work, result, context=None) -> TreeNode:
         """Turn a final split outcome into a leaf or a split with pending children."""
@@ -642,62 +669,61 @@ class RegressionTreeBuilder:
         return _NodeSamples(np.vstack([X, X_new]), np.concatenate([y, y_new]), kept, missing,
                             info)
 
-    def _fit_split(self, X, y, parent_model, parent_mse, depth, prepared_design):
-        state = self._split_state(X, y, depth, prepared_design)
-        while True:
-            try:
-                result = self._run_splitter(X, y, parent_model, parent_mse, state.temperature,
-                                            prepared_design, min_side_fallback=state.fallback)
-            except SplitNotFoundError as exc:
-                result = exc
-            if self._retry_split(state, result):
-                continue
-            if isinstance(result, SplitNotFoundError):
-                raise state.error(result) from result
-            return result, state.context, state.metadata
+    def _attempt_split(self, ledger, X, y, parent_model, parent_mse):
+        """One run of the splitter under the ledger's temperature and fallback
+        setting: a SplitResult, or the SplitNotFoundError it raised."""
+        try:
+            return self._run_splitter(X, y, parent_model, parent_mse, ledger.temperature, None,
+                                      min_side_fallback=ledger.fallback)
+        except SplitNotFoundError as exc:
+            return exc
 
-    def _split_state(self, X, y, depth, prepared_design) -> _SplitState:
-        temperature, context, tuned = self._resolve_split_temperature(X, y, depth,
-                                                                      prepared_design)
-        return _SplitState(temperature, context, 0 if tuned else self.max_retries_on_failure,
-                           bool(getattr(self.splitter, "min_side_fallback", False)))
+    def _open_ledger(self, X, y, depth) -> _SplitLedger:
+        """Pick the node's temperature (its random draws come here, before any
+        attempt) and open its ledger; a node whose temperature was tuned gets
+        no retries."""
+        temperature, context, tuned = self._pick_temperature(X, y, depth)
+        return _SplitLedger(temperature, context,
+                            budget=0 if tuned else self.max_retries_on_failure,
+                            fallback_enabled=bool(getattr(self.splitter, "min_side_fallback",
+                                                          False)))
 
-    def _retry_split(self, state: _SplitState, result) -> bool:
-        """Retry min-side failures and rejected gains while retries remain."""
-        failed = isinstance(result, SplitNotFoundError)
-        reason = result.reason if failed else self._split_rejection_reason(result)
-        if reason is not None:
-            state.reasons.append(reason)
-        eligible = (reason == "min_side_points" if failed else
-                    reason in ("insufficient_split_gain", "insufficient_relative_split_gain"))
-        if eligible and state.retries < state.limit:
-            state.retries += 1
-            self.restarts_on_failure += 1
-            return True
-        return False
+    def _settle_attempt(self, ledger, outcome) -> bool:
+        """Classify an attempt and log it; True when the node tries again."""
+        if isinstance(outcome, SplitNotFoundError):
+            reason = outcome.reason
+            retryable = reason in _RETRY_FAILURES
+        else:
+            reason = self._split_rejection_reason(outcome)
+            retryable = reason in _RETRY_REJECTIONS
+        again = ledger.settle(reason, retryable)
+        self.restarts_on_failure += int(again)
+        return again
 
-    def _resolve_split_temperature(self, X, y, depth, prepared_design):
+    def _pick_temperature(self, X, y, depth):
+        """(temperature or None for the splitter's own, context, tuned).
+
+        The scale constant c comes from the configuration ("fixed"), from the
+        root's tuning ("tune_root" after the root), or from tuning at this
+        node; the temperature is then c times the kNN scale of X."""
         config = self.temperature_config
         if config is None or config.strategy == "splitter":
-            return None, {"temperature": getattr(self.splitter, "temperature", None),
-                          "temperature_c": None, "temperature_tuned_at_node": False}, False
+            own = getattr(self.splitter, "temperature", None)
+            return None, {"temperature": own, "temperature_c": None,
+                          "temperature_tuned_at_node": False}, False
+        known = {"fixed": config.c, "tune_root": self._root_temperature_c}.get(config.strategy)
         scores = None
-        if config.strategy == "fixed":
-            c, tuned = config.c, False
-        elif config.strategy == "tune_root" and self._root_temperature_c is not None:
-            c, tuned = self._root_temperature_c, False
-        else:
-            c, scores = self._tune_temperature(X, y, prepared_design)
-            tuned = True
+        if known is None:
+            known, scores = self._tune_temperature(X, y, None)
             if config.strategy == "tune_root" and depth == 0:
-                self._root_temperature_c = c
-        temperature = self._temperature_for_c(X, c)
-        context = {"temperature": temperature, "temperature_c": c,
+                self._root_temperature_c = known
+        tuned = scores is not None
+        context = {"temperature": self._temperature_for_c(X, known), "temperature_c": known,
                    "temperature_scale_mode": config.scale_mode,
                    "temperature_tuned_at_node": tuned}
-        if self.store_diagnostics and tuned:
+        if tuned and self.store_diagnostics:
             context["temperature_candidate_scores"] = scores
-        return temperature, context, tuned
+        return context["temperature"], context, tuned
 
     def _tune_temperature(self, X, y, prepared_design):
         """Hold out a validation fraction and keep the c of lowest validation MSE."""

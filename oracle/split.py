@@ -10,8 +10,14 @@ Restates, for one node and one start theta0:
 * stop reasons, last-feasible min-side fallback and the hard refit --
   art/splitters.py:400-690;
 * the annealed fixed-step mode (temperature T_k = T0 (T1/T0)^(k/(K-1)), step
-  eta along the negative projected gradient, unit-norm retraction), which has
-  no reference counterpart (parity unpinned beyond its own closed forms);
+  eta along the negative projected gradient, unit-norm retraction).  The
+  reference has no annealed mode; its pieces are the reference's objective,
+  gradient and projections, and the mode is pinned by the schedule's known
+  answers (T_0 = T0, T_(K-1) = T1, t1 = t0/16 over five iterations giving
+  t0 2^-k exactly), by step-by-step composition of the pinned gradient and
+  projections on the reference's seed-801 fixture, and by planted mistakes
+  (exponent k/K, a step before the tangent projection, a dropped or an extra
+  iteration, an ascent step) that these checks must reject;
 * the device random start (art/splitters.py:692-700 with the device's own
   stream): FP64 Box-Muller from Philox4x32-10 words, w = normal / |normal|
   (w_0 = 1 when the norm is <= 1e-12, art/_cuda_batched_optimizer.cu:7-23),
@@ -283,18 +289,29 @@ def optimize(X, y, theta0, s: Settings, T=None):
                 last_beta=cur.beta, last_lambda=cur.lam, **hist)
 
 
+def anneal_schedule(iters, t0, t1):
+    """The annealed mode's temperatures: geometric from T_0 = t0 to
+    T_(iters-1) = t1, T_k = t0 (t1/t0)^(k/(iters-1)) (T_0 = t0 for one
+    iteration)."""
+    k = np.arange(int(iters), dtype=float)
+    return t0 * (t1 / t0) ** (k / max(int(iters) - 1, 1))
+
+
 def annealed(X, y, theta0, s: Settings, iters=200, t0=0.2, t1=0.01, step=0.05):
     """Fixed-step projected gradient with a geometric temperature schedule.
 
-    theta <- retract(theta - step * P(grad_T_k(theta))), T_k = t0 (t1/t0)^(k/(iters-1)),
-    P the tangent projection of art/splitters.py:86-98, retraction w/|w|.
-    Returns (theta, loss history at the start of each iteration).
+    theta <- retract(theta - step * P(grad_T_k(theta))), T_k from
+    anneal_schedule, P the tangent projection of art/splitters.py:86-98,
+    retraction w/|w| (art/splitters.py:71-83).  The reference optimizes at
+    one fixed temperature; this mode is the split-optimizer configuration's
+    annealed schedule built from the reference's objective, gradient and
+    projections.  Returns (theta, loss history at the start of each
+    iteration).
     """
     phi = design(X, s)
     theta = project_unit_w(theta0)
     losses = []
-    for k in range(iters):
-        T = t0 * (t1 / t0) ** (k / max(iters - 1, 1))
+    for T in anneal_schedule(iters, t0, t1):
         e = value_and_grad(X, y, phi, theta, T, s)
         losses.append(e.loss)
         theta = project_unit_w(theta - step * project_unit_w_gradient(theta, e.grad))

@@ -325,3 +325,110 @@ def test_tree_leaf_ids_match_host_routing(torch_gpu, d):
                                      if hasattr(n, "w")])), axis=0)
     ok = margin > 1e-12
     np.testing.assert_array_equal(ids[ok], host[ok])
+
+
+# --------------------------------------------------------------------------- plain ART fit
+
+@pytest.fixture(scope="module")
+def plain_fit_builds(torch_gpu):
+    """The plain ART fit at its full size (d = 2, 4000 samples per node, depth
+    <= 6, soft oblique T = 0.05 on the batched device optimizer, quadratic
+    ridge leaves at 1e-6, 256 samples per leaf polytope), built with the GPU
+    ridge template and again with the host FP64 ridge double."""
+    from art.synthetic import plain_fit_builder
+
+    gpu = plain_fit_builder(seed=0, store_samples=True, store_diagnostics=True).build()
+    host = plain_fit_builder(seed=0, store_samples=True, store_diagnostics=True,
+                             model_template=HostRidge(1e-6)).build()
+    return gpu, host
+
+
+def test_plain_fit_build_properties(plain_fit_builds):
+    from art.sampling import PolytopeBatch  # noqa: F401  (device sampler is loaded)
+    from art.synthetic import PlainFitOracle
+
+    gpu, _ = plain_fit_builds
+    tree = gpu.tree
+    assert tree.max_depth() <= 6 and tree.num_leaves() >= 8
+    total_new = 0
+    for node in tree.iter_nodes():
+        X = node.metadata["X"]
+        assert X.shape == (4000, 2)
+        assert node.region.contains(X).all(), node.node_id
+        total_new += node.metadata["n_new"]
+    assert total_new == gpu.oracle_queries
+    # 256 hit-and-run samples in every leaf polytope, inside it.
+    for leaf in tree.iter_leaves():
+        pts = leaf_samples(leaf, 256)
+        assert pts.shape == (256, 2) and leaf.region.contains(pts, tol=1e-12).all()
+    Xt = np.random.default_rng(9).uniform(-1, 1, (20000, 2))
+    pred = tree.predict(Xt)
+    np.testing.assert_allclose(pred, tree._predict_models(Xt), rtol=1e-12, atol=1e-12)
+    rmse = float(np.sqrt(np.mean((pred - PlainFitOracle.clean(Xt)) ** 2)))
+    assert rmse < 1e-2, rmse
+
+
+def leaf_samples(leaf, n):
+    from art.sampling import HitAndRunSampler
+
+    x0 = leaf.metadata["X"][0]
+    return HitAndRunSampler(burn_in=20, thinning=5).sample(leaf.region, n, random_state=1, x0=x0)
+
+
+def test_plain_fit_build_structure_matches_the_host_ridge_build(plain_fit_builds):
+    gpu, host = plain_fit_builds
+    a, b = list(gpu.tree.iter_nodes()), list(host.tree.iter_nodes())
+    assert [n.node_id for n in a] == [n.node_id for n in b]
+    assert [n.status for n in a] == [n.status for n in b]
+    assert gpu.oracle_queries == host.oracle_queries
+    for u, v in zip(a, b):
+        np.testing.assert_array_equal(u.metadata["X"], v.metadata["X"])
+        if hasattr(u, "w"):
+            np.testing.assert_array_equal(u.w, v.w)
+            assert u.z == v.z
+    for u, v in zip(gpu.tree.iter_leaves(), host.tree.iter_leaves()):
+        X = u.metadata["X"]
+        np.testing.assert_allclose(u.model.predict(X), v.model.predict(X), rtol=1e-9,
+                                   atol=1e-10)
+
+
+def test_plain_fit_build_replays_on_the_oracle(plain_fit_builds):
+    """Every decision of the GPU build, replayed on the FP64 oracle from the
+    build's own node samples and device starts: each split's final theta (the
+    oracle's optimizer from the recorded initial theta, art/splitters.py:365-601)
+    to 1e-7 and its partition of the node's samples exactly (rows within 1e-9
+    of the hyperplane excepted); each node fit (the oracle's normal-equation
+    ridge, art/models.py:218-343) on its fitted values to 1e-9; each node's
+    stop decision from the oracle's fit error."""
+    from art.metrics import relative_l2_error
+    from oracle import split as O
+
+    gpu, _ = plain_fit_builds
+    s = O.Settings(temperature=0.05, ridge=1e-6, min_side_points=64, degree=2,
+                   bias_last=False)
+    splits = 0
+    for node in gpu.tree.iter_nodes():
+        X, y = node.metadata["X"], node.metadata["y"]
+        phi = R.features(X, 2, True, False)
+        beta, _ = R.solve_weighted_ridge(phi, y, np.ones(len(y)), 1e-6)
+        err = relative_l2_error(y, phi @ beta)
+        assert node.metadata["fit_error"] == pytest.approx(err, rel=1e-8, abs=1e-12)
+        if node.status == "tolerance_met":
+            assert err <= 1e-2
+        if not hasattr(node, "w"):
+            if node.model is not None:
+                np.testing.assert_allclose(node.model.predict(X), phi @ beta, rtol=1e-9,
+                                           atol=1e-10)
+            continue
+        assert err > 1e-2 and node.depth < 6
+        theta0 = node.metadata["splitter_metadata"]["initial_theta"]
+        ref = O.optimize(X, y, theta0, s)
+        theta = np.append(node.w, node.z)
+        assert ref["stop_reason"] == node.metadata["split_stop_reason"], node.node_id
+        np.testing.assert_allclose(theta, ref["theta"], rtol=1e-7, atol=1e-9)
+        u_gpu, u_ref = X @ node.w - node.z, X @ ref["theta"][:-1] - ref["theta"][-1]
+        clear = np.abs(u_ref) > 1e-9
+        np.testing.assert_array_equal((u_gpu >= 0)[clear], (u_ref >= 0)[clear])
+        assert node.metadata["split_n_right"] == int(np.count_nonzero(u_gpu >= 0))
+        splits += 1
+    assert splits >= 4

@@ -412,6 +412,47 @@ def test_batched_annealer_at_full_width_matches_the_oracle(torch_gpu):
         np.testing.assert_allclose(two["beta"][r, j], h["beta"], rtol=1e-8, atol=1e-10)
 
 
+def test_batched_annealer_on_the_split_workload_matches_the_oracle(torch_gpu):
+    """The split-optimizer configuration's own shape and schedule: 2 nodes of
+    8192 rows in d = 16 with an 8-bump target (seed 3), 16 device starts each,
+    200 iterations cooling geometrically from T = 0.2 to 0.01 at step 0.05,
+    affine children at ridge 1e-4.  The batched kernel equals the per-start
+    kernel, and three starts equal the FP64 oracle run from the same start
+    over the whole schedule (saturated gates and the +-50 clip included)."""
+    from art import AffineRidgeModel, BatchedOptimizerWorkspace, SoftObliqueSplitter
+    from art.synthetic import split_workload
+
+    X, y = split_workload(n_nodes=2, seed=3)
+    d = X.shape[2]
+    s = SoftObliqueSplitter(AffineRidgeModel(1e-4), .2, n_restarts=16, min_side_points=64,
+                            random_state=3)
+    with BatchedOptimizerWorkspace(s, X, y) as ws:
+        starts = ws._device_starts(np.tile(np.arange(2), 16)).reshape(16, 2, d + 1)
+        one = ws.anneal(starts, t0=.2, t1=.01, iters=200, step=.05, batched=False)
+        two = ws.anneal(starts, t0=.2, t1=.01, iters=200, step=.05, batched=True)
+    np.testing.assert_array_equal(two["ok"], one["ok"])
+    assert two["ok"].mean() > .5
+    np.testing.assert_allclose(two["theta"], one["theta"], rtol=1e-9, atol=1e-10)
+    np.testing.assert_allclose(two["soft_loss_history"], one["soft_loss_history"], rtol=1e-9,
+                               atol=1e-12)
+    np.testing.assert_array_equal(two["n_right"], one["n_right"])
+    so = settings_of(s, .2)
+    for r, j in ((0, 0), (9, 1), (15, 0)):
+        th, losses = O.annealed(X[j], y[j], starts[r, j], so, iters=200, t0=.2, t1=.01,
+                                step=.05)
+        np.testing.assert_allclose(two["theta"][r, j], O.project_unit_w(th), rtol=1e-9,
+                                   atol=1e-10)
+        np.testing.assert_allclose(two["soft_loss_history"][r, j], losses, rtol=1e-9,
+                                   atol=1e-12)
+        try:
+            h = O.hard_result(X[j], y[j], th, so)
+        except ValueError:  # a side below the minimum: the device flags it too
+            assert not two["ok"][r, j]
+            continue
+        assert two["ok"][r, j] and two["n_right"][r, j] == h["n_right"]
+        assert two["loss"][r, j] == pytest.approx(h["loss"], rel=1e-9, abs=1e-14)
+
+
 @pytest.mark.parametrize("n,d,ties", [(8192, 16, False), (301, 3, False), (1, 2, False),
                                        (64, 31, False), (8193, 3, False), (10000, 2, False),
                                        (2, 1, False), (400, 3, True), (9000, 2, True)])

@@ -241,6 +241,84 @@ def test_converged_split_is_near_the_brute_force_optimum():
     assert found <= best + 1e-3
 
 
+# Known answers of a geometric schedule from t0 to t1 over K iterations:
+# the first temperature is t0, the last t1; with t1 = t0 / 16 over five
+# iterations each step halves the temperature.
+_T0, _T1 = 0.2, 0.2 / 16
+_SCHEDULES = {1: [0.2], 2: [0.2, 0.0125], 3: [0.2, 0.05, 0.0125],
+              5: [0.2, 0.1, 0.05, 0.025, 0.0125]}
+
+
+def _composed(X, y, theta0, s, temps, step):
+    """K annealing steps built from the pinned pieces, in the documented
+    order: loss and gradient at T_k, tangent projection, step, retraction."""
+    phi = O.design(X, s)
+    theta, losses = O.project_unit_w(theta0), []
+    for T in temps:
+        e = O.value_and_grad(X, y, phi, theta, T, s)
+        losses.append(e.loss)
+        theta = O.project_unit_w(theta - step * O.project_unit_w_gradient(theta, e.grad))
+    return theta, np.array(losses)
+
+
+def anneal_failures(annealer, schedule):
+    """Names of the checks an annealer / schedule pair fails (empty: pinned)."""
+    failed = []
+    for K, temps in _SCHEDULES.items():
+        if not np.allclose(schedule(K, _T0, _T1), temps, rtol=1e-15, atol=0.0):
+            failed.append(f"schedule K={K}")
+    X, y, _ = problems()
+    s = O.Settings(ridge=1e-4, min_side_points=2)
+    for node, theta0 in ((1, np.array([.6, .8, .1])), (2, np.array([1., -.3, -.2]))):
+        for K, temps in _SCHEDULES.items():
+            theta, losses = annealer(X[node], y[node], theta0, s, iters=K, t0=_T0, t1=_T1,
+                                     step=.05)
+            ref_theta, ref_losses = _composed(X[node], y[node], theta0, s, temps, .05)
+            if len(losses) != K or not np.allclose(losses, ref_losses, rtol=1e-13, atol=0.0):
+                failed.append(f"losses node={node} K={K}")
+            if not np.allclose(theta, ref_theta, rtol=1e-13, atol=1e-15):
+                failed.append(f"theta node={node} K={K}")
+    return failed
+
+
+def test_annealed_mode_is_pinned_by_its_schedule_and_composition():
+    assert anneal_failures(O.annealed, O.anneal_schedule) == []
+
+
+def _planted(mistake):
+    """The annealer with one plausible mistake planted."""
+    def schedule(iters, t0, t1):
+        k = np.arange(iters, dtype=float)
+        if mistake == "exponent_k_over_K":
+            return t0 * (t1 / t0) ** (k / iters)
+        return O.anneal_schedule(iters, t0, t1)
+
+    def annealer(X, y, theta0, s, iters=200, t0=0.2, t1=0.01, step=0.05):
+        phi = O.design(X, s)
+        theta, losses = O.project_unit_w(theta0), []
+        count = {"dropped_iteration": iters - 1, "extra_iteration": iters + 1}.get(mistake, iters)
+        temps = list(schedule(max(count, 1), t0, t1))[:count] if mistake in (
+            "dropped_iteration", "extra_iteration") else schedule(iters, t0, t1)
+        for T in temps:
+            e = O.value_and_grad(X, y, phi, theta, T, s)
+            losses.append(e.loss)
+            if mistake == "step_before_projection":
+                theta = O.project_unit_w(theta - step * e.grad)
+            elif mistake == "ascent_step":
+                theta = O.project_unit_w(theta + step * O.project_unit_w_gradient(theta, e.grad))
+            else:
+                theta = O.project_unit_w(theta - step * O.project_unit_w_gradient(theta, e.grad))
+        return theta, np.array(losses)
+
+    return annealer, schedule
+
+
+@pytest.mark.parametrize("mistake", ["exponent_k_over_K", "step_before_projection",
+                                     "dropped_iteration", "extra_iteration", "ascent_step"])
+def test_annealed_checks_reject_planted_mistakes(mistake):
+    assert anneal_failures(*_planted(mistake)), mistake
+
+
 def test_annealed_mode_decreases_the_loss_on_a_clean_split():
     rng = np.random.default_rng(17)
     X = rng.uniform(-1, 1, (100, 2))
