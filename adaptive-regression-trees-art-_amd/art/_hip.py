@@ -84,6 +84,7 @@ _PROTOTYPES = {
     "art_check_finite": (_i, [_p, _p, _i64, _i64, _i, _p, _p]),
     "art_leaf_plan_create": (_i, [_p, _i, _i, _i, _i, _i, _i, ctypes.POINTER(_p)]),
     "art_leaf_ridge_fit": (_i, [_p, _p, _p, _p, _d, _d, _i, _p, _p, _p, _p]),
+    "art_leaf_ridge_fit_static": (_i, [_p, _i, Basis, _p, _p, _p, _d, _d, _p, _p, _p, _p]),
     "art_gated_plan_create": (_i, [_i, _i64, _i, _i, ctypes.POINTER(_p)]),
     "art_gated_ridge_fit": (_i, [_p, _p, _p, _p, _p, _p, _i, _d, _d, _p, _p, _p, _p]),
     "art_plan_info": (_i, [_p, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)]),

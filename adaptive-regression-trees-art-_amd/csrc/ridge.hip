@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -1459,6 +1460,31 @@ static void launch_order(std::vector<LeafChunk>& table) {
   });
 }
 
+// Gram work items of K leaves: leaf j's rows [offsets[j], offsets[j+1]) in
+// chunks of at most chunk_rows, numbered (slot) in leaf order, then sorted
+// into launch order -- largest chunk first, so that the long chunks do not
+// start last and leave a tail (rows differ by up to 256x between leaves).
+// sys[j] names leaf j's slots.  False when the offsets decrease.
+static bool leaf_tables(const int64_t* offsets, int K, int chunk_rows,
+                        std::vector<LeafChunk>& chunks, std::vector<SystemDesc>& sys) {
+  chunks.clear();
+  sys.assign(K, SystemDesc{});
+  for (int j = 0; j < K; ++j) {
+    const int64_t b = offsets[j], e = offsets[j + 1];
+    if (e < b) return false;
+    sys[j].chunk_begin = (int)chunks.size();
+    for (int64_t r = b; r < e; r += chunk_rows)
+      chunks.push_back(
+          LeafChunk{r, r + chunk_rows < e ? r + chunk_rows : e, j, (int)chunks.size()});
+    sys[j].chunk_end = (int)chunks.size();
+    sys[j].out = j;
+    sys[j].side = 0;
+  }
+  if (chunks.empty()) chunks.push_back(LeafChunk{0, 0, 0, 0});
+  launch_order(chunks);
+  return true;
+}
+
 int art_leaf_plan_create(const int64_t* offsets, int K, int d, int degree, int include_bias,
                          int bias_last, int chunk_rows, art_ridge_plan** out) {
   if (out == nullptr || offsets == nullptr || K < 1 || d < 1)
@@ -1478,22 +1504,11 @@ int art_leaf_plan_create(const int64_t* offsets, int K, int d, int degree, int i
   P->npairs = P->T * (P->T + 1) / 2;
   P->chunk_rows = chunk_rows;
   std::vector<LeafChunk> chunks;
-  std::vector<SystemDesc> sys(K);
-  for (int j = 0; j < K; ++j) {
-    const int64_t b = offsets[j], e = offsets[j + 1];
-    if (e < b) {
-      plan_free(P);
-      return fail(ART_ERR_ARG, "art_leaf_plan_create: offsets must be nondecreasing");
-    }
-    sys[j].chunk_begin = (int)chunks.size();
-    for (int64_t r = b; r < e; r += chunk_rows)
-      chunks.push_back(
-          LeafChunk{r, r + chunk_rows < e ? r + chunk_rows : e, j, (int)chunks.size()});
-    sys[j].chunk_end = (int)chunks.size();
-    sys[j].out = j;
-    sys[j].side = 0;
+  std::vector<SystemDesc> sys;
+  if (!leaf_tables(offsets, K, chunk_rows, chunks, sys)) {
+    plan_free(P);
+    return fail(ART_ERR_ARG, "art_leaf_plan_create: offsets must be nondecreasing");
   }
-  if (chunks.empty()) chunks.push_back(LeafChunk{0, 0, 0, 0});
   P->nchunks = (int)chunks.size();
   P->nsys = K;
   std::vector<int> goff(K + 1), roff(K + 1);
@@ -1510,9 +1525,6 @@ int art_leaf_plan_create(const int64_t* offsets, int K, int d, int degree, int i
   roff[K] = (int)rchunks.size();
   if (rchunks.empty()) rchunks.push_back(LeafChunk{0, 0, 0, 0});
   P->nrchunks = (int)rchunks.size();
-  // Launch order: largest chunk first, so that the long chunks do not start
-  // last and leave a tail (rows differ by up to 256x between leaves).
-  launch_order(chunks);
   launch_order(rchunks);
   if (hipMalloc(&P->d_goff, (K + 1) * sizeof(int)) != hipSuccess ||
       hipMalloc(&P->d_roff, (K + 1) * sizeof(int)) != hipSuccess ||
@@ -1903,6 +1915,107 @@ int art_leaf_set_refine_tol(art_ridge_plan* P, double tol) {
   if (!P || P->kind != 0 || !(tol > 0.0) || std::isinf(tol))
     return fail(ART_ERR_ARG, "art_leaf_set_refine_tol: tol must be finite and > 0");
   P->refine_tol = tol;
+  return ART_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free leaf fits: every work buffer of art_leaf_ridge_fit_static
+// (chunk table, leaf table, per-chunk partial systems and weight flags) lives
+// in one static device array per GPU, so a call allocates no device memory.
+
+static __device__ __align__(256) unsigned char scratch[ART_STATIC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's copy of `scratch` (resolved once per device).  Host
+// code uses only this address: the array's own name is a host-side stand-in.
+int static_scratch(unsigned char** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "art_leaf_ridge_fit_static: hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(ART_ERR_ARG, "art_leaf_ridge_fit_static: device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "art_leaf_ridge_fit_static: hipGetSymbolAddress");
+    g_scratch_ptr[dev] = static_cast<unsigned char*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return ART_OK;
+}
+
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+}  // namespace
+
+extern "C" {
+
+int art_leaf_ridge_fit_static(const int64_t* offsets, int K, art_basis basis, const double* X,
+                              const double* y, const double* w, double ridge,
+                              double weight_floor, double* beta, int* status, double* lambda,
+                              hipStream_t stream) {
+  if (offsets == nullptr || K < 1 || !X || !y || !w || !beta || !status || !lambda)
+    return fail(ART_ERR_ARG, "art_leaf_ridge_fit_static: bad arguments");
+  if (basis.d < 1 || (basis.degree != 1 && basis.degree != 2) ||
+      (basis.bias_last && basis.degree != 1) ||
+      basis.p != basis_size(basis.d, basis.degree, basis.bias))
+    return fail(ART_ERR_ARG, "art_leaf_ridge_fit_static: inconsistent basis");
+  if (!(ridge >= 0.0) || std::isinf(ridge) || !(weight_floor >= 0.0) || std::isinf(weight_floor))
+    return fail(ART_ERR_ARG,
+                "art_leaf_ridge_fit_static: ridge and weight_floor must be finite >= 0");
+  const int p = basis.p, T = tiles_for(p), npairs = T * (T + 1) / 2;
+  if (T > 6 || basis.d > kMaxStagedD)
+    return fail(ART_ERR_ARG, "art_leaf_ridge_fit_static: needs p <= 95 and d <= 24");
+  std::vector<LeafChunk> chunks;
+  std::vector<SystemDesc> sys;
+  if (!leaf_tables(offsets, K, ART_DEFAULT_LEAF_CHUNK_ROWS, chunks, sys))
+    return fail(ART_ERR_ARG, "art_leaf_ridge_fit_static: offsets must be nondecreasing");
+  const int nchunks = (int)chunks.size();
+  const size_t chunk_bytes = align256(chunks.size() * sizeof(LeafChunk));
+  const size_t sys_bytes = align256(sys.size() * sizeof(SystemDesc));
+  const size_t part_bytes = align256((size_t)nchunks * npairs * 256 * sizeof(double));
+  const size_t flag_bytes = align256((size_t)nchunks * sizeof(int));
+  if (chunk_bytes + sys_bytes + part_bytes + flag_bytes > ART_STATIC_SCRATCH_BYTES)
+    return fail(ART_ERR_ARG,
+                "art_leaf_ridge_fit_static: problem exceeds the static scratch "
+                "(ART_STATIC_SCRATCH_BYTES); use a plan (art_leaf_plan_create)");
+  unsigned char* base = nullptr;
+  int rc = static_scratch(&base);
+  if (rc != ART_OK) return rc;
+  LeafChunk* d_chunks = reinterpret_cast<LeafChunk*>(base);
+  SystemDesc* d_sys = reinterpret_cast<SystemDesc*>(base + chunk_bytes);
+  double* d_partials = reinterpret_cast<double*>(base + chunk_bytes + sys_bytes);
+  int* d_flags = reinterpret_cast<int*>(base + chunk_bytes + sys_bytes + part_bytes);
+  // Pageable host sources: the copy has read them when the call returns, so
+  // the tables may go out of scope; stream order keeps earlier calls' kernels
+  // on this stream ahead of the overwrite.
+  ART_HIP(hipMemcpyAsync(d_chunks, chunks.data(), chunks.size() * sizeof(LeafChunk),
+                         hipMemcpyHostToDevice, stream));
+  ART_HIP(hipMemcpyAsync(d_sys, sys.data(), sys.size() * sizeof(SystemDesc),
+                         hipMemcpyHostToDevice, stream));
+  // Every partial tile and weight flag is written by the Gram kernel before
+  // the solve reads it: nothing needs clearing.
+  const Basis B{basis.d, p, basis.degree, basis.bias ? 1 : 0, basis.bias_last ? 1 : 0};
+  const size_t smem = leaf_stage_smem_bytes(B.d);
+  switch (T) {
+#define ART_CASE(TT)                                                                        \
+  case TT:                                                                                  \
+    gram_leaf_f64_kernel<TT><<<dim3(nchunks), dim3(256), smem, stream>>>(                   \
+        X, y, w, B, weight_floor, d_chunks, nullptr, d_partials, d_flags);                  \
+    break;
+    ART_CASE(1) ART_CASE(2) ART_CASE(3) ART_CASE(4) ART_CASE(5) ART_CASE(6)
+#undef ART_CASE
+  }
+  ART_HIP(hipGetLastError());
+  launch_rows_solve(d_sys, K, nchunks, T, npairs, p, ridge, d_partials, d_flags, nullptr,
+                    nullptr, nullptr, beta, status, lambda, stream);
+  ART_HIP(hipGetLastError());
   return ART_OK;
 }
 

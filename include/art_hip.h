@@ -91,6 +91,25 @@ int art_leaf_ridge_fit(art_ridge_plan* plan, const double* X, const double* y, c
                        double ridge, double weight_floor, int precision, double* beta,
                        int* status, double* lambda, hipStream_t stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the FP64 fit of art_leaf_ridge_fit (the plan's default
+ * chunking and kernels, so the same results as a plan fit) with no plan and
+ * no allocation.  offsets (K+1) is host memory; basis gives d, p, degree,
+ * bias and bias_last (p must match them; p <= 95, d <= 24).  Every work
+ * buffer (chunk and leaf tables, per-chunk partial systems, weight flags)
+ * lives in one static device array of ART_STATIC_SCRATCH_BYTES per GPU,
+ * held until the process ends; a problem needing more is rejected with
+ * ART_ERR_ARG.  Every call on a GPU shares that GPU's array: calls that
+ * overlap on one GPU (different streams, or threads) race; calls on one
+ * stream are ordered.  Needs about 12 KB per 4096-row chunk at p <= 47
+ * (e.g. ~1300 chunks, ~5M rows, for the quadratic basis in d = 8). */
+#define ART_STATIC_SCRATCH_BYTES (16u << 20)
+int art_leaf_ridge_fit_static(const int64_t* offsets, int K, art_basis basis, const double* X,
+                              const double* y, const double* w, double ridge,
+                              double weight_floor, double* beta, int* status, double* lambda,
+                              hipStream_t stream);
+
 /* ART_PRECISION_MIXED (p <= 64): split-bf16 MFMA Gram with exact FP64
  * diagonal/right-hand side, one FP64 data-space refinement pass, and the exact
  * FP64 path for every leaf whose refinement correction exceeds the tolerance
