@@ -56,7 +56,7 @@ class SplitArgs(ctypes.Structure):
         ("theta_out", ctypes.c_void_p), ("beta_out", ctypes.c_void_p), ("hard", ctypes.c_void_p),
         ("info", ctypes.c_void_p), ("history", ctypes.c_void_p), ("scalars", ctypes.c_void_p),
         ("row_offsets", ctypes.c_void_p), ("min_side_points", ctypes.c_int),
-        ("min_side_fraction", ctypes.c_double),
+        ("min_side_fraction", ctypes.c_double), ("child_precision", ctypes.c_int),
     ]
 
 

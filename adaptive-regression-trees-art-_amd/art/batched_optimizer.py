@@ -155,6 +155,35 @@ class BatchedOptimizerWorkspace:
             raise ValueError("X/y must have shapes (k,n,d)/(k,n), 0<k<=capacity.")
         return self._load(X, y, temp)
 
+    def load_device(self, X, y, *, temperature=None):
+        """Load a cohort already resident on the device: X (k, n, d) and
+        y (k, n) contiguous float64 tensors on the workspace's device, copied
+        into the fixed buffers on the workspace stream (no host staging; the
+        data are finite by the caller's contract).  The host views X and y
+        stay empty until a host load."""
+        self._check_open()
+        torch = self.torch
+        if (not isinstance(X, torch.Tensor) or not isinstance(y, torch.Tensor) or not X.is_cuda
+                or X.dtype != torch.float64 or y.dtype != torch.float64 or X.dim() != 3
+                or tuple(y.shape) != tuple(X.shape[:2])):
+            raise ValueError("Device X/y must be float64 tensors of shapes (k,n,d)/(k,n).")
+        k = int(X.shape[0])
+        if not 0 < k <= self.K or tuple(X.shape[1:]) != (self.n, self.d):
+            raise ValueError("X/y must have shapes (k,n,d)/(k,n), 0<k<=capacity.")
+        temp = np.broadcast_to(np.asarray(self.s.temperature if temperature is None
+                                          else temperature, dtype=float), (k,))
+        if not np.isfinite(temp).all() or np.any(temp <= 0):
+            raise ValueError("Temperatures must be finite and positive.")
+        self.temperature[:k] = temp
+        with torch.cuda.stream(self.stream):
+            self.x[:k].copy_(X)
+            self.yd[:k].copy_(y)
+            self.temp[:k].copy_(torch.from_numpy(self.temperature[:k].copy()))
+        self.k = k
+        self.X, self.y = self.host_X[:0], self.host_y[:0]
+        self.fit_sizes.clear()
+        return self
+
     def _load(self, X, y, temp):
         k = X.shape[0]
         self.k = 0
@@ -199,7 +228,7 @@ class BatchedOptimizerWorkspace:
         return out.cpu().numpy()
 
     def _launch(self, nodes, theta0, mode, fallback, hard_refit, anneal=None, timing=None,
-                batched=False):
+                batched=False, child_precision=_hip.PRECISION_FP64):
         """One device launch over problems (nodes[i], theta0[i]); host copies out.
 
         anneal = (iters, t0, t1, step) for mode 1; timing, a list, receives
@@ -237,6 +266,7 @@ class BatchedOptimizerWorkspace:
             a.anneal_iters, a.t0, a.t1, a.step = int(anneal[0]), *map(float, anneal[1:])
         a.theta_out, a.beta_out, a.hard = _hip.ptr(out_theta), _hip.ptr(out_beta), _hip.ptr(hard)
         a.info, a.history, a.scalars = _hip.ptr(info), _hip.ptr(hist), _hip.ptr(scal)
+        a.child_precision = int(child_precision)
         self._ragged_args(a)
         with torch.cuda.stream(self.stream):
             if timing is not None:
@@ -381,7 +411,7 @@ class BatchedOptimizerWorkspace:
         return out, ok, reason
 
     def anneal(self, theta0=None, *, nodes=None, t0=0.2, t1=0.01, iters=200, step=0.05,
-               timing=None, batched=None):
+               timing=None, batched=None, child_precision="fp64"):
         """Annealed fixed-step projected gradient from every start, then a hard refit.
 
         theta0: (r, d+1) or (R, r, d+1) starts for the r selected nodes, or
@@ -400,7 +430,14 @@ class BatchedOptimizerWorkspace:
         runs 16 starts of a node per workgroup, the node's rows staged once
         per pass for all of them (art_split_anneal_batched); otherwise one
         workgroup per start.
+
+        child_precision="mixed" (batched only) forms the soft fits' 32 child
+        systems per workgroup by split-bf16 MFMA with FP32 accumulation into
+        FP64; gates, predictions, gradients, solves and the hard refit stay
+        FP64.  "fp64" (default) is FP64 throughout.
         """
+        if child_precision not in ("fp64", "mixed"):
+            raise ValueError("child_precision must be 'fp64' or 'mixed'.")
         self._check_open()
         if not self.k:
             raise RuntimeError("Load data before optimizing.")
@@ -423,6 +460,9 @@ class BatchedOptimizerWorkspace:
             batched = eligible
         if batched and not eligible:
             raise ValueError("The batched annealer needs d <= 16 and at most 17 basis features.")
+        if child_precision == "mixed" and not batched:
+            raise ValueError("child_precision='mixed' needs the batched annealer.")
+        child = _hip.PRECISION_MIXED if child_precision == "mixed" else _hip.PRECISION_FP64
         if batched:
             # Node-major blocks of 16 starts; a node's last block repeats its
             # last start (results of the copies are dropped).
@@ -431,7 +471,8 @@ class BatchedOptimizerWorkspace:
             nodes_all = np.repeat(ids, B)
             thetas = starts.transpose(1, 0, 2)[:, order].reshape(-1, self.d + 1)
             outs = self._launch(nodes_all, thetas, 1, False, True,
-                                anneal=(iters, t0, t1, step), timing=timing, batched=True)
+                                anneal=(iters, t0, t1, step), timing=timing, batched=True,
+                                child_precision=child)
             keep = (np.arange(count)[:, None] * B + np.arange(Rr)[None, :]).T.reshape(-1)
             theta, beta, hard, info, hist, _ = (o[keep] for o in outs)
         else:

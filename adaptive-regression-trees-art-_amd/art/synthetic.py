@@ -168,6 +168,34 @@ def split_workload(n_nodes=2048, n=8192, d=16, n_bumps=8, seed=3):
     return X, y
 
 
+def split_workload_device(n_nodes, device, n=8192, d=16, n_bumps=8, seed=3, block=64):
+    """The split workload's shape generated on the device: node k's rows
+    X_k ~ U[-1, 1]^d from a torch generator seeded (seed, k) and its target
+    a BumpMixture drawn from numpy's default_rng(seed) in node order (the
+    bumps of split_workload's distribution; the rows are another seeded
+    stream).  Returns float64 device tensors X (K, n, d), y (K, n) and the
+    per-node mixtures."""
+    import torch
+
+    rng = np.random.default_rng(seed)
+    mixtures = [BumpMixture.random(rng, n_bumps, d) for _ in range(n_nodes)]
+    X = torch.empty((n_nodes, n, d), dtype=torch.float64, device=device)
+    y = torch.zeros((n_nodes, n), dtype=torch.float64, device=device)
+    g = torch.Generator(device=device)
+    for k in range(n_nodes):
+        g.manual_seed(int(seed) * 1_000_003 + k)
+        X[k].uniform_(-1.0, 1.0, generator=g)
+    for k0 in range(0, n_nodes, block):
+        k1 = min(k0 + block, n_nodes)
+        c = torch.as_tensor(np.stack([m.centers for m in mixtures[k0:k1]]), device=device)
+        w = torch.as_tensor(np.stack([m.widths for m in mixtures[k0:k1]]), device=device)
+        a = torch.as_tensor(np.stack([m.amplitudes for m in mixtures[k0:k1]]), device=device)
+        for b in range(n_bumps):
+            r2 = ((X[k0:k1] - c[:, b][:, None, :]) ** 2).sum(dim=2)
+            y[k0:k1] += a[:, b][:, None] * torch.exp(-r2 / (2.0 * w[:, b][:, None] ** 2))
+    return X, y, mixtures
+
+
 class PlainFitOracle:
     """The 2-D plain-fit target: f(x) = sin(3 x_1) cos(2 x_2)
     + 0.5 exp(-8 |x - (0.3, -0.2)|^2), plus N(0, noise^2) from a seeded

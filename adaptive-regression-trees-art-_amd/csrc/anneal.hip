@@ -24,6 +24,8 @@
 // Limits: d <= 16 and p <= 17 (so [phi, y] has at most 171 packed products);
 // problems come in blocks of 16 that share a node.
 
+#include <cstddef>
+
 #include "art_common.h"
 #include "art_hip.h"
 
@@ -42,6 +44,7 @@ constexpr int kAGroups = kAThreads / 32;           // systems factored at once
 constexpr int kFStr = kAP + 2;                     // F row stride (odd)
 constexpr int kQStr = 2 * kAR + 1;                 // Q / P row stride (odd)
 constexpr int kSlots = (kARows * (kADim + 1) + kAThreads - 1) / kAThreads;
+constexpr int kQBStr = kARows + 8;                 // bf16 side-weight plane row stride
 
 // Gate of a score s = (u - z) * (1/T): 1 / (1 + e^-s) with s clipped to
 // [-50, 50], the quotient from the hardware reciprocal and two Newton steps
@@ -74,12 +77,18 @@ struct AnnealShared {
   double bc[kAGroups][2];
   double xs[2][kARows * kAStr];  // double-buffered row chunks
   signed char fa[kAP], fb[kAP];
-  union {
+  union __attribute__((aligned(16))) {
     double G[2 * kAR][kATri];    // augmented systems, index 2 r + side
     struct {
       double F[kARows][kFStr];   // [phi, y] per row
       double Q[kARows][kQStr];   // side weights, column 2 r + side
     } fit;
+    struct {
+      double F[kARows][kFStr];
+      // Side weights split in bf16 hi, mid, lo planes, start-side major: a
+      // lane's 8 rows of one start-side are 16 contiguous bytes.
+      unsigned short QB[3][2 * kAR][kQBStr];
+    } fitb;
     struct {
       double pi[kARows][kAR + 1];
       double P[kARows][kQStr];   // predictions, column 2 r + side
@@ -87,6 +96,11 @@ struct AnnealShared {
     double red[8][kAR][kADim + 3];  // per-wave gradient tiles and sums
   } u;
 };
+
+// The bf16 side-weight planes are read 16 bytes at a time.
+static_assert(offsetof(AnnealShared, u) % 16 == 0, "LDS union must be 16-byte aligned");
+static_assert((kFStr * kARows * sizeof(double)) % 16 == 0 && (kQBStr * 2) % 16 == 0,
+              "bf16 plane rows must start on 16-byte boundaries");
 
 typedef const __attribute__((address_space(1))) double gf64;
 
@@ -307,6 +321,170 @@ __device__ __attribute__((noinline)) int anneal_fit(AnnealCtx C, AnnealShared& S
       for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
         for (int r = 0; r < 4; ++r) S.u.G[16 * mt + hi + 4 * r][t] = acc[s][mt][r];
+  }
+  if (wave < 4) {
+    bad |= __shfl_xor(bad, 16, kWave);
+    bad |= __shfl_xor(bad, 32, kWave);
+    if (hi == 0) S.badw[wave][lo] = bad;
+  }
+  __syncthreads();
+  if (tid < kAR) S.bad[tid] = S.badw[0][tid] | S.badw[1][tid] | S.badw[2][tid] | S.badw[3][tid];
+  __syncthreads();
+  return buf;
+}
+
+// One chunk of the wave's NT product tiles in split bf16: per 32-row step,
+// the side-weight planes of both start-side halves are read from LDS once;
+// then, tile by tile, the lane's 8 rows of psi (column 16 tile + lo) are
+// formed and split in registers and the 6 products of the tile's two output
+// tiles run interleaved (two independent accumulator chains) before being
+// added into FP64.  One tile at a time keeps the pass within 128 VGPRs.
+template <int NT>
+__device__ inline void bf16_chunk(const AnnealShared& S, int lo, int hi, const int* oa,
+                                  const int* ob, const NegPairs& neg, f64x4 (&acc)[2][2]) {
+  constexpr int ta[6] = {0, 2, 1, 0, 1, 0}, tb[6] = {2, 0, 1, 1, 0, 0};
+#pragma unroll
+  for (int ks = 0; ks < kARows / 32; ++ks) {
+    const int r0 = 32 * ks + 8 * hi;  // this lane's 8 rows of the step
+    bf16x8 A[2][3];
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+      for (int pl = 0; pl < 3; ++pl)
+        A[mt][pl] = __builtin_bit_cast(
+            bf16x8, *reinterpret_cast<const u32x4*>(&S.u.fitb.QB[pl][16 * mt + lo][r0]));
+#pragma unroll
+    for (int s = 0; s < NT; ++s) {
+      u32x4 bw[3];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double* f0 = S.u.fitb.F[r0 + 2 * k];
+        const double* f1 = S.u.fitb.F[r0 + 2 * k + 1];
+        unsigned h, md, l;
+        split3_pair((float)(f0[oa[s]] * f0[ob[s]]), (float)(f1[oa[s]] * f1[ob[s]]), neg, h,
+                    md, l);
+        bw[0][k] = h;
+        bw[1][k] = md;
+        bw[2][k] = l;
+      }
+      bf16x8 B[3];
+#pragma unroll
+      for (int pl = 0; pl < 3; ++pl) B[pl] = __builtin_bit_cast(bf16x8, bw[pl]);
+      // Products small first: hi*lo, lo*hi, mid*mid, hi*mid, mid*hi, hi*hi.
+      f32x4 z[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+      for (int term = 0; term < 6; ++term)
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+          z[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[mt][ta[term]], B[tb[term]], z[mt],
+                                                         0, 0, 0);
+#pragma unroll
+      for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[s][mt][r] += (double)z[mt][r];
+    }
+  }
+}
+
+// The soft fit with split-bf16 child systems (child_precision MIXED):
+// the same gates, side weights and [phi, y] rows as anneal_fit, but the 32
+// systems Q^T Psi come from v_mfma_f32_16x16x32_bf16 over 32-row steps.
+// Each side weight q (split once, as it is written) and each packed product
+// psi (split in the lane that uses it) is hi + mid + lo in bf16 (exact to
+// 2^-27 of its FP32 value); the six products hi*lo, lo*hi, mid*mid, hi*mid,
+// mid*hi, hi*hi accumulate in FP32 over a step and are added into FP64.
+// The output tile holds start-side 16 mt + 4 hi + r, column 16 tile + lo.
+__device__ __attribute__((noinline)) int anneal_fit_bf16(AnnealCtx C, AnnealShared& S, int buf,
+                                                         double T, double floor) {
+  Stream X;
+  stream_init(C, X);
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lo = lane & 15, hi = lane >> 4;
+  const int d = C.d, p = C.p, m = p + 1, n = C.n;
+  const int ntri = m * (m + 1) / 2, ntiles = (ntri + 15) >> 4;
+  const double invT = 1.0 / T;
+  const int nw = (wave + 8 < ntiles) ? 2 : (wave < ntiles ? 1 : 0);
+  int oa[2], ob[2];
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const int t = 16 * (wave + 8 * s) + lo;
+    int a = kFStr - 1, b = kFStr - 1;
+    if (t < ntri) {
+      a = 0;
+      while (tri(a + 1, 0) <= t) ++a;
+      b = t - tri(a, 0);
+    }
+    oa[s] = a;
+    ob[s] = b;
+  }
+  const NegPairs neg = neg_pairs();
+  f64x4 acc[2][2];
+#pragma unroll
+  for (int s = 0; s < 2; ++s) acc[s][0] = acc[s][1] = f64x4{0.0, 0.0, 0.0, 0.0};
+  int bad = 0;
+  const int chunks = (n + kARows - 1) / kARows;
+  for (int c = 0; c < chunks; ++c) {
+    const int rows = min(kARows, n - c * kARows);
+    const double* xs = S.xs[buf];
+    double pf[kSlots];
+    stream_load(C, X, (c + 1 < chunks ? c + 1 : 0) * kARows, pf);
+    if (wave < 4) {
+      const f64x4 u = gate_scores(S, xs, d, wave);
+      const double z = S.theta[lo][d];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = 16 * wave + hi + 4 * r;
+        double q0 = 0.0, q1 = 0.0;
+        if (row < rows) {
+          const double score = (u[r] - z) * invT;
+          const double pi = isfinite(score) ? fast_gate(score) : __builtin_nan("");
+          const bool ok = isfinite(pi) && pi >= 0.0 && pi <= 1.0;
+          bad |= !ok;
+          if (ok) {
+            q0 = fmax(1.0 - pi, floor);
+            q1 = fmax(pi, floor);
+          }
+        }
+        unsigned w3[3];
+        split3_pair((float)q0, (float)q1, neg, w3[0], w3[1], w3[2]);
+#pragma unroll
+        for (int pl = 0; pl < 3; ++pl) {
+          S.u.fitb.QB[pl][2 * lo][row] = (unsigned short)(w3[pl] & 0xffffu);
+          S.u.fitb.QB[pl][2 * lo + 1][row] = (unsigned short)(w3[pl] >> 16);
+        }
+      }
+    } else {
+      const int t = tid - 256, row = t >> 2;
+      const double* xr = xs + row * kAStr;
+      for (int f = t & 3; f < kFStr; f += 4) {
+        double v = 0.0;
+        if (f == p) {
+          v = xr[d];
+        } else if (f < p) {
+          const int a = S.fa[f], b = S.fb[f];
+          v = a >= 0 ? xr[a] : 1.0;
+          if (b >= 0) v *= xr[b];
+        }
+        S.u.fitb.F[row][f] = v;
+      }
+    }
+    __syncthreads();
+    if (nw == 2) bf16_chunk<2>(S, lo, hi, oa, ob, neg, acc);
+    else if (nw == 1) bf16_chunk<1>(S, lo, hi, oa, ob, neg, acc);
+    stream_store(X, S.xs[buf ^ 1], pf);
+    __syncthreads();
+    buf ^= 1;
+  }
+  // The systems overwrite F and QB: every wave passed the loop's last barrier.
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const int t = 16 * (wave + 8 * s) + lo;
+    if (s < nw && t < ntri)
+#pragma unroll
+      for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) S.u.G[16 * mt + 4 * hi + r][t] = acc[s][mt][r];
   }
   if (wave < 4) {
     bad |= __shfl_xor(bad, 16, kWave);
@@ -644,7 +822,8 @@ __global__ __launch_bounds__(kAThreads, 4) void anneal_kernel(art_split_args arg
   const int MI = a.max_iters;
   for (int k = 0; k < K; ++k) {
     const double T = a.t0 * pow(a.t1 / a.t0, (double)k / (double)max(K - 1, 1));
-    buf = anneal_fit(C, S, buf, 0, T, C.floor);
+    if (a.child_precision == ART_PRECISION_MIXED) buf = anneal_fit_bf16(C, S, buf, T, C.floor);
+    else buf = anneal_fit(C, S, buf, 0, T, C.floor);
     anneal_solve(C, S);
     buf = anneal_stats(C, S, buf, T);
     if (tid < kAR) {
@@ -739,6 +918,8 @@ extern "C" int art_split_anneal_batched(const art_split_args* args, int problems
     return fail(ART_ERR_ARG, "art_split_anneal_batched: needs iterations and positive temperatures");
   if (args->row_offsets)
     return fail(ART_ERR_ARG, "art_split_anneal_batched: nodes must have n rows each");
+  if (args->child_precision != ART_PRECISION_FP64 && args->child_precision != ART_PRECISION_MIXED)
+    return fail(ART_ERR_ARG, "art_split_anneal_batched: child_precision must be FP64 or MIXED");
   if (!args->X || !args->y || !args->theta0 || !args->node || !args->theta_out ||
       !args->beta_out || !args->hard || !args->info || !args->history || !args->scalars)
     return fail(ART_ERR_ARG, "art_split_anneal_batched: null device pointer");

@@ -771,53 +771,6 @@ __global__ __launch_bounds__(256) void reduce_leaf_partials_kernel(
 //    failed) is flagged for the exact FP64 path, which then runs only on the
 //    flagged leaves' chunks.
 
-typedef float f32x2 __attribute__((ext_vector_type(2)));
-typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-
-// Round two floats to bf16 (RNE) with one v_cvt_pk_bf16_f32.  The operands
-// are FP32 values (products of FP32 table entries), so no double->bf16
-// conversion (emulated by a round-to-odd sequence) can form; no asm barrier
-// is used, since a volatile one orders every LDS read behind the previous
-// pair's conversions.
-__device__ inline unsigned pack_bf16(float a, float b) {
-  const f32x2 v = {a, b};
-  return __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2));
-}
-
-// x minus the low (c = (-1, 0)) or high (c = (0, -1)) bf16 of the packed
-// word u, in one v_dot2_f32_bf16: both products are exact and the remainders
-// taken here are exactly representable in FP32, so the result is exact.
-__device__ inline float sub_bf16(unsigned u, unsigned c, float x) {
-  return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, u),
-                                         __builtin_bit_cast(bf16x2, c), x, false);
-}
-
-// The bf16 pairs (-1, 0) and (0, -1) for sub_bf16, held in scalar
-// registers: this compiler encodes the (-1, 0) pair as the inline constant
-// -1.0, which v_dot2c_f32_bf16 reads as the FP32 bit pattern 0xbf800000, i.e.
-// the pair (0, -1).  The asm (not volatile, so free to be hoisted) hides the
-// value from constant folding.
-struct NegPairs {
-  unsigned low, high;
-};
-__device__ inline NegPairs neg_pairs() {
-  unsigned lo = 0x0000bf80u, hi = 0xbf800000u;
-  asm("" : "+s"(lo), "+s"(hi));
-  return NegPairs{lo, hi};
-}
-
-// Split a pair of floats into packed bf16 hi, mid and lo words:
-// v = hi + mid + lo + O(2^-27 v).  Each remainder v - hi (then r - mid) is
-// exact in FP32: it has at most 16 significant bits.
-__device__ inline void split3_pair(float a, float b, const NegPairs& c, unsigned& hi,
-                                   unsigned& mid, unsigned& lo) {
-  hi = pack_bf16(a, b);
-  const float ra = sub_bf16(hi, c.low, a), rb = sub_bf16(hi, c.high, b);
-  mid = pack_bf16(ra, rb);
-  lo = pack_bf16(sub_bf16(mid, c.low, ra), sub_bf16(mid, c.high, rb));
-}
-
-typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 // Split-bf16 Gram.  One thread per row loads a block of kMixedRows rows
 // straight into registers (the next block's loads are issued before the

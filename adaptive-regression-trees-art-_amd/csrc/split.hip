@@ -893,6 +893,10 @@ extern "C" int art_split_optimize(const art_split_args* args, int problems, hipS
       args->max_backtracks < 0 ||
       args->max_failures < 1 || args->mode < 0 || args->mode > 2 || args->anneal_iters < 0)
     return fail(ART_ERR_ARG, "art_split_optimize: invalid sizes or controls");
+  if (args->child_precision != ART_PRECISION_FP64)
+    return fail(ART_ERR_ARG,
+                "art_split_optimize: FP64 child systems only (split-bf16 children are an "
+                "option of art_split_anneal_batched)");
   if (!args->X || !args->y || !args->temperature || !args->theta0 || !args->node ||
       !args->theta_out || !args->beta_out || !args->hard || !args->info || !args->history ||
       !args->scalars)

@@ -211,40 +211,66 @@ def bench_sampler(args):
 
 
 def bench_split(args):
-    """Annealed split leg: nodes of n=8192 points in d=16 (8-bump target), 64
-    random restarts each, affine children (p=17, ridge 1e-4), T from 0.2 to
-    0.01 over 200 iterations, step 0.05; one launch runs every (node, restart)
-    problem and its hard refit."""
+    """Annealed split leg: nodes of n=8192 points in d=16 (8-bump target,
+    generated on the device), 64 random restarts each, affine children
+    (p=17, ridge 1e-4), T from 0.2 to 0.01 over 200 iterations, step 0.05;
+    one launch runs every (node, restart) problem and its hard refit.  The
+    value is the split-bf16 child-Gram mode (the configuration's precision);
+    the FP64 mode runs on the same starts beside it, and both are compared."""
     import torch
 
     from art import AffineRidgeModel, BatchedOptimizerWorkspace, SoftObliqueSplitter
-    from art.synthetic import split_workload
+    from art.synthetic import split_workload_device
 
     K, R, iters = args.split_nodes, 64, 200
-    X, y = split_workload(K, seed=3)
+    dev = torch.device("cuda", 0)
+    X, y, _ = split_workload_device(K, dev, seed=3)
+    n, d = int(X.shape[1]), int(X.shape[2])
     s = SoftObliqueSplitter(AffineRidgeModel(1e-4), 0.2, n_restarts=R, random_state=3)
-    ws = BatchedOptimizerWorkspace(s, X, y)
-    times = []
-    res = ws.anneal(None, iters=iters, timing=times)
-    starts = res["starts"]
-    init_ms, anneal_ms = times
+    ws = BatchedOptimizerWorkspace.allocate(s, K, n, d)
+    ws.load_device(X, y)
+    init = []
+    starts = ws._device_starts(np.tile(np.arange(K), R), init).reshape(R, K, d + 1)
+    runs = {}
+    for prec in ("mixed", "mixed", "fp64"):  # the first mixed launch is the warmup
+        t = []
+        runs[prec] = (ws.anneal(starts, iters=iters, timing=t, child_precision=prec), t[-1])
+    (res, anneal_ms), (ref, fp64_ms) = runs["mixed"], runs["fp64"]
+    init_ms = init[0]
     ms = init_ms + anneal_ms
-    n, d, p = X.shape[1], X.shape[2], ws.p
-    # Algorithmic FP64 flops per point, problem and iteration: two weighted
-    # Grams and right-hand sides (2 x 2 (p+1)^2 / 2 symmetric) plus residuals,
-    # gradient and gates (2 x 2p + 2d + 20).
+    p = ws.p
+    # Algorithmic flops per point, problem and iteration: two weighted Grams
+    # and right-hand sides (2 x 2 (p+1)^2 / 2 symmetric) plus residuals,
+    # gradient and gates (2 x 2p + 2d + 20), all counted at FP64 weight.
     per_point = 2 * (p + 1) * (p + 1) + 4 * p + 2 * d + 20
     flops = float(K) * R * iters * n * per_point
-    print(f"[bench] split {ms:.1f} ms ({init_ms:.1f} starts + {anneal_ms:.1f} annealer) for "
-          f"{K} nodes x {R} restarts", file=sys.stderr, flush=True)
+    ok = res["ok"] & ref["ok"]
+    dtheta = np.abs(res["theta"] - ref["theta"]).max(axis=-1)[ok]
+    dloss = (np.abs(res["soft_loss_history"] - ref["soft_loss_history"])
+             / np.abs(ref["soft_loss_history"])).max(axis=-1)[ok]
+    print(f"[bench] split {ms:.1f} ms ({init_ms:.1f} starts + {anneal_ms:.1f} annealer, "
+          f"FP64 children {fp64_ms:.1f}) for {K} nodes x {R} restarts", file=sys.stderr,
+          flush=True)
     result = {
         "metric": "annealed node splits/s", "value": K / (ms * 1e-3), "unit": "node splits/s",
         "ms_per_step": ms, "starts_ms": init_ms, "annealer_ms": anneal_ms, "problems": K * R,
-        "dtype": "f64 (MFMA child Grams, FP64 Cholesky)",
+        "warmup": 1, "steps": 1,
+        "dtype": ("bf16 3-way split child Grams (6 MFMA products, f32 acc into f64); f64 gates, "
+                  "predictions, gradients, Cholesky and hard refit"),
+        "fp64_children": {"annealer_ms": fp64_ms, "node_splits_per_s": K / ((init_ms + fp64_ms)
+                                                                           * 1e-3),
+                          "speedup_of_mixed": fp64_ms / anneal_ms},
+        "err_vs_fp64_children": {
+            "theta_max_abs": float(dtheta.max()), "theta_p99_abs": float(np.quantile(dtheta, .99)),
+            "soft_loss_max_rel": float(dloss.max()),
+            "hard_loss_max_rel": float((np.abs(res["loss"] - ref["loss"]) / np.abs(ref["loss"]))[ok].max()),
+            "same_partition": float(np.mean(res["n_right"][ok] == ref["n_right"][ok])),
+            "same_best_restart": float(np.mean(res["best"] == ref["best"]))},
         "error_codes": {str(k): int(v) for k, v in
                         zip(*np.unique(res["info"][..., 0], return_counts=True))},
-        "config": {"workload": (f"{K} nodes x 64 restarts, n=8192, d=16, affine p=17, "
-                                "ridge 1e-4, T 0.2->0.01 over 200 iterations, step 0.05, seed 3"),
+        "config": {"workload": (f"{K} nodes x 64 restarts, n=8192, d=16 (rows on the device, "
+                                "8-bump targets), affine p=17, ridge 1e-4, T 0.2->0.01 over 200 "
+                                "iterations, step 0.05, seed 3"),
                    "nodes": K, "restarts": R, "iterations": iters},
         "failed_problems": int((~res["ok"]).sum()),
         "roofline": {"bound": "mfma", "achieved": flops / (anneal_ms * 1e-3) / 1e12,
@@ -252,7 +278,8 @@ def bench_split(args):
                      "frac": flops / (anneal_ms * 1e-3) / 1e12 / PEAK_TFLOPS["fp64"],
                      "algorithmic_flops_per_launch": flops,
                      "flops_per_point_iteration": per_point,
-                     "kernel": "anneal_kernel (16 starts per workgroup)"},
+                     "kernel": "anneal_kernel (16 starts per workgroup, split-bf16 child Grams)",
+                     "note": "algorithmic flops at FP64 weight against the FP64 MFMA peak"},
     }
     if not args.no_cpu:
         from threadpoolctl import threadpool_limits
@@ -261,16 +288,22 @@ def bench_split(args):
 
         so = O.Settings(ridge=1e-4, degree=1, bias_last=True)
         k = 2
+        Xh, yh = X[:k].cpu().numpy(), y[:k].cpu().numpy()
         with threadpool_limits(1):
             t0 = time.perf_counter()
             for q in range(k):
-                O.annealed(X[q], y[q], starts[q, 0], so, iters=iters, t0=0.2, t1=0.01, step=0.05)
+                th, _ = O.annealed(Xh[q], yh[q], starts[0, q], so, iters=iters, t0=0.2, t1=0.01,
+                                   step=0.05)
             dt = time.perf_counter() - t0
+        result["err_vs_oracle"] = {"theta_max_abs_first_start": float(np.max(np.abs(
+            O.project_unit_w(th) - res["theta"][0, k - 1])))}
         result["cpu_baseline"] = {"value": k / dt / R, "unit": "node splits/s", "cores": 1,
                                   "kind": "port",
                                   "sample": (f"{k} (node, restart) problems, NumPy FP64 annealed "
                                              f"loop, {dt:.2f} s; node splits/s = problems/s / 64")}
     ws.close()
+    del X, y
+    torch.cuda.empty_cache()
     return result
 
 
@@ -671,6 +704,7 @@ def main():
     del X, y, w
     torch.cuda.empty_cache()
     if args.only == "all":
+        result["split_optimizer"] = bench_split(args)
         result["hit_and_run"] = bench_sampler(args)
         result["tree_builds"] = bench_tree(args)
         result["serving"] = bench_serving(args)

@@ -204,6 +204,14 @@ typedef struct {
   const int64_t* row_offsets;
   int min_side_points;
   double min_side_fraction;
+  /* Child-system precision of the annealed soft fits (art_split_anneal_batched
+   * only; 0 elsewhere): ART_PRECISION_FP64, or ART_PRECISION_MIXED for the
+   * split-bf16 MFMA product of the 32 child systems (bf16 hi + mid + lo of
+   * each side weight and packed product, six products on
+   * v_mfma_f32_16x16x32_bf16, FP32 accumulation per 32 rows into FP64).
+   * Gates, predictions, gradients, solves and the final hard refit stay
+   * FP64. */
+  int child_precision;
 } art_split_args;
 
 /* Runs every problem of a cohort to completion in one launch (p <= 32,
