@@ -4,13 +4,17 @@
 // chains).  Per move a lane draws a direction, recomputes every slack
 // s_i = b_i - A_i.x exactly in FP64 together with A_i.d (the constraint rows
 // are uniform across the wave and arrive through scalar loads), takes the
-// chord [max_{A_i.d<0} s_i/A_i.d, min_{A_i.d>0} s_i/A_i.d] and moves to
-// t = t_lo + u (t_hi - t_lo) (art/sampling.py:877-962,
-// art/_numba_sampling.py:9-40).  Recomputing the slack instead of updating it
-// incrementally keeps every recorded sample within rounding of A x <= b for
-// any chain length and needs no per-chain slack storage.  Ratios use a
-// reciprocal (v_rcp_f64 and two Newton steps) times the slack, a few ulp from
-// the quotient.
+// chord [max_{A_i.d<0} s_i/A_i.d, min_{A_i.d>0} s_i/A_i.d] and moves to a
+// uniform point of it (art/sampling.py:877-962, art/_numba_sampling.py:9-40).
+// Recomputing the slack instead of updating it incrementally needs no
+// per-chain slack storage and keeps rounding from accumulating along a chain.
+// Ratios use a reciprocal (v_rcp_f64 and two Newton steps) times the slack, a
+// few ulp from the quotient.  Feasibility by construction: the point is drawn
+// from the chord shrunk by kChordMargin of its length at each end,
+// t = (t_lo + m) + u ((t_hi - m) - (t_lo + m)), m = kChordMargin (t_hi - t_lo),
+// so a binding row keeps a slack of ~2^-40 of the chord times |A_i.d|, far
+// above the few-ulp error of the ratios and of evaluating A x - b; the
+// sampled distribution moves by 2^-40 of the chord length.
 //
 // Schedule (one rule for the reference's batched and scalar samplers): chain
 // c records after moves first, first + stride, ..., K_c records in all, with
@@ -27,6 +31,17 @@
 
 #include "art_common.h"
 #include "art_hip.h"
+
+namespace art {
+// Relative margin kept inside each end of a chord (2^-40).
+constexpr double kChordMargin = 0x1p-40;
+
+__device__ __forceinline__ double chord_point(double tlo, double thi, double u) {
+  const double m = kChordMargin * (thi - tlo);
+  const double lo = tlo + m, hi = thi - m;
+  return (hi - lo) * u + lo;
+}
+}  // namespace art
 
 #include <utility>
 
@@ -223,7 +238,7 @@ __device__ __attribute__((always_inline)) inline void har_chain_lane(const art_h
       tlo = (ad < 0.0 && ratio > tlo) ? ratio : tlo;
       thi = (ad > 0.0 && ratio < thi) ? ratio : thi;
     }
-    const double t = (thi - tlo) * u + tlo;
+    const double t = chord_point(tlo, thi, u);
 #pragma unroll
     for (int j = 0; j < DM; ++j) x[j] += dir[j] * t;
     const int done = s + 1;
@@ -377,7 +392,7 @@ __device__ __attribute__((always_inline)) inline void chain_run(const art_har_ar
           thi = (ad > 0.0 && ratio < thi) ? ratio : thi;
         }
       }
-      const double t = (thi - tlo) * e[2 * DM] + tlo;
+      const double t = chord_point(tlo, thi, e[2 * DM]);
 #pragma unroll
       for (int j = 0; j < DM; ++j) x[j] += e[j] * t;
       if (--until_record == 0) {  // moves first, first + stride, ...

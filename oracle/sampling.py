@@ -119,14 +119,28 @@ def step_interval(A, b, x, direction, tol=1e-10):
     return float(lo), float(hi)
 
 
-def advance(A, points, slack, directions, uniforms):
+# The device sampler's chord margin: the point is drawn from the chord shrunk
+# by this fraction of its length at each end (feasibility by construction).
+# The reference's rule is margin 0.
+DEVICE_CHORD_MARGIN = 2.0 ** -40
+
+
+def chord_point(t_lo, t_hi, u, margin=0.0):
+    """t_lo + u (t_hi - t_lo) on the chord shrunk by margin (t_hi - t_lo) at
+    each end (art/sampling.py:877-962 at margin 0)."""
+    m = margin * (t_hi - t_lo)
+    lo, hi = t_lo + m, t_hi - m
+    return (hi - lo) * u + lo
+
+
+def advance(A, points, slack, directions, uniforms, margin=0.0):
     """art/sampling.py:877-962: one move of every chain, in place."""
     Ad = directions @ A.T
     with np.errstate(divide="ignore", invalid="ignore"):
         ratios = np.where(Ad != 0.0, slack / Ad, 0.0)
     t_low = np.max(np.where(Ad < 0.0, ratios, -np.inf), axis=1)
     t_high = np.min(np.where(Ad > 0.0, ratios, np.inf), axis=1)
-    step = (t_high - t_low) * uniforms + t_low
+    step = chord_point(t_low, t_high, uniforms, margin)
     points += directions * step[:, None]
     slack -= Ad * step[:, None]
 
@@ -140,7 +154,7 @@ def shaping_matrix(eigenvectors, eigenvalues, floor_ratio):
     return np.asarray(eigenvectors, dtype=float) * s[None, :]
 
 
-def run_chains(A, b, starts, n, first, stride, draw):
+def run_chains(A, b, starts, n, first, stride, draw, margin=0.0):
     """Generic schedule: chain c makes first + (K_c - 1) * stride moves and
     records after moves first, first + stride, ...; K_c = n // C + (c < n % C);
     record k of chain c is sample k * C + c.  Requires 1 <= C <= n.
@@ -161,7 +175,7 @@ def run_chains(A, b, starts, n, first, stride, draw):
         live = np.flatnonzero(moves > step)
         dirs, us = draw(step, live)
         p, s = points[live], slack[live]
-        advance(A, p, s, np.array(dirs, dtype=float), np.asarray(us, dtype=float))
+        advance(A, p, s, np.array(dirs, dtype=float), np.asarray(us, dtype=float), margin)
         points[live], slack[live] = p, s
         done = step + 1
         if done >= first and (done - first) % stride == 0:
@@ -171,7 +185,7 @@ def run_chains(A, b, starts, n, first, stride, draw):
 
 
 def batched_sample(A, b, starts, n, burn_in, thinning, directions, uniforms,
-                   tail_directions=None, tail_uniforms=None):
+                   tail_directions=None, tail_uniforms=None, margin=0.0):
     """BatchedHitAndRunSampler.sample with given arrays (art/sampling.py:618-803):
     directions (burn_in + full*stride, C, d), uniforms (same, C); tail arrays
     (stride, rem, d) / (stride, rem) for the first rem = n % C chains."""
@@ -183,10 +197,10 @@ def batched_sample(A, b, starts, n, burn_in, thinning, directions, uniforms,
             return directions[step][live], uniforms[step][live]
         return tail_directions[step - n_full][live], tail_uniforms[step - n_full][live]
 
-    return run_chains(A, b, starts, n, burn_in + stride, stride, draw)
+    return run_chains(A, b, starts, n, burn_in + stride, stride, draw, margin)
 
 
-def scalar_sample(A, b, x0, n, burn_in, thinning, directions, uniforms):
+def scalar_sample(A, b, x0, n, burn_in, thinning, directions, uniforms, margin=0.0):
     """HitAndRunSampler.sample with given arrays (art/sampling.py:445-494):
     directions (burn_in + n*thinning, d) normalized before use; uniforms
     (same,) mapped to t_lo + u (t_hi - t_lo)."""
@@ -199,7 +213,7 @@ def scalar_sample(A, b, x0, n, burn_in, thinning, directions, uniforms):
             continue
         d = d / nd
         lo, hi = step_interval(A, b, x, d)
-        x = x + (lo + uniforms[step] * (hi - lo)) * d
+        x = x + chord_point(lo, hi, uniforms[step], margin) * d
         if step >= burn_in and (step - burn_in) % thinning == 0:
             out.append(x.copy())
     return np.asarray(out)

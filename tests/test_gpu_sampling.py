@@ -16,6 +16,9 @@ from oracle import sampling as S
 
 pytestmark = pytest.mark.gpu
 
+# The device draws from the chord shrunk by this fraction at each end.
+M = MARGIN = S.DEVICE_CHORD_MARGIN
+
 
 def _cut_box(rng, d, k, lo=-1., hi=1.):
     A, b = S.box_region(np.full(d, lo), np.full(d, hi))
@@ -71,7 +74,7 @@ def test_fixed_stream_matches_the_batched_oracle(torch_gpu, C, n, burn, thin, sp
     dirs, us = rng.normal(size=(steps, C, 3)), rng.random((steps, C))
     td = rng.normal(size=(thin, rem, 3)) if rem else None
     tu = rng.random((thin, rem)) if rem else None
-    expected = S.batched_sample(A, b, starts, n, burn, thin, dirs, us, td, tu)
+    expected = S.batched_sample(A, b, starts, n, burn, thin, dirs, us, td, tu, margin=M)
     D, U = _pack(dirs, us, td, tu, C)
     got = PolytopeBatch(([A], [b]), split_box=split_box).sample(
         starts[None], n, first=burn + thin, stride=thin, directions=D[None], uniforms=U[None])
@@ -95,7 +98,7 @@ def test_many_polytopes_of_different_sizes_in_one_launch(torch_gpu):
     for p in range(P):
         nf = burn + full * thin
         exp = S.batched_sample(As[p], bs[p], starts[p], n, burn, thin, dirs[p, :nf], us[p, :nf],
-                               dirs[p, nf:, :rem], us[p, nf:, :rem])
+                               dirs[p, nf:, :rem], us[p, nf:, :rem], margin=M)
         np.testing.assert_allclose(got[p], exp, rtol=1e-12, atol=1e-12)
 
 
@@ -109,7 +112,7 @@ def test_scalar_schedule_matches_the_scalar_oracle(torch_gpu, thin):
     rng = np.random.default_rng(29)
     steps = burn + n * thin
     dirs, us = rng.normal(size=(steps, 2)), rng.random(steps)
-    expected = S.scalar_sample(A, b, x0, n, burn, thin, dirs, us)
+    expected = S.scalar_sample(A, b, x0, n, burn, thin, dirs, us, margin=M)
     got = PolytopeBatch(([A], [b])).sample(x0[None, None], n, first=burn + 1, stride=thin,
                                            directions=dirs[None, :, None], uniforms=us[None, :, None])
     np.testing.assert_allclose(got[0].cpu().numpy(), expected, rtol=1e-12, atol=1e-12)
@@ -127,7 +130,7 @@ def test_shaped_directions_match_the_oracle(torch_gpu):
     starts = np.array([[0., 0.], [.5, -.5], [-.75, .25], [1., .5]])
     steps = burn + (n // C) * thin
     dirs, us = rng.normal(size=(steps, C, 2)), rng.random((steps, C))
-    expected = S.batched_sample(A, b, starts, n, burn, thin, dirs @ M.T, us)
+    expected = S.batched_sample(A, b, starts, n, burn, thin, dirs @ M.T, us, margin=MARGIN)
     got = PolytopeBatch(([A], [b])).sample(starts[None], n, first=burn + thin, stride=thin,
                                            shape=M, directions=dirs[None], uniforms=us[None])
     np.testing.assert_allclose(got[0].cpu().numpy(), expected, rtol=1e-12, atol=1e-12)
@@ -146,7 +149,7 @@ def test_philox_stream_first_move_matches_the_oracle(torch_gpu, d):
     for p, (A, b, star) in enumerate(regs):
         dirs, us = S.step_randoms(seed, p, np.arange(C), 0, d)
         pts, slack = starts[p].copy(), b[None] - starts[p] @ A.T
-        S.advance(A, pts, slack, dirs.astype(float), us)
+        S.advance(A, pts, slack, dirs.astype(float), us, M)
         np.testing.assert_allclose(got[p], pts, rtol=1e-5, atol=1e-6)
 
 
@@ -269,7 +272,7 @@ def test_single_chain_with_cuts_matches_the_scalar_oracle(torch_gpu):
         np.stack([r[2] for r in regs])[:, None], n, first=burn + 1, stride=thin,
         directions=dirs, uniforms=us).cpu().numpy()
     for p, (A, b, star) in enumerate(regs):
-        exp = S.scalar_sample(A, b, star, n, burn, thin, dirs[p, :, 0], us[p, :, 0])
+        exp = S.scalar_sample(A, b, star, n, burn, thin, dirs[p, :, 0], us[p, :, 0], margin=M)
         np.testing.assert_allclose(got[p], exp, rtol=1e-12, atol=1e-12)
 
 
@@ -294,3 +297,41 @@ def test_single_chain_understated_row_bound_replays_safely(torch_gpu, d, k):
     np.testing.assert_array_equal(low, exact)
     for (A, b, _), pts in zip(regs, low):
         assert np.all(pts @ A.T <= b + 1e-9)
+
+
+def test_multi_move_fixed_stream_on_hit_and_run_workload_polytopes(torch_gpu):
+    """The hit-and-run configuration's shapes: d = 12 polytopes of the box and
+    4-40 random cuts (synthetic.polytope_workload), 64 chains per polytope,
+    200 moves recorded every 10 (20 records per chain), fixed directions and
+    uniforms: the device equals the oracle's batched sampler (with the
+    device's chord margin) over the whole chains, and every sample keeps a
+    strictly positive slack on every row."""
+    from art.sampling import PolytopeBatch
+    from art.synthetic import polytope_workload
+
+    d, C, thin, records = 12, 64, 10, 20
+    wl = polytope_workload(n_polytopes=6, d=d, seed=44)
+    n, steps = C * records, thin * records
+    rng = np.random.default_rng(45)
+    P = wl.n_polytopes
+    dirs, us = rng.normal(size=(P, steps, C, d)), rng.random((P, steps, C))
+    starts = np.repeat(wl.starts[:, None, :], C, axis=1)
+    got = PolytopeBatch.from_csr(wl.A, wl.b, wl.offsets).sample(
+        starts, n, first=thin, stride=thin, directions=dirs, uniforms=us).cpu().numpy()
+    for q in range(P):
+        A, b = wl.polytope(q)
+        exp = S.batched_sample(A, b, starts[q], n, 0, thin, dirs[q], us[q], margin=M)
+        # Chord ends move with x, so the few-ulp differences of the device's
+        # slacks and reciprocal ratios (b - a.x cancels near a face) grow along
+        # a chain; the oracle itself, rerun with every uniform moved by one
+        # ulp, spreads by up to 2e-11 on these chains.  A wrong chord (sign,
+        # row, face) is off by O(1) at once.  So: the first five records of
+        # every chain (50 moves) agree to 1e-12, every coordinate of all 20
+        # records to 1e-8 (measured 4e-9), and the median chain to 1e-10
+        # (measured 1e-12).
+        # Record k of chain c is sample k * C + c.
+        diff = np.abs(got[q] - exp)
+        assert diff.max() <= 1e-8, diff.max()
+        assert np.median(diff.reshape(records, C, d).max(axis=(0, 2))) <= 1e-10
+        assert np.abs(got[q] - exp)[:5 * C].max() <= 1e-12
+        assert np.all(b[None, :] - got[q] @ A.T > 0.0)

@@ -134,3 +134,37 @@ def test_shaping_matrix_floor_and_scale():
     # give sqrt scales [1, sqrt(5e-5)].
     M = S.shaping_matrix(np.eye(2), [1., 0.], 1e-4)
     np.testing.assert_allclose(np.diag(M), [1., np.sqrt(5e-5)])
+
+
+def test_device_chord_margin_stays_inside_and_reduces_to_the_reference_rule():
+    """chord_point at margin 0 is t_lo + u (t_hi - t_lo) (the reference's
+    step, art/sampling.py:877-962); at the device margin every point lies at
+    least margin (t_hi - t_lo) inside each end, the ends map to exactly that
+    distance, and the move differs from the reference's by at most
+    margin (t_hi - t_lo)."""
+    rng = np.random.default_rng(3)
+    lo, hi = rng.uniform(-2, 0, 1000), rng.uniform(0, 3, 1000)
+    u = rng.random(1000)
+    np.testing.assert_array_equal(S.chord_point(lo, hi, u), (hi - lo) * u + lo)
+    m = S.DEVICE_CHORD_MARGIN
+    t = S.chord_point(lo, hi, u, m)
+    L = hi - lo
+    assert np.all(t - lo >= 0.999 * m * L) and np.all(hi - t >= 0.999 * m * L)
+    assert np.all(np.abs(t - S.chord_point(lo, hi, u)) <= 2.001 * m * L)
+    np.testing.assert_allclose(S.chord_point(lo, hi, np.zeros(1000), m) - lo, m * L, rtol=1e-3)
+    np.testing.assert_allclose(hi - S.chord_point(lo, hi, np.ones(1000), m), m * L, rtol=1e-3)
+    # On a thin cut box every sample keeps a strictly positive slack.
+    A, b, star = _cut_box_for_margin(rng)
+    C, n = 8, 400
+    dirs, us = rng.normal(size=(n // C * 5, C, 3)), rng.random((n // C * 5, C))
+    out = S.batched_sample(A, b, np.repeat(star[None], C, axis=0), n, 0, 5, dirs, us,
+                           margin=m)
+    assert np.all(b[None, :] - out @ A.T > 0.0)
+
+
+def _cut_box_for_margin(rng):
+    A, b = S.box_region(np.full(3, -1.0), np.full(3, 1.0))
+    a = rng.normal(size=(6, 3))
+    a /= np.linalg.norm(a, axis=1, keepdims=True)
+    star = rng.uniform(-.5, .5, 3)
+    return np.vstack([A, a]), np.concatenate([b, a @ star + 0.01]), star
