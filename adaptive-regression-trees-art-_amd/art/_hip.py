@@ -19,6 +19,7 @@ LIB_PATH = LIB_DIR / "libart_hip.so"
 ART_OK = 0
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
+PRECISION_FP64_REFINED = 2
 
 _lib = None
 
@@ -89,6 +90,7 @@ _PROTOTYPES = {
     "art_gated_ridge_fit": (_i, [_p, _p, _p, _p, _p, _p, _i, _d, _d, _p, _p, _p, _p]),
     "art_plan_info": (_i, [_p, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "art_leaf_fallback_count": (_i, [_p, ctypes.POINTER(_i)]),
+    "art_leaf_rcond": (_i, [_p, _p, _p, _p]),
     "art_leaf_set_refine_tol": (_i, [_p, _d]),
     "art_leaf_set_refine_steps": (_i, [_p, _i]),
     "art_plan_timing_enable": (_i, [_p, _i]),

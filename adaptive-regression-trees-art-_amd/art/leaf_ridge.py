@@ -18,7 +18,8 @@ import numpy as np
 from . import _hip
 from .features import _columns
 
-PRECISIONS = {"fp64": _hip.PRECISION_FP64, "mixed": _hip.PRECISION_MIXED}
+PRECISIONS = {"fp64": _hip.PRECISION_FP64, "mixed": _hip.PRECISION_MIXED,
+              "fp64_refined": _hip.PRECISION_FP64_REFINED}
 DEFAULT_CHUNK_ROWS = 4096  # the library's default for leaf plans (chunk_rows=0)
 
 
@@ -83,6 +84,9 @@ class LeafRidgeBatch:
         precision="mixed" (p <= 64): split-bf16 MFMA Gram with an exact FP64
         diagonal and right-hand side, one FP64 residual refinement, and the
         FP64 path for every leaf whose correction exceeds the tolerance.
+        precision="fp64_refined" (p <= 64): the FP64 fit followed by FP64
+        data-space refinement to the least-squares solution (the "auto"
+        solver's answer for ill-conditioned leaves).
         """
         if self.plan is None:
             raise RuntimeError("The plan is closed.")
@@ -115,6 +119,17 @@ class LeafRidgeBatch:
                                           float(weight_floor), _hip.ptr(grams), _hip.ptr(bad),
                                           _hip.stream_handle(self.stream)))
         return grams, bad
+
+    def rcond(self):
+        """Device tensor (K,): 1 / (|A|_1 |A^-1|_1) of each leaf's regularized
+        system from the last "mixed" or "fp64_refined" fit (0 where the fit
+        failed); the quantity LAPACK dpocon estimates (art/models.py:288-296)."""
+        if self.plan is None:
+            raise RuntimeError("The plan is closed.")
+        out = self.torch.empty(self.K, dtype=self.torch.float64, device=self.device)
+        _hip.check(self.lib.art_leaf_rcond(self.plan, _hip.ptr(self.status), _hip.ptr(out),
+                                           _hip.stream_handle(self.stream)))
+        return out
 
     def fallback_count(self):
         """Leaves that took the exact FP64 path in the last mixed fit (synchronizes)."""

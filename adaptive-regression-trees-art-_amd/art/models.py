@@ -67,8 +67,47 @@ def _design(X, degree, include_bias, bias_last):
     return ext[:, a] * ext[:, b]
 
 
-def device_fit(X, y, weights, *, degree, include_bias, bias_last, ridge, weight_floor):
-    """One weighted ridge fit on the GPU; returns (coefficients, ridge_effective).
+def _check_solver(solver, auto_rcond_threshold=DEFAULT_AUTO_RCOND_THRESHOLD):
+    """The reference's solver validation (art/models.py:460-470) and the GPU
+    path's support: "normal" and "auto"; "qr" and "svd" (host LAPACK
+    least squares in the reference) are not provided on the device."""
+    if solver not in RIDGE_SOLVERS:
+        raise ValueError(f"solver must be one of {RIDGE_SOLVERS}, got {solver!r}.")
+    if solver in ("qr", "svd"):
+        raise ValueError(f"solver={solver!r} is not available on the GPU ridge path, which "
+                         "solves the normal equations; use solver='normal' or solver='auto' "
+                         "(Cholesky with FP64 iterative refinement of ill-conditioned systems).")
+    threshold = float(auto_rcond_threshold)
+    if not np.isfinite(threshold) or not 0.0 <= threshold <= 1.0:
+        raise ValueError("auto_rcond_threshold must be finite and in [0, 1].")
+
+
+def _solve_results(solver, coef, lam, rcond, threshold):
+    """RidgeSolveResult per fit.  "normal": Cholesky, no estimate.  "auto":
+    the condition estimate sqrt(1 / min(rcond, 1)) (art/models.py:288-296)
+    and, below the threshold, the FP64-refined solution that replaces the
+    reference's QR fallback (solver_used "cholesky_refined",
+    fallback_reason "rcond_below_threshold")."""
+    out = []
+    for j in range(coef.shape[0]):
+        if solver == "normal":
+            out.append(RidgeSolveResult(coef[j], solver, "cholesky", None, None, float(lam[j])))
+            continue
+        rc = float(rcond[j])
+        ok = rc > 0.0
+        cond = float(np.sqrt(1.0 / min(rc, 1.0))) if ok else float("inf")
+        below = not ok or rc < threshold
+        out.append(RidgeSolveResult(
+            coef[j], solver, "cholesky_refined" if below else "cholesky",
+            "sqrt_gram_1norm" if ok else "condition_estimate_failure", cond, float(lam[j]),
+            fallback_reason="rcond_below_threshold" if below else None))
+    return out
+
+
+def device_fit(X, y, weights, *, degree, include_bias, bias_last, ridge, weight_floor,
+               solver="normal", auto_rcond_threshold=DEFAULT_AUTO_RCOND_THRESHOLD):
+    """One weighted ridge fit on the GPU; returns (coefficients,
+    RidgeSolveResult).
 
     Raises ``np.linalg.LinAlgError`` when the Cholesky factorization fails, as
     the reference's normal-equation solver does (art/models.py:280-286).
@@ -88,25 +127,48 @@ def device_fit(X, y, weights, *, degree, include_bias, bias_last, ridge, weight_
         raise ValueError("weights must be nonnegative.")
     if ridge < 0.0:
         raise ValueError("ridge must be nonnegative.")
+    _check_solver(solver, auto_rcond_threshold)
     dev = torch.device("cuda", torch.cuda.current_device())
     with LeafRidgeBatch(np.array([0, X.shape[0]]), X.shape[1], degree=degree,
                         include_bias=include_bias, bias_last=bias_last) as batch:
-        coef, status, lam = batch.fit(torch.from_numpy(X).to(dev), torch.from_numpy(y).to(dev),
-                                      torch.from_numpy(w).to(dev), ridge, weight_floor)
-        st = int(status.cpu()[0])
-        if st != 0:
+        coef, status, lam, rcond = _fit_batch(
+            batch, solver, auto_rcond_threshold, torch.from_numpy(X).to(dev),
+            torch.from_numpy(y).to(dev), torch.from_numpy(w).to(dev), ridge, weight_floor)
+        if status[0] != 0:
             raise np.linalg.LinAlgError(
                 "Normal-equation Cholesky factorization failed; increase ridge regularization.")
-        return coef.cpu().numpy()[0].copy(), float(lam.cpu()[0])
+        result = _solve_results(solver, coef, lam, rcond, float(auto_rcond_threshold))[0]
+        return coef[0].copy(), result
+
+
+def _fit_batch(batch, solver, threshold, X, y, w, ridge, weight_floor):
+    """Host arrays (coef, status, lambda, rcond or None) of one batch fit.
+    "auto": the FP64 Cholesky solve and its condition estimate; leaves below
+    the threshold are refitted with FP64 data-space refinement (p <= 64)."""
+    coef, status, lam = batch.fit(X, y, w, ridge, weight_floor)
+    if solver != "auto":
+        return coef.cpu().numpy(), status.cpu().numpy(), lam.cpu().numpy(), None
+    if batch.p > 64 or batch.d > 16:
+        raise ValueError("solver='auto' on the GPU needs at most 64 basis features and d <= 16 "
+                         "(its condition estimate uses the refined fit's stored factors).")
+    plain = coef.cpu().numpy()
+    rc, st, lam_h = batch.fit(X, y, w, ridge, weight_floor, precision="fp64_refined")
+    rcond = batch.rcond().cpu().numpy()
+    refined = rc.cpu().numpy()
+    st, lam_h = st.cpu().numpy(), lam_h.cpu().numpy()
+    below = (rcond < float(threshold)) | (rcond <= 0.0)
+    out = np.where(below[:, None], refined, plain)
+    return out, st, lam_h, rcond
 
 
 class _RidgeBase:
     def _fit(self, X, y, weights, weight_floor):
-        coef, lam = device_fit(X, y, weights, degree=self._degree, include_bias=self._bias,
-                               bias_last=self._bias_last, ridge=self.ridge,
-                               weight_floor=weight_floor)
+        coef, result = device_fit(X, y, weights, degree=self._degree, include_bias=self._bias,
+                                  bias_last=self._bias_last, ridge=self.ridge,
+                                  weight_floor=weight_floor, solver=self.solver,
+                                  auto_rcond_threshold=self.auto_rcond_threshold)
         self.coef_ = coef
-        self.solve_result_ = RidgeSolveResult(coef, self.solver, "cholesky", None, None, lam)
+        self.solve_result_ = result
         return self
 
     def fit(self, X, y):
@@ -225,6 +287,7 @@ def fit_models(template, samples, stream=None):
     from .leaf_ridge import LeafRidgeBatch
 
     degree, bias, bias_last = model_basis(template)
+    _check_solver(template.solver, template.auto_rcond_threshold)
     if not samples:
         return [], []
     d = int(np.asarray(samples[0][0]).shape[1])
@@ -237,10 +300,12 @@ def fit_models(template, samples, stream=None):
     with LeafRidgeBatch(offsets, d, degree=degree, include_bias=bias, bias_last=bias_last,
                         stream=stream) as batch:
         Xt = torch.from_numpy(X).to(dev)
-        coef, status, lam = batch.fit(Xt, torch.from_numpy(y).to(dev),
-                                      torch.ones(len(y), dtype=torch.float64, device=dev),
-                                      template.ridge, 1e-12)
-        coef, status, lam = coef.cpu().numpy(), status.cpu().numpy(), lam.cpu().numpy()
+        coef, status, lam, rcond = _fit_batch(
+            batch, template.solver, template.auto_rcond_threshold, Xt,
+            torch.from_numpy(y).to(dev), torch.ones(len(y), dtype=torch.float64, device=dev),
+            template.ridge, 1e-12)
+    results = _solve_results(template.solver, coef, lam, rcond,
+                             float(template.auto_rcond_threshold))
     models, predictions = [], []
     for j, (x, _) in enumerate(samples):
         if status[j] != 0:
@@ -248,8 +313,7 @@ def fit_models(template, samples, stream=None):
                 "Normal-equation Cholesky factorization failed; increase ridge regularization.")
         m = template.clone()
         m.coef_ = coef[j].copy()
-        m.solve_result_ = RidgeSolveResult(m.coef_, m.solver, "cholesky", None, None,
-                                           float(lam[j]))
+        m.solve_result_ = results[j]
         models.append(m)
         predictions.append(m.predict(x))
     return models, predictions

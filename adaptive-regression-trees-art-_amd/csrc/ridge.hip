@@ -1216,6 +1216,71 @@ __global__ __launch_bounds__(64) void correct_wave_kernel(
   }
 }
 
+// Reciprocal 1-norm condition number of each leaf's regularized system
+// A = L L^T from its stored factor (p <= PMAX <= 64), one wave per leaf:
+// lane k forms column k of A (sum_j L_ij L_kj) and column k of A^-1 (L v =
+// e_k, then L^T x = v), and rcond = 1 / (max_k |A e_k|_1 max_k |A^-1 e_k|_1).
+// This is the quantity LAPACK dpocon estimates (art/models.py:288-296 uses
+// the estimate); here it is exact up to rounding.  Failed leaves get 0.
+template <int PMAX>
+__global__ __launch_bounds__(64) void rcond_wave_kernel(const double* __restrict__ factor,
+                                                         int p, const int* __restrict__ status,
+                                                         double* __restrict__ rcond) {
+  const int leaf = blockIdx.x, lane = threadIdx.x;
+  if (status[leaf] != 0) {
+    if (lane == 0) rcond[leaf] = 0.0;
+    return;
+  }
+  extern __shared__ double Fs[];
+  const int nf = p * (p + 1) / 2;
+  const double* F = factor + (size_t)leaf * nf;
+  for (int e = lane; e < nf; e += 64) Fs[e] = F[e];
+  __syncthreads();
+  const int k = lane;
+  double colA = 0.0, colInv = 0.0;
+  if (k < p) {
+    // |A e_k|_1 with A_ik = sum_{j <= min(i,k)} L_ij L_kj.
+    for (int i = 0; i < p; ++i) {
+      const int m = i < k ? i : k;
+      double a = 0.0;
+      for (int j = 0; j <= m; ++j) a = fma(Fs[tri(i, j)], Fs[tri(k, j)], a);
+      colA += fabs(a);
+    }
+    // v = L^-1 e_k (v_i = 0 for i < k), then x = L^-T v.
+    double v[PMAX];
+#pragma unroll
+    for (int i = 0; i < PMAX; ++i) {
+      double r = (i == k) ? 1.0 : 0.0;
+      if (i > k && i < p) {
+#pragma unroll
+        for (int j = 0; j < i; ++j)
+          if (j >= k) r = fma(-Fs[tri(i, j)], v[j], r);
+      }
+      v[i] = (i >= k && i < p) ? r / Fs[tri(i, i)] : 0.0;
+    }
+#pragma unroll
+    for (int i = PMAX - 1; i >= 0; --i) {
+      if (i < p) {
+        double r = v[i];
+#pragma unroll
+        for (int j = i + 1; j < PMAX; ++j)
+          if (j < p) r = fma(-Fs[tri(j, i)], v[j], r);
+        v[i] = r / Fs[tri(i, i)];
+        colInv += fabs(v[i]);
+      }
+    }
+  }
+  // Wave maxima of the column norms.
+  for (int o = 32; o > 0; o >>= 1) {
+    colA = fmax(colA, __shfl_xor(colA, o, kWave));
+    colInv = fmax(colInv, __shfl_xor(colInv, o, kWave));
+  }
+  if (lane == 0) {
+    const double prod = colA * colInv;
+    rcond[leaf] = (prod > 0.0 && isfinite(prod)) ? 1.0 / prod : 0.0;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Feature materialization and input checks.
 
@@ -1618,11 +1683,19 @@ static void launch_gram_leaf_f64(const art_ridge_plan* P, const double* X, const
   }
 }
 
+// Mixed (fp64_gram false): split-bf16 Gram, refinement, FP64 path for the
+// leaves still flagged.  FP64-refined (fp64_gram true): FP64 Gram and
+// Cholesky, then the same data-space refinement steps with the tighter
+// acceptance kRefinedTol; a leaf still flagged keeps its refined solution.
+constexpr double kRefinedTol = 1e-8;
+
 static int leaf_fit_mixed(art_ridge_plan* P, const double* X, const double* y, const double* w,
                           double ridge, double floor, double* beta, int* status,
-                          double* lambda, hipStream_t stream) {
+                          double* lambda, hipStream_t stream, bool fp64_gram = false) {
   if (P->p > 64 || P->T > 5 || P->basis.d > kMixedMaxD)
-    return fail(ART_ERR_ARG, "art_leaf_ridge_fit: mixed precision needs p <= 64 and d <= 16");
+    return fail(ART_ERR_ARG, fp64_gram
+                                 ? "art_leaf_ridge_fit: refined FP64 needs p <= 64 and d <= 16"
+                                 : "art_leaf_ridge_fit: mixed precision needs p <= 64 and d <= 16");
   const int p = P->p, T = P->T;
   if (P->d_factor == nullptr) {
     ART_HIP(hipMalloc(&P->d_factor, (size_t)P->K * (p * (p + 1) / 2) * sizeof(double)));
@@ -1636,7 +1709,9 @@ static int leaf_fit_mixed(art_ridge_plan* P, const double* X, const double* y, c
   const dim3 cgrid(P->nchunks), sgrid(P->nsys), block(256);
   timing_mark(P, 0, stream);
   const size_t msmem = mixed_stage_smem_bytes(P->basis.d);
-  if (T == 3 && P->basis.d == 8) {
+  if (fp64_gram) {
+    launch_gram_leaf_f64(P, X, y, w, floor, nullptr, stream);
+  } else if (T == 3 && P->basis.d == 8) {
     gram_leaf_mixed_kernel<3, 8><<<cgrid, block, msmem, stream>>>(X, y, w, P->basis, floor,
                                                                  P->d_chunks, P->d_partials,
                                                                  P->d_tpart, P->d_flags);
@@ -1655,8 +1730,9 @@ static int leaf_fit_mixed(art_ridge_plan* P, const double* X, const double* y, c
   ART_HIP(hipGetLastError());
   timing_mark(P, 1, stream);
   int rc = launch_solve(P, P->nsys, P->nchunks, ridge, beta, status, lambda, stream, nullptr,
-                        P->d_factor, P->d_tpart);
+                        P->d_factor, fp64_gram ? nullptr : P->d_tpart);
   if (rc != ART_OK) return rc;
+  const double tau = fp64_gram ? kRefinedTol : P->refine_tol;
   // Refinement: one FP64 residual pass and correction for every leaf, then up
   // to refine_steps more on the leaves still flagged.  The residual uses the
   // register kernel for quadratic bases with bias-first or no bias
@@ -1698,7 +1774,7 @@ static int leaf_fit_mixed(art_ridge_plan* P, const double* X, const double* y, c
   correct_wave_kernel<PM><<<sgrid, dim3(64), (size_t)p * (p + 1) / 2 * sizeof(double),        \
                             stream>>>(                                                          \
       P->d_sys, ro, p, rs, P->d_rpart, P->d_factor, lambda, beta, status, P->d_need,        \
-      P->refine_tol, mask, P->d_dnorm)
+      tau, mask, P->d_dnorm)
       if (p <= 16) ART_CORRECT(16);
       else if (p <= 32) ART_CORRECT(32);
       else if (p <= 48) ART_CORRECT(48);
@@ -1706,6 +1782,10 @@ static int leaf_fit_mixed(art_ridge_plan* P, const double* X, const double* y, c
 #undef ART_CORRECT
     }
     ART_HIP(hipGetLastError());
+  }
+  if (fp64_gram) {
+    timing_mark(P, 2, stream);
+    return ART_OK;
   }
   // Exact FP64 path for the flagged leaves only.
   launch_gram_leaf_f64(P, X, y, w, floor, P->d_need, stream);
@@ -1726,6 +1806,8 @@ int art_leaf_ridge_fit(art_ridge_plan* P, const double* X, const double* y, cons
     return fail(ART_ERR_ARG, "art_leaf_ridge_fit: ridge and weight_floor must be finite >= 0");
   if (precision == ART_PRECISION_MIXED)
     return leaf_fit_mixed(P, X, y, w, ridge, weight_floor, beta, status, lambda, stream);
+  if (precision == ART_PRECISION_FP64_REFINED)
+    return leaf_fit_mixed(P, X, y, w, ridge, weight_floor, beta, status, lambda, stream, true);
   if (precision != ART_PRECISION_FP64)
     return fail(ART_ERR_ARG, "art_leaf_ridge_fit: unsupported precision");
   timing_mark(P, 0, stream);
@@ -1842,6 +1924,25 @@ int art_gram_solve(const double* grams, const int* bad, int K, art_basis basis, 
   const int T = tiles_for(p), npairs = T * (T + 1) / 2;
   launch_rows_solve(nullptr, K, K, T, npairs, p, ridge, grams, bad, nullptr, nullptr, nullptr,
                     beta, status, lambda, stream);
+  ART_HIP(hipGetLastError());
+  return ART_OK;
+}
+
+int art_leaf_rcond(art_ridge_plan* P, const int* status, double* rcond, hipStream_t stream) {
+  if (!P || P->kind != 0 || !status || !rcond)
+    return fail(ART_ERR_ARG, "art_leaf_rcond: bad arguments");
+  if (P->d_factor == nullptr)
+    return fail(ART_ERR_ARG,
+                "art_leaf_rcond: needs a previous mixed or refined FP64 fit (stored factors)");
+  const int p = P->p;
+  const size_t lds = (size_t)p * (p + 1) / 2 * sizeof(double);
+#define ART_RCOND(PM) \
+  rcond_wave_kernel<PM><<<dim3(P->K), dim3(64), lds, stream>>>(P->d_factor, p, status, rcond)
+  if (p <= 16) ART_RCOND(16);
+  else if (p <= 32) ART_RCOND(32);
+  else if (p <= 48) ART_RCOND(48);
+  else ART_RCOND(64);
+#undef ART_RCOND
   ART_HIP(hipGetLastError());
   return ART_OK;
 }

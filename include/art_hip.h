@@ -29,6 +29,7 @@ extern "C" {
 
 #define ART_PRECISION_FP64 0  /* FP64 Gram + FP64 Cholesky ("plain") */
 #define ART_PRECISION_MIXED 1 /* low-precision MFMA Gram + FP64 refinement */
+#define ART_PRECISION_FP64_REFINED 2 /* FP64 Gram + Cholesky + FP64 data-space refinement */
 
 #define ART_DEFAULT_CHUNK_ROWS 2048      /* equal-size node plans */
 #define ART_DEFAULT_LEAF_CHUNK_ROWS 4096 /* uneven leaf plans (launched largest chunk first) */
@@ -116,6 +117,21 @@ int art_leaf_ridge_fit_static(const int64_t* offsets, int K, art_basis basis, co
  * (or whose mixed factorization fails).  Number of leaves that took the FP64
  * path in the last mixed fit (synchronizes the device): */
 int art_leaf_fallback_count(art_ridge_plan* plan, int* count);
+
+/* ART_PRECISION_FP64_REFINED (p <= 64, d <= 16): the FP64 fit, its factor
+ * kept, then FP64 data-space refinement steps beta += (L L^T)^-1 (Phi^T W
+ * (y - Phi beta) - lambda beta) until |delta| <= 1e-8 |beta| (at most
+ * 1 + refine_steps steps).  For systems whose Cholesky succeeds this
+ * converges to the least-squares solution of the augmented weighted design,
+ * the solution the reference reaches by its QR fallback for ill-conditioned
+ * Grams (art/models.py:298-357).
+ *
+ * rcond (K,) device output: 1 / (|A|_1 |A^-1|_1) of each leaf's regularized
+ * system A = G + lambda I from the factor of the last mixed or refined fit
+ * (exact up to rounding; LAPACK dpocon, art/models.py:288-296, estimates
+ * it); 0 for failed leaves (status != 0).  Replaces the reference's dpocon
+ * call. */
+int art_leaf_rcond(art_ridge_plan* plan, const int* status, double* rcond, hipStream_t stream);
 
 /* Acceptance tolerance of the mixed refinement (default ART_MIXED_REFINE_TOL). */
 int art_leaf_set_refine_tol(art_ridge_plan* plan, double tol);

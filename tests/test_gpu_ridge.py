@@ -511,3 +511,99 @@ def test_resident_grams_of_disjoint_rows_add(torch_gpu):
     ref, st_ref, _ = R.leaf_fit(Xu, f(Xu), wu, offsets, 1e-6, 1e-12, 2, True, False)
     np.testing.assert_array_equal(status.cpu().numpy(), st_ref)
     assert _rel_err(coef.cpu().numpy(), ref).max() < 1e-9
+
+
+# --------------------------------------------------------------------------- solvers
+
+def _exact_rcond(G):
+    return 1.0 / (np.linalg.norm(G, 1) * np.linalg.norm(np.linalg.inv(G), 1))
+
+
+def test_auto_solver_on_a_well_conditioned_fit_is_the_cholesky_fit(torch_gpu):
+    from art.models import PolynomialRidgeModel
+
+    rng = np.random.default_rng(5)
+    X = rng.uniform(-1, 1, (400, 3))
+    y = np.sin(X @ np.array([1.0, -2.0, .5]))
+    normal = PolynomialRidgeModel(2, ridge=1e-6).fit(X, y)
+    auto = PolynomialRidgeModel(2, ridge=1e-6, solver="auto").fit(X, y)
+    r = auto.solve_result_
+    assert r.solver_requested == "auto" and r.solver_used == "cholesky"
+    assert r.condition_estimator == "sqrt_gram_1norm" and r.fallback_reason is None
+    phi = R.features(X, 2, True, False)
+    G = phi.T @ phi
+    G += 1e-6 * np.trace(G) / G.shape[0] * np.eye(G.shape[0])
+    assert r.cond_estimate == pytest.approx(np.sqrt(1.0 / _exact_rcond(G)), rel=1e-8)
+    np.testing.assert_allclose(auto.coef_, normal.coef_, rtol=1e-10, atol=1e-12)
+    assert normal.solve_result_.cond_estimate is None
+
+
+def test_auto_solver_refines_an_ill_conditioned_fit_to_the_least_squares_answer(torch_gpu):
+    """Nearly collinear quadratic features (x in [1, 1.02]^2, ridge 1e-15):
+    the Gram's rcond is ~1e-15, below the 1e-10 threshold, where the
+    reference switches to QR (art/models.py:298-357).  The refined answer
+    matches SciPy's gelsy least squares on the augmented design (the
+    reference's fallback) far better than the plain Cholesky answer does."""
+    from scipy import linalg
+
+    from art.models import PolynomialRidgeModel
+
+    rng = np.random.default_rng(8)
+    X = rng.uniform(1.0, 1.02, (600, 2))
+    y = np.exp(X[:, 0] - X[:, 1]) + 0.1 * X[:, 0] ** 3
+    ridge = 1e-15
+    normal = PolynomialRidgeModel(2, ridge=ridge).fit(X, y)
+    auto = PolynomialRidgeModel(2, ridge=ridge, solver="auto").fit(X, y)
+    r = auto.solve_result_
+    assert r.solver_used == "cholesky_refined" and r.fallback_reason == "rcond_below_threshold"
+    phi = R.features(X, 2, True, False)
+    G = phi.T @ phi
+    lam = ridge * np.trace(G) / G.shape[0]
+    assert r.ridge_effective == pytest.approx(lam, rel=1e-12)
+    aug = np.vstack([phi, np.sqrt(lam) * np.eye(phi.shape[1])])
+    ref = linalg.lstsq(aug, np.concatenate([y, np.zeros(phi.shape[1])]), lapack_driver="gelsy")[0]
+    err = lambda c: np.linalg.norm(phi @ c - phi @ ref) / np.linalg.norm(phi @ ref)
+    assert err(auto.coef_) <= 1e-9, err(auto.coef_)
+    assert err(auto.coef_) < 1e-2 * err(normal.coef_), (err(auto.coef_), err(normal.coef_))
+    # The estimate is the exact 1-norm reciprocal condition of G + lambda I,
+    # to the accuracy an inverse of that conditioning allows.
+    assert 1.0 / r.cond_estimate ** 2 == pytest.approx(_exact_rcond(G + lam * np.eye(6)), rel=1e-2)
+
+
+def test_refined_fp64_and_rcond_on_the_leaf_batch(torch_gpu):
+    from art import LeafRidgeBatch
+    from art.synthetic import leaf_ridge_workload
+
+    torch = torch_gpu
+    wl = leaf_ridge_workload(n_leaves=48, seed=4)
+    X, y, w = dev(torch, wl.X), dev(torch, wl.y), dev(torch, wl.w)
+    with LeafRidgeBatch(wl.offsets, 8) as batch:
+        c64, s64, _ = (t.clone() for t in batch.fit(X, y, w, wl.ridge, precision="fp64"))
+        cr, sr, _ = batch.fit(X, y, w, wl.ridge, precision="fp64_refined")
+        rc = batch.rcond().cpu().numpy()
+        np.testing.assert_array_equal(sr.cpu().numpy(), s64.cpu().numpy())
+        cr = cr.cpu().numpy()
+    from scipy import linalg
+
+    _, _, lam = R.leaf_fit(wl.X, wl.y, wl.w, wl.offsets, wl.ridge)
+    # Against a stable least-squares solve of the augmented weighted design
+    # (SVD-based gelsd): the refined fit removes the normal equations' kappa^2
+    # rounding, so it is far closer than the plain FP64 fit on the worst leaf.
+    c64 = c64.cpu().numpy()
+    e_plain, e_ref = np.zeros(wl.n_leaves), np.zeros(wl.n_leaves)
+    for j in range(wl.n_leaves):
+        a, b = wl.offsets[j], wl.offsets[j + 1]
+        phi = R.features(wl.X[a:b], 2, True, False)
+        sw = np.sqrt(np.maximum(wl.w[a:b], 1e-12))
+        aug = np.vstack([phi * sw[:, None], np.sqrt(lam[j]) * np.eye(phi.shape[1])])
+        ls = linalg.lstsq(aug, np.concatenate([wl.y[a:b] * sw, np.zeros(phi.shape[1])]),
+                          lapack_driver="gelsd")[0]
+        e_plain[j] = np.linalg.norm(c64[j] - ls) / np.linalg.norm(ls)
+        e_ref[j] = np.linalg.norm(cr[j] - ls) / np.linalg.norm(ls)
+    assert e_ref.max() <= max(1e-10, 0.1 * e_plain.max()), (e_ref.max(), e_plain.max())
+    for j in range(0, wl.n_leaves, 6):
+        a, b = wl.offsets[j], wl.offsets[j + 1]
+        phi = R.features(wl.X[a:b], 2, True, False)
+        ww = np.maximum(wl.w[a:b], 1e-12)
+        G = phi.T @ (phi * ww[:, None]) + lam[j] * np.eye(phi.shape[1])
+        assert rc[j] == pytest.approx(_exact_rcond(G), rel=1e-3), j

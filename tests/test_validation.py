@@ -61,3 +61,18 @@ def test_leaf_batch_validation_precedes_torch(monkeypatch, offsets, message):
     _guard(monkeypatch)
     with pytest.raises(ValueError, match=message):
         LeafRidgeBatch(offsets, 2)
+
+
+@pytest.mark.parametrize("solver,match", [
+    ("lu", "solver must be one of"), ("qr", "not available on the GPU ridge path"),
+    ("svd", "not available on the GPU ridge path")])
+def test_ridge_solver_literals_are_checked_before_any_device_work(solver, match):
+    # art/models.py:460-470 validates the literal; "qr" and "svd" (host
+    # LAPACK least squares in the reference) are refused by the GPU path
+    # instead of silently running Cholesky.
+    from art.models import AffineRidgeModel, PolynomialRidgeModel
+
+    X = np.random.default_rng(0).uniform(-1, 1, (20, 2))
+    for model in (AffineRidgeModel(1e-6, solver=solver), PolynomialRidgeModel(solver=solver)):
+        with pytest.raises(ValueError, match=match):
+            model.fit(X, X[:, 0])
