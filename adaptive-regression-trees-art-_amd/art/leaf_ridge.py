@@ -67,6 +67,23 @@ class LeafRidgeBatch:
         self.ridge_effective = torch.zeros(self.K, **f64)
         self.status = torch.zeros(self.K, dtype=torch.int32, device=self.device)
 
+    @classmethod
+    def autotuned(cls, offsets, d, X, y, w, ridge, *, degree=2, include_bias=True,
+                  bias_last=False, precision="fp64", stream=None, candidates=None):
+        """A batch whose chunk size the autotuner picked for these leaves
+        (``art.autotune.tune_chunk_rows``: timed once per shape key, then
+        cached); ``batch.tuning`` holds (chunk_rows, timings or None)."""
+        from . import autotune
+
+        kwargs = {} if candidates is None else {"candidates": candidates}
+        chunk, timings = autotune.tune_chunk_rows(
+            offsets, d, X, y, w, ridge, degree=degree, include_bias=include_bias,
+            bias_last=bias_last, precision=precision, stream=stream, **kwargs)
+        batch = cls(offsets, d, degree=degree, include_bias=include_bias, bias_last=bias_last,
+                    chunk_rows=chunk, stream=stream)
+        batch.tuning = (chunk, timings)
+        return batch
+
     def _check_inputs(self, X, y, w):
         torch = self.torch
         for name, t, shape in (("X", X, (self.n_rows, self.d)), ("y", y, (self.n_rows,)),
@@ -103,6 +120,38 @@ class LeafRidgeBatch:
             PRECISIONS[precision], _hip.ptr(self.coefficients), _hip.ptr(self.status),
             _hip.ptr(self.ridge_effective), _hip.stream_handle(self.stream)))
         return self.coefficients, self.status, self.ridge_effective
+
+    def capture(self, X, y, w, ridge, weight_floor=1e-12, precision="fp64"):
+        """Record one fit of these inputs as a HIP graph; returns a
+        ``LeafFitGraph`` whose ``replay()`` reruns every kernel of the fit
+        (Gram, solve, refinement rounds, fallback) as one graph launch on
+        this batch's stream, writing the same outputs as ``fit``.
+
+        The fit is run once eagerly first, so that every buffer it needs
+        exists before capture (a captured fit allocates nothing and issues
+        no memset or host synchronization).  X, y and w stay bound to the
+        graph: refill them in place between replays to fit new values.
+        Capturing switches the plan's kernel timing off (its event records
+        would otherwise enter the graph)."""
+        if self.plan is None:
+            raise RuntimeError("The plan is closed.")
+        torch = self.torch
+        _hip.check(self.lib.art_plan_timing_enable(self.plan, 0))
+        self.fit(X, y, w, ridge, weight_floor, precision)
+        self.synchronize()
+        # Capture on a stream of the graph's own (the default stream cannot be
+        # captured); the batch's fits go there for the duration.
+        side = torch.cuda.Stream(device=self.device)
+        graph = torch.cuda.CUDAGraph()
+        home = self.stream
+        try:
+            self.stream = side
+            with torch.cuda.graph(graph, stream=side):
+                self.fit(X, y, w, ridge, weight_floor, precision)
+        finally:
+            self.stream = home
+        side.synchronize()
+        return LeafFitGraph(self, graph, side, (X, y, w))
 
     def grams(self, X, y, w, weight_floor=1e-12):
         """Each leaf's FP64 augmented system, without ridge: a device tensor
@@ -165,6 +214,26 @@ class LeafRidgeBatch:
             self.close()
         except Exception:
             pass
+
+
+class LeafFitGraph:
+    """A captured leaf fit (``LeafRidgeBatch.capture``): ``replay()`` enqueues
+    the whole fit as one graph launch, ordered after the work already on the
+    batch's stream and before the work enqueued there next, and returns the
+    batch's (coefficients, status, ridge_effective) tensors."""
+
+    def __init__(self, batch, graph, stream, inputs):
+        self.batch, self.graph, self.stream, self.inputs = batch, graph, stream, inputs
+
+    def replay(self):
+        batch = self.batch
+        if batch.plan is None:
+            raise RuntimeError("The plan is closed.")
+        self.stream.wait_stream(batch.stream)
+        with batch.torch.cuda.stream(self.stream):
+            self.graph.replay()
+        batch.stream.wait_stream(self.stream)
+        return batch.coefficients, batch.status, batch.ridge_effective
 
 
 def solve_grams(grams, bad, d, ridge, *, degree=2, include_bias=True, bias_last=False,

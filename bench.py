@@ -112,6 +112,33 @@ def timed_fits(batch, lib, X, y, w, ridge, precision, steps, warmup):
             "first_kernel_ms": float(np.mean(first[:n])), "rest_ms": float(np.mean(rest[:n]))}
 
 
+def graph_replays(batch, X, y, w, ridge, precision, steps, reference):
+    """The fit captured as one HIP graph: `steps` timed replays (events on the
+    batch stream after one untimed replay) and a bitwise check of every
+    replay's coefficients against the eager fit's."""
+    import torch
+
+    graph = batch.capture(X, y, w, ridge, precision=precision)
+    graph.replay()
+    batch.synchronize()
+    stream = graph.stream  # events on the stream the graph launches on
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    equal = True
+    ev[0].record(stream)
+    for i in range(steps):
+        graph.replay()
+        ev[i + 1].record(stream)
+    batch.synchronize()
+    equal &= bool(np.array_equal(batch.coefficients.cpu().numpy(), reference, equal_nan=True))
+    step_ms = np.array([ev[i].elapsed_time(ev[i + 1]) for i in range(steps)])
+    return {"ms_per_step": float(ev[0].elapsed_time(ev[-1]) / steps),
+            "latency_ms": {"p50": float(np.percentile(step_ms, 50)),
+                           "p99": float(np.percentile(step_ms, 99))},
+            "equal_to_eager_bitwise": equal,
+            "note": "the whole mixed fit (Gram, solve, refinement rounds, FP64 fallback) as one "
+                    "graph launch"}
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed counter summary
     (profiles/hbm_traffic.json: FETCH_SIZE + WRITE_SIZE per dispatch, each
@@ -657,6 +684,15 @@ def main():
         print(f"[bench] mixed {run['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     coef = batch.coefficients.cpu().numpy()
     st = batch.status.cpu().numpy()
+    graphed = graph_replays(batch, X, y, w, wl.ridge, args.precision, args.steps, coef)
+    print(f"[bench] graph replay {graphed['ms_per_step']:.3f} ms/step, equal to eager: "
+          f"{graphed['equal_to_eager_bitwise']}", file=sys.stderr, flush=True)
+    from art import autotune
+
+    tuned_chunk, tuned_ms = autotune.tune_chunk_rows(
+        wl.offsets, 8, X, y, w, wl.ridge, precision=args.precision, force=True)
+    print(f"[bench] autotuned chunk rows {tuned_chunk} ({tuned_ms})", file=sys.stderr,
+          flush=True)
     ok = (st == 0) & (ref_status == 0)
     err = rel_err(coef[ok], ref[ok])
     fallback = batch.fallback_count() if args.precision == "mixed" else 0
@@ -714,6 +750,10 @@ def main():
         "fallback_leaves": fallback,
         "kernels_ms": {"first": kernel_ms, "rest": run["rest_ms"]},
         "failed_leaves": int((st != 0).sum()),
+        "graph_replay": graphed,
+        "autotune": {"chunk_rows_ms": {str(k): v for k, v in tuned_ms.items()},
+                     "chosen_chunk_rows": tuned_chunk,
+                     "key": "leaf_fit: d, p, basis, precision, K and N power-of-two buckets"},
         "roofline": roofline,
     }
     if not args.no_cpu:

@@ -607,3 +607,73 @@ def test_refined_fp64_and_rcond_on_the_leaf_batch(torch_gpu):
         ww = np.maximum(wl.w[a:b], 1e-12)
         G = phi.T @ (phi * ww[:, None]) + lam[j] * np.eye(phi.shape[1])
         assert rc[j] == pytest.approx(_exact_rcond(G), rel=1e-3), j
+
+
+# --------------------------------------------------------------------------- graphs, tuning
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_captured_leaf_fit_replays_equal_the_eager_fit(torch_gpu, precision):
+    """A fit captured as one HIP graph replays to the eager fit's outputs bit
+    for bit, on every replay, with eager work and synchronizations between
+    the replays; refilling the bound inputs in place and replaying equals an
+    eager fit of the new values."""
+    from art import LeafRidgeBatch
+    from art.synthetic import leaf_ridge_workload
+
+    torch = torch_gpu
+    wl = leaf_ridge_workload(n_leaves=96, seed=6)
+    X, y, w = dev(torch, wl.X), dev(torch, wl.y), dev(torch, wl.w)
+    other = leaf_ridge_workload(n_leaves=20, seed=7)
+    with LeafRidgeBatch(wl.offsets, 8, stream=torch.cuda.Stream()) as batch, \
+            LeafRidgeBatch(other.offsets, 8) as side:
+        # Outputs live on the batch's stream: synchronize before reading them.
+        fitted = batch.fit(X, y, w, wl.ridge, precision=precision)
+        batch.synchronize()
+        eager = [t.clone() for t in fitted]
+        graph = batch.capture(X, y, w, wl.ridge, precision=precision)
+        for rep in range(4):
+            with torch.cuda.stream(batch.stream):
+                batch.coefficients.fill_(float("nan"))
+                batch.status.fill_(99)
+            side.fit(dev(torch, other.X), dev(torch, other.y), dev(torch, other.w), 1e-3,
+                     precision=precision)
+            torch.cuda.synchronize()
+            out = graph.replay()
+            batch.synchronize()
+            for got, ref in zip(out, eager):
+                np.testing.assert_array_equal(got.cpu().numpy(), ref.cpu().numpy())
+        y.mul_(-0.5).add_(0.25)
+        torch.cuda.synchronize()
+        replayed = graph.replay()
+        batch.synchronize()
+        out = [t.clone() for t in replayed]
+        fitted = batch.fit(X, y, w, wl.ridge, precision=precision)
+        batch.synchronize()
+        fresh = [t.clone() for t in fitted]
+        for got, ref in zip(out, fresh):
+            np.testing.assert_array_equal(got.cpu().numpy(), ref.cpu().numpy())
+        assert not np.array_equal(out[0].cpu().numpy(), eager[0].cpu().numpy())
+
+
+def test_autotuned_chunk_rows_are_measured_once_and_fit_like_the_oracle(torch_gpu, tmp_path,
+                                                                        monkeypatch):
+    from art import LeafRidgeBatch, autotune
+    from art.synthetic import leaf_ridge_workload
+
+    torch = torch_gpu
+    monkeypatch.setenv("ART_TUNE_CACHE", str(tmp_path / "tune.json"))
+    monkeypatch.setattr(autotune, "_memory", {})
+    wl = leaf_ridge_workload(n_leaves=64, seed=9)
+    X, y, w = dev(torch, wl.X), dev(torch, wl.y), dev(torch, wl.w)
+    with LeafRidgeBatch.autotuned(wl.offsets, 8, X, y, w, wl.ridge,
+                                  candidates=(512, 4096)) as batch:
+        chunk, timings = batch.tuning
+        assert set(timings) == {512, 4096} and chunk in timings
+        assert chunk == min(timings, key=timings.get)
+        coef, status, _ = batch.fit(X, y, w, wl.ridge)
+        ref, st_ref, _ = R.leaf_fit(wl.X, wl.y, wl.w, wl.offsets, wl.ridge)
+        np.testing.assert_array_equal(status.cpu().numpy(), st_ref)
+        assert _rel_err(coef.cpu().numpy(), ref).max() < 1e-7
+    with LeafRidgeBatch.autotuned(wl.offsets, 8, X, y, w, wl.ridge,
+                                  candidates=(512, 4096)) as again:
+        assert again.tuning == (chunk, None)

@@ -76,3 +76,25 @@ def test_ridge_solver_literals_are_checked_before_any_device_work(solver, match)
     for model in (AffineRidgeModel(1e-6, solver=solver), PolynomialRidgeModel(solver=solver)):
         with pytest.raises(ValueError, match=match):
             model.fit(X, X[:, 0])
+
+
+def test_autotune_keys_and_cache_round_trip(tmp_path, monkeypatch):
+    from art import autotune
+
+    monkeypatch.setenv("ART_TUNE_CACHE", str(tmp_path / "tune.json"))
+    monkeypatch.setattr(autotune, "_memory", {})
+    k1 = autotune.tune_key(8, 45, 2, True, False, "mixed", 4096, 12_217_562)
+    # Leaf and row counts are bucketed by powers of two.
+    assert k1 == autotune.tune_key(8, 45, 2, True, False, "mixed", 4100, 9_000_000)
+    assert k1 != autotune.tune_key(8, 45, 2, True, False, "mixed", 8192, 12_217_562)
+    assert k1 != autotune.tune_key(8, 45, 2, True, False, "fp64", 4096, 12_217_562)
+    assert autotune.lookup(k1) is None
+    autotune.store(k1, 2048)
+    monkeypatch.setattr(autotune, "_memory", {})  # a fresh process reads the file
+    assert autotune.lookup(k1) == 2048
+    # A cached key runs no timing (the device is never touched here).
+    offsets = np.array([0, 3000, 12_217_562 - 7], dtype=np.int64)
+    chunk, timings = autotune.tune_chunk_rows(
+        np.concatenate([offsets[:1], np.linspace(0, 12_217_562, 4097).astype(np.int64)[1:]]),
+        8, None, None, None, 1e-6, precision="mixed")
+    assert chunk == 2048 and timings is None
