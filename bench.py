@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--sampler-steps", type=int, default=3)
     ap.add_argument("--tree-steps", type=int, default=3)
     ap.add_argument("--polytopes", type=int, default=65536)
-    ap.add_argument("--requests", type=int, default=100,
+    ap.add_argument("--requests", type=int, default=1000,
                     help="requests of the serving stream (its first requests, in order)")
     ap.add_argument("--rate", type=float, default=200.0, help="serving arrivals per second")
     ap.add_argument("--split-nodes", type=int, default=2048,
@@ -574,7 +574,7 @@ def bench_serving(args):
         res = []
         for i, f in enumerate(futs):
             res.append(f.result())
-            if (i + 1) % 20 == 0:
+            if (i + 1) % 100 == 0:
                 print(f"[bench] serving: {i + 1} results at {time.perf_counter() - t0:.1f} s",
                       file=sys.stderr, flush=True)
         makespan = time.perf_counter() - t0
@@ -594,7 +594,7 @@ def bench_serving(args):
             t = time.perf_counter()
             eng.submit(request(i, "s")).result()
             service.append(time.perf_counter() - t)
-            if (i + 1) % 20 == 0:
+            if (i + 1) % 100 == 0:
                 print(f"[bench] one at a time: {i + 1} done, {sum(service):.1f} s",
                       file=sys.stderr, flush=True)
     finish, lat1 = 0.0, []
@@ -610,6 +610,8 @@ def bench_serving(args):
     return {
         "metric": "served requests/s and p50/p99 request latency",
         "value": len(events) / makespan, "unit": "requests/s",
+        "arrival_rate": args.rate,
+        "keeps_up_with_arrivals": bool(len(events) / makespan >= args.rate),
         "latency_ms": {"p50": float(np.percentile(lat, 50)) * 1e3,
                        "p99": float(np.percentile(lat, 99)) * 1e3,
                        "max": float(lat.max()) * 1e3},
