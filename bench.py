@@ -338,6 +338,8 @@ def bench_ridge_kernels(args):
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t0) / max(args.steps, 1) * 1e3
         n_chunks = batch.n_chunks
+    if args.copy_probe:
+        copy_probe(torch)
     print(f"[bench] {args.steps} {args.precision} fits, {ms:.3f} ms each (host clock)",
           file=sys.stderr, flush=True)
     return {"leg": "ridge_kernels", "precision": args.precision, "steps": args.steps,
@@ -347,8 +349,8 @@ def bench_ridge_kernels(args):
 
 def graph_replays(batch, X, y, w, ridge, precision, steps, reference):
     """The fit captured as one HIP graph: `steps` timed replays (events on the
-    batch stream after one untimed replay) and a bitwise check of every
-    replay's coefficients against the eager fit's."""
+    batch stream after one untimed replay), then a bitwise check of the
+    coefficients the replays left against the eager fit's."""
     import torch
 
     graph = batch.capture(X, y, w, ridge, precision=precision)
@@ -920,6 +922,11 @@ def emit(args, full):
                        f"{full['leaves']} leaves): FP64 normal-equation fits, the oracle's soft "
                        f"oblique optimizer, a scalar NumPy hit-and-run chain; "
                        f"{full['seconds']:.1f} s on one thread, run during the GPU legs")}
+    if isinstance(line, dict):
+        import multiprocessing
+
+        # Every forked worker and the background build have been joined.
+        line["child_processes_at_end"] = len(multiprocessing.active_children())
     print(json.dumps(line))
 
 
