@@ -105,6 +105,7 @@ _PROTOTYPES = {
     "art_philox4x32_10": (_i, [_p, _p, _i64, _p, _p]),
     "art_tree_predict": (_i, [_p, _i64, _p, _p, _p, _i, _p, Basis, _p, _p]),
     "art_leaf_gram": (_i, [_p, _p, _p, _p, ctypes.c_double, _p, _p, _p]),
+    "art_leaf_gram_ex": (_i, [_p, _p, _p, _p, ctypes.c_double, _i, _p, _p, _p, _p]),
     "art_gram_size": (_i, [_i]),
     "art_gram_solve": (_i, [_p, _p, _i, Basis, ctypes.c_double, _p, _p, _p, _p]),
     "art_knn_median": (_i, [_p, _p, _p, _i, _i, _i, _p, _p]),

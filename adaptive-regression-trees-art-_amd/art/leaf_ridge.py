@@ -153,20 +153,42 @@ class LeafRidgeBatch:
         side.synchronize()
         return LeafFitGraph(self, graph, side, (X, y, w))
 
-    def grams(self, X, y, w, weight_floor=1e-12):
-        """Each leaf's FP64 augmented system, without ridge: a device tensor
+    def grams(self, X, y, w, weight_floor=1e-12, *, precision="fp64", mask=None, out=None):
+        """Each leaf's augmented system, without ridge: a device tensor
         (K, gram_size) in the kernels' tile layout, and its invalid-weight
-        flags (K,).  Systems of disjoint row sets add (rank-k refits)."""
+        flags (K,).  Systems of disjoint row sets add (rank-k refits).
+
+        precision "fp64" (default) or "mixed" (the split-bf16 MFMA system of
+        the mixed fit, p <= 64, d <= 16: entry (i, j) within 4e-7 of
+        sqrt(G_ii G_jj) of the FP64 system, ~1e-8 on large leaves).
+        mask, a (K,) device int32 tensor, selects the leaves to form; the
+        others keep what ``out`` = (grams, bad) held (fresh tensors when out
+        is None, so pass out with a mask)."""
         torch = self.torch
         if self.plan is None:
             raise RuntimeError("The plan is closed.")
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'.")
         self._check_inputs(X, y, w)
         size = self.lib.art_gram_size(self.p)
-        grams = torch.empty((self.K, size), dtype=torch.float64, device=self.device)
-        bad = torch.empty(self.K, dtype=torch.int32, device=self.device)
-        _hip.check(self.lib.art_leaf_gram(self.plan, _hip.ptr(X), _hip.ptr(y), _hip.ptr(w),
-                                          float(weight_floor), _hip.ptr(grams), _hip.ptr(bad),
-                                          _hip.stream_handle(self.stream)))
+        if out is None:
+            grams = torch.empty((self.K, size), dtype=torch.float64, device=self.device)
+            bad = torch.empty(self.K, dtype=torch.int32, device=self.device)
+        else:
+            grams, bad = out
+            if grams.shape != (self.K, size) or grams.dtype != torch.float64 \
+                    or not grams.is_contiguous() or bad.shape != (self.K,) \
+                    or bad.dtype != torch.int32 or not bad.is_contiguous():
+                raise ValueError("out must be contiguous (K, gram_size) float64 and (K,) int32 "
+                                 "device tensors.")
+        if mask is not None and (mask.shape != (self.K,) or mask.dtype != torch.int32
+                                 or not mask.is_contiguous() or mask.device != grams.device):
+            raise ValueError("mask must be a contiguous (K,) int32 tensor on the batch's device.")
+        code = _hip.PRECISION_MIXED if precision == "mixed" else _hip.PRECISION_FP64
+        _hip.check(self.lib.art_leaf_gram_ex(
+            self.plan, _hip.ptr(X), _hip.ptr(y), _hip.ptr(w), float(weight_floor), code,
+            None if mask is None else _hip.ptr(mask), _hip.ptr(grams), _hip.ptr(bad),
+            _hip.stream_handle(self.stream)))
         return grams, bad
 
     def rcond(self):

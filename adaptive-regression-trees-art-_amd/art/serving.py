@@ -29,6 +29,18 @@ mix (bit for bit).
 
 Leaf systems are exact FP64: they stay resident for refits, and a reduced
 precision system would carry its error into every later rank-k update.
+
+``ServingConfig(precision="mixed")`` (bases with p <= 64 and d <= 16) forms
+the statistics of the nodes that may still split -- whose fit is read only
+for the stop test -- as split-bf16 MFMA systems (``art_leaf_gram_ex``: entry (i, j) within 4e-7
+of sqrt(G_ii G_jj) of the FP64 system), and FP64 systems only for the nodes
+that end as leaves: those the depth and size rules stop are formed in FP64
+directly, the others once the mixed statistics (or a failed split search, or
+a partition that leaves one side empty) make them leaves, from the rows still
+in their level buffer.  Every kept system and leaf model is the FP64 one, so
+refits are exact and a leaf's result is that of the FP64 engine whenever the
+stop tests agree; the FP64 Gram pass over a request's rows drops from one per
+level to one in all.
 """
 
 from __future__ import annotations
@@ -143,6 +155,7 @@ class ServingConfig:
     max_iters: int = 30
     min_side_fraction: float = 0.05
     quadratic_max_d: int = 12
+    precision: str = "fp64"        # "fp64", or "mixed": split-bf16 statistics of open nodes
 
 
 def _splitmix64(x):
@@ -155,6 +168,26 @@ def _splitmix64(x):
 def start_key(seed, heap, restart):
     """Random-start key of node ``heap`` (root 1, children 2h and 2h+1)."""
     return _splitmix64((int(seed) & _MASK64) ^ _splitmix64((int(heap) << 8) | int(restart)))
+
+
+# Mixed-precision stop test: a node goes to the split search on its mixed
+# statistics only when its mixed MSE exceeds the error threshold by
+# _MIXED_SSE_MARGIN * S^2 / sum(w), S = sum_i |b_i| sqrt(G_ii) + sqrt(y'Wy);
+# the others are decided on their FP64 fit.  The split-bf16 system's entries
+# are within 4e-7 of sqrt(G_ii G_jj), so the SSE y'Wy - 2 b'h + b'G b,
+# i.e. [b; -1]' G [b; -1], errs by at most 4e-7 S^2 at a given b; the margin
+# leaves a factor of 25 for the fitted coefficients' own error.  Ill-
+# conditioned fits (large cancelling b) widen S and go to FP64 more often.
+_MIXED_SSE_MARGIN = 1e-5
+
+
+def _diag_offset(col, p):
+    """Offset of diagonal entry (col, col) in a resident system of the
+    (p+1)-column augmented matrix (tile pairs (t1 <= t2), t1 outermost)."""
+    T = (p + 1 + 15) // 16
+    t, a = divmod(int(col), 16)
+    q = sum(T - u for u in range(t))
+    return 256 * q + 16 * a + a
 
 
 def _heap_path(heap):
@@ -248,6 +281,7 @@ class _Node:
     bad: int = 0
     leaf_candidate: Optional[dict] = None   # the node's own fit while its split is open
     gram_candidate: Any = None
+    index: int = -1                    # row of the node in its level's tables
 
 
 @dataclass
@@ -349,6 +383,9 @@ class _Group:
         self.new_jobs = []
         self.offsets = None      # row offsets of entries in the level buffer
         self.stream = None       # the group's split-search stream
+        self.plan = None         # mixed precision: the level's leaf plan, kept for late leaves
+        self.grams = None        # mixed precision: the level's FP64 systems (leaves only)
+        self.bad = None
 
 
 class ServingEngine:
@@ -356,6 +393,8 @@ class ServingEngine:
 
     def __init__(self, config: Optional[ServingConfig] = None, *, start=True):
         self.config = ServingConfig() if config is None else config
+        if self.config.precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'.")
         torch = _hip.torch_device()
         self.torch, self.lib = torch, _hip.load()
         self.device = torch.device("cuda", torch.cuda.current_device())
@@ -462,6 +501,12 @@ class ServingEngine:
         except BaseException as exc:  # fail every open request loudly
             self._error = exc
             self._fail_all(exc)
+        finally:
+            for g in self._groups.values():
+                try:
+                    self._close_plan(g)
+                except Exception:  # the failure is already reported to every request
+                    pass
 
     def _fail_all(self, exc):
         with self._cv:
@@ -654,6 +699,7 @@ class ServingEngine:
         root_rows = sum(job.n_rows for job in g.new_jobs)
         total = child_rows + root_rows
         if total == 0:
+            self._close_plan(g)
             g.entries, g.sizes, g.pending = [], [], None
             g.X = g.y = g.w = None
             return
@@ -668,6 +714,12 @@ class ServingEngine:
             base = np.concatenate([[0], np.cumsum(np.subtract(end, begin))[:-1]])
             right = partition_rows(g.X, g.y, g.w, begin, end, np.stack([p[4] for p in pend]),
                                    [p[5] for p in pend], (X, y, w), base, stream=self.stream)
+            if g.plan is not None:
+                # A partition that leaves one side empty makes its node a leaf
+                # (below); its FP64 system comes from the previous level buffer.
+                self._late_leaves(g, [node for (_, node, b, e, _, _), nr in zip(pend, right)
+                                      if int(nr) == 0 or int(nr) == int(e - b)])
+            self._close_plan(g)
             for (job, node, b, e, _, _), nr in zip(pend, right):
                 n = int(e - b)
                 nl = n - int(nr)
@@ -690,6 +742,7 @@ class ServingEngine:
                     node.leaf_candidate = node.gram_candidate = None
                 job.open -= 1
                 self._maybe_finish(job)
+        self._close_plan(g)
         off = child_rows
         for job in g.new_jobs:
             n = job.n_rows
@@ -707,6 +760,9 @@ class ServingEngine:
         g.X, g.y, g.w = X, y, w
         g.entries, g.sizes, g.pending = entries, sizes, None
 
+    def _mixed(self, d, p):
+        return self.config.precision == "mixed" and p <= 64 and d <= 16
+
     def _level(self, g):
         from .leaf_ridge import LeafRidgeBatch, solve_grams
 
@@ -715,6 +771,8 @@ class ServingEngine:
         t = time.perf_counter()
         offsets = np.concatenate([[0], np.cumsum(g.sizes)]).astype(np.int64)
         g.offsets = offsets
+        if self._mixed(d, p):
+            return self._level_mixed(g, degree, p, t)
         with LeafRidgeBatch(offsets, d, degree=degree, stream=self.stream) as lb:
             grams, bad = lb.grams(g.X, g.y, g.w, cfg.weight_floor)
             coef, status, _ = solve_grams(grams, bad, d, cfg.ridge, degree=degree,
@@ -741,9 +799,144 @@ class ServingEngine:
             else:
                 node.leaf_candidate, node.gram_candidate = leaf, (grams, i)
                 node.bad = int(bad_h[i])
+                node.index = i
                 split_idx.append(i)
         self._clock("decide", t)
         return split_idx
+
+    def _final_by_size(self, job, node, d):
+        req = job.req
+        return (node.depth >= req.max_depth or node.n_rows < 2 * req.min_leaf
+                or node.n_rows < 2 * (d + 2))
+
+    def _level_mixed(self, g, degree, p, t):
+        """One level at mixed precision.  Nodes the depth and size rules leave
+        open get split-bf16 statistics; a node whose mixed MSE clears its
+        error threshold by the margin goes to the split search on them.
+        Every other live node -- stopped by depth or size, or with a mixed MSE
+        within the margin of its threshold -- gets its FP64 system and fit,
+        and the stop test of the FP64 engine decides it.  So the decisions are
+        the FP64 engine's whenever the mixed MSE errs by less than the margin.
+        The level's plan and FP64 system tensor stay with the group until the
+        next level, for nodes that become leaves after their split search."""
+        from .leaf_ridge import LeafRidgeBatch, solve_grams
+
+        torch, cfg, d = self.torch, self.config, g.d
+        K = len(g.entries)
+        live = np.array([job is not None for job, _ in g.entries], dtype=bool)
+        final = np.array([job is not None and self._final_by_size(job, node, d)
+                          for job, node in g.entries], dtype=bool)
+        cand = live & ~final
+        thr = np.array([job.req.error_threshold if job is not None else 0.0
+                        for job, _ in g.entries])
+        g.plan = LeafRidgeBatch(g.offsets, d, degree=degree, stream=self.stream)
+        size = self.lib.art_gram_size(p)
+        f64 = dict(dtype=torch.float64, device=self.device)
+        g.grams = torch.zeros((K, size), **f64)
+        g.bad = torch.zeros(K, dtype=torch.int32, device=self.device)
+        clear = np.zeros(K, dtype=bool)
+        # FP64 systems of the nodes the depth and size rules stop go first, on
+        # the same stream, so that both passes complete under one synchronization.
+        early = self._fp64_launch(g, degree, p, final) if final.any() else None
+        if cand.any():
+            mg = torch.zeros((K, size), **f64)
+            mb = torch.zeros(K, dtype=torch.int32, device=self.device)
+            mask = torch.from_numpy(cand.astype(np.int32)).to(self.device)
+            g.plan.grams(g.X, g.y, g.w, cfg.weight_floor, precision="mixed", mask=mask,
+                         out=(mg, mb))
+            coef, status, _ = solve_grams(mg, mb, d, cfg.ridge, degree=degree, stream=self.stream)
+            sse, wsum = gram_sse(mg, coef, p, bias_col=0, stream=self.stream)
+            diag = torch.tensor([_diag_offset(c, p) for c in range(p + 1)], device=self.device)
+            root = mg.index_select(1, diag).clamp_min(0.0).sqrt()
+            scale = (coef.abs() * root[:, :p]).sum(dim=1) + root[:, p]
+            scale = scale * scale
+            self.stream.synchronize()
+            coef_h, status_h = coef.cpu().numpy(), status.cpu().numpy()
+            sse_h, wsum_h, s2_h = sse.cpu().numpy(), wsum.cpu().numpy(), scale.cpu().numpy()
+            with np.errstate(divide="ignore", invalid="ignore"):
+                mse = np.where(wsum_h > 0, sse_h / wsum_h, np.inf)
+                margin = np.where(wsum_h > 0, _MIXED_SSE_MARGIN * s2_h / wsum_h, np.inf)
+            clear = cand & (status_h == 0) & np.isfinite(mse) & (mse > thr + margin)
+        fits = self._fp64_collect(g, final, *early) if early is not None else {}
+        self._clock("node_fits", t)
+        near = cand & ~clear
+        if near.any():
+            fits.update(self._fp64_fits(g, degree, p, near))
+        t = time.perf_counter()
+        split_idx = []
+        for i in np.flatnonzero(live):
+            job, node = g.entries[i]
+            node.index = int(i)
+            if clear[i]:
+                node.leaf_candidate = {"coef": coef_h[i], "status": int(status_h[i]),
+                                       "mse": float(mse[i]), "n_rows": node.n_rows}
+                node.gram_candidate = None   # formed in FP64 only if the node ends as a leaf
+                split_idx.append(int(i))
+                continue
+            leaf, gram, bad = fits[i]
+            if final[i] or leaf["mse"] <= job.req.error_threshold:
+                node.leaf, node.gram, node.bad = leaf, gram, bad
+                job.open -= 1
+                self._maybe_finish(job)
+            else:
+                node.leaf_candidate, node.gram_candidate, node.bad = leaf, gram, bad
+                split_idx.append(int(i))
+        self._clock("decide", t)
+        return split_idx
+
+    def _fp64_launch(self, g, degree, p, sel):
+        """Enqueue the FP64 systems and fits of the level's nodes sel (bool
+        (K,)), from the rows in the level buffer, into the group's FP64
+        tensors; returns the device results (coefficients, status, sse, wsum)."""
+        from .leaf_ridge import solve_grams
+
+        torch, cfg, d = self.torch, self.config, g.d
+        mask = torch.from_numpy(sel.astype(np.int32)).to(self.device)
+        g.plan.grams(g.X, g.y, g.w, cfg.weight_floor, mask=mask, out=(g.grams, g.bad))
+        coef, status, _ = solve_grams(g.grams, g.bad, d, cfg.ridge, degree=degree,
+                                      stream=self.stream)
+        sse, wsum = gram_sse(g.grams, coef, p, bias_col=0, stream=self.stream)
+        return coef, status, sse, wsum
+
+    def _fp64_fits(self, g, degree, p, sel):
+        """_fp64_launch, then its host records (see _fp64_collect)."""
+        t = time.perf_counter()
+        out = self._fp64_collect(g, sel, *self._fp64_launch(g, degree, p, sel))
+        self._clock("node_fits", t)
+        return out
+
+    def _fp64_collect(self, g, sel, coef, status, sse, wsum):
+        """{i: (leaf record, kept system reference, weight flag)} of the nodes
+        sel of an FP64 launch (synchronizes the engine stream)."""
+        self.stream.synchronize()
+        coef_h, status_h = coef.cpu().numpy(), status.cpu().numpy()
+        sse_h, wsum_h, bad_h = sse.cpu().numpy(), wsum.cpu().numpy(), g.bad.cpu().numpy()
+        out = {}
+        for i in np.flatnonzero(sel):
+            node = g.entries[i][1]
+            mse = sse_h[i] / wsum_h[i] if wsum_h[i] > 0 else float("inf")
+            out[int(i)] = ({"coef": coef_h[i], "status": int(status_h[i]), "mse": float(mse),
+                            "n_rows": node.n_rows}, (g.grams, int(i)), int(bad_h[i]))
+        return out
+
+    def _late_leaves(self, g, nodes):
+        """Mixed precision: nodes split on mixed statistics that end as leaves
+        (failed search, one-sided partition) take their FP64 system and fit
+        from the rows still in their level buffer."""
+        todo = [n for n in nodes if n.gram_candidate is None]
+        if not todo:
+            return
+        sel = np.zeros(len(g.entries), dtype=bool)
+        sel[[n.index for n in todo]] = True
+        degree, p = self._basis(g.d, g.degree)
+        fits = self._fp64_fits(g, degree, p, sel)
+        for n in todo:
+            n.leaf_candidate, n.gram_candidate, n.bad = fits[n.index]
+
+    def _close_plan(self, g):
+        if g.plan is not None:
+            g.plan.close()
+            g.plan = None
 
     def _search(self, g, idx):
         """Split search of nodes idx of group g on the group's own stream:
@@ -781,6 +974,8 @@ class ServingEngine:
         from .batched_optimizer import STOP_REASONS
 
         pending = []
+        if g.plan is not None:
+            self._late_leaves(g, [g.entries[i][1] for k, i in enumerate(idx) if not ok[k]])
         for k, i in enumerate(idx):
             job, node = g.entries[i]
             if not ok[k]:

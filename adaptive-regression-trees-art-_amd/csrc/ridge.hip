@@ -739,8 +739,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TT <= 3 ? 4 
 // partial tiles (chunks goff[j] .. goff[j+1]); bad[j] ORs their weight flags.
 __global__ __launch_bounds__(256) void reduce_leaf_partials_kernel(
     const double* __restrict__ partials, const int* __restrict__ flags,
-    const int* __restrict__ goff, int total, double* __restrict__ grams, int* __restrict__ bad) {
+    const int* __restrict__ goff, int total, const int* __restrict__ mask,
+    double* __restrict__ grams, int* __restrict__ bad) {
   const int j = blockIdx.x;
+  if (mask != nullptr && !mask[j]) return;  // a skipped leaf keeps its system and flag
   const int c0 = goff[j], c1 = goff[j + 1];
   for (int e = threadIdx.x; e < total; e += blockDim.x) {
     double v = 0.0;
@@ -791,10 +793,11 @@ __global__ __launch_bounds__(256) void reduce_leaf_partials_kernel(
 template <int T, int DC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DC > 0 ? 3 : 2))) void gram_leaf_mixed_kernel(
     const double* __restrict__ X, const double* __restrict__ y, const double* __restrict__ w,
-    Basis B, double floor, const LeafChunk* __restrict__ chunks, double* __restrict__ partials,
-    double* __restrict__ tpart, int* __restrict__ flags) {
+    Basis B, double floor, const LeafChunk* __restrict__ chunks, const int* __restrict__ mask,
+    double* __restrict__ partials, double* __restrict__ tpart, int* __restrict__ flags) {
   constexpr int NP = T * (T + 1) / 2;
   constexpr int MD = DC > 0 ? DC : kMixedMaxD;
+  if (mask != nullptr && !mask[chunks[blockIdx.x].leaf]) return;  // uniform: before any barrier
   extern __shared__ double smem[];
   __shared__ double tred[4];
   const int d = DC > 0 ? DC : B.d;
@@ -1683,6 +1686,30 @@ static void launch_gram_leaf_f64(const art_ridge_plan* P, const double* X, const
   }
 }
 
+// The split-bf16 Gram of every chunk (or of the chunks of leaves with
+// mask[leaf] != 0) into the FP64 kernel's partial layout and the exact
+// per-chunk traces (p <= 64, T <= 5, d <= kMixedMaxD; P->d_tpart allocated).
+static void launch_gram_leaf_mixed(const art_ridge_plan* P, const double* X, const double* y,
+                                   const double* w, double floor, const int* mask,
+                                   hipStream_t stream) {
+  const size_t msmem = mixed_stage_smem_bytes(P->basis.d);
+  const dim3 grid(P->nchunks), block(256);
+  if (P->T == 3 && P->basis.d == 8) {
+    gram_leaf_mixed_kernel<3, 8><<<grid, block, msmem, stream>>>(
+        X, y, w, P->basis, floor, P->d_chunks, mask, P->d_partials, P->d_tpart, P->d_flags);
+    return;
+  }
+  switch (P->T) {
+#define ART_CASE(TT)                                                                    \
+  case TT:                                                                              \
+    gram_leaf_mixed_kernel<TT, 0><<<grid, block, msmem, stream>>>(                      \
+        X, y, w, P->basis, floor, P->d_chunks, mask, P->d_partials, P->d_tpart, P->d_flags); \
+    break;
+    ART_CASE(1) ART_CASE(2) ART_CASE(3) ART_CASE(4) ART_CASE(5)
+#undef ART_CASE
+  }
+}
+
 // Mixed (fp64_gram false): split-bf16 Gram, refinement, FP64 path for the
 // leaves still flagged.  FP64-refined (fp64_gram true): FP64 Gram and
 // Cholesky, then the same data-space refinement steps with the tighter
@@ -1708,24 +1735,10 @@ static int leaf_fit_mixed(art_ridge_plan* P, const double* X, const double* y, c
   const size_t smem = leaf_stage_smem_bytes(P->basis.d);
   const dim3 cgrid(P->nchunks), sgrid(P->nsys), block(256);
   timing_mark(P, 0, stream);
-  const size_t msmem = mixed_stage_smem_bytes(P->basis.d);
   if (fp64_gram) {
     launch_gram_leaf_f64(P, X, y, w, floor, nullptr, stream);
-  } else if (T == 3 && P->basis.d == 8) {
-    gram_leaf_mixed_kernel<3, 8><<<cgrid, block, msmem, stream>>>(X, y, w, P->basis, floor,
-                                                                 P->d_chunks, P->d_partials,
-                                                                 P->d_tpart, P->d_flags);
   } else {
-    switch (T) {
-#define ART_CASE(TT)                                                                          \
-  case TT:                                                                                    \
-    gram_leaf_mixed_kernel<TT, 0><<<cgrid, block, msmem, stream>>>(X, y, w, P->basis, floor,  \
-                                                                  P->d_chunks, P->d_partials, \
-                                                                  P->d_tpart, P->d_flags);    \
-    break;
-      ART_CASE(1) ART_CASE(2) ART_CASE(3) ART_CASE(4) ART_CASE(5)
-#undef ART_CASE
-    }
+    launch_gram_leaf_mixed(P, X, y, w, floor, nullptr, stream);
   }
   ART_HIP(hipGetLastError());
   timing_mark(P, 1, stream);
@@ -1901,7 +1914,46 @@ int art_leaf_gram(art_ridge_plan* P, const double* X, const double* y, const dou
   }
   ART_HIP(hipGetLastError());
   reduce_leaf_partials_kernel<<<dim3(P->K), dim3(256), 0, stream>>>(
-      P->d_partials, P->d_flags, P->d_goff, P->npairs * 256, grams, bad);
+      P->d_partials, P->d_flags, P->d_goff, P->npairs * 256, nullptr, grams, bad);
+  ART_HIP(hipGetLastError());
+  return ART_OK;
+}
+
+int art_leaf_gram_ex(art_ridge_plan* P, const double* X, const double* y, const double* w,
+                     double weight_floor, int precision, const int* mask, double* grams,
+                     int* bad, hipStream_t stream) {
+  if (precision == ART_PRECISION_FP64 && mask == nullptr)
+    return art_leaf_gram(P, X, y, w, weight_floor, grams, bad, stream);
+  if (!P || P->kind != 0) return fail(ART_ERR_ARG, "art_leaf_gram_ex: not a leaf plan");
+  if (!X || !y || !w || !grams || !bad)
+    return fail(ART_ERR_ARG, "art_leaf_gram_ex: null pointer");
+  if (!(weight_floor >= 0.0) || std::isinf(weight_floor))
+    return fail(ART_ERR_ARG, "art_leaf_gram_ex: weight_floor must be finite and >= 0");
+  if (precision == ART_PRECISION_MIXED) {
+    if (P->p > 64 || P->T > 5 || P->basis.d > kMixedMaxD)
+      return fail(ART_ERR_ARG, "art_leaf_gram_ex: mixed precision needs p <= 64 and d <= 16");
+    if (P->d_tpart == nullptr)
+      ART_HIP(hipMalloc(&P->d_tpart, (size_t)P->nchunks * sizeof(double)));
+    launch_gram_leaf_mixed(P, X, y, w, weight_floor, mask, stream);
+  } else if (precision == ART_PRECISION_FP64) {
+    if (P->p > kRowsMaxP) return fail(ART_ERR_ARG, "art_leaf_gram_ex: needs p <= 120");
+    if (P->T <= 6 && P->basis.d <= kMaxStagedD) {
+      launch_gram_leaf_f64(P, X, y, w, weight_floor, mask, stream);
+    } else {
+      // The generic kernel has no mask: every chunk is formed and the reduce
+      // below keeps only the selected leaves.
+      PolySource src{X, y, w, P->basis, weight_floor};
+      TableChunks map{P->d_chunks};
+      const int groups = (P->npairs + kGramMaxQ - 1) / kGramMaxQ;
+      gram_f64_kernel<<<dim3(P->nchunks, groups, 1), dim3(256), 0, stream>>>(
+          src, map, P->nchunks, P->T, P->npairs, P->d_partials, P->d_flags);
+    }
+  } else {
+    return fail(ART_ERR_ARG, "art_leaf_gram_ex: precision must be ART_PRECISION_FP64 or _MIXED");
+  }
+  ART_HIP(hipGetLastError());
+  reduce_leaf_partials_kernel<<<dim3(P->K), dim3(256), 0, stream>>>(
+      P->d_partials, P->d_flags, P->d_goff, P->npairs * 256, mask, grams, bad);
   ART_HIP(hipGetLastError());
   return ART_OK;
 }

@@ -775,8 +775,10 @@ def bench_serving(args):
     earlier tree; quadratic leaves for d <= 12, linear above; depth <= 10,
     >= 64 rows to split a node's half; 200 GB HBM budget).  Rows are generated
     on the device before timing.  Leg 1 serves the stream as it arrives
-    (continuous batching); leg 2 serves the same requests one at a time, and
-    its latencies are those of a FIFO single server fed the same arrivals."""
+    (continuous batching) with mixed-precision statistics of open nodes, and
+    again in FP64 for comparison (the served trees must agree); leg 2 serves
+    the same requests one at a time, and its latencies are those of a FIFO
+    single server fed the same arrivals."""
     import torch
 
     from art.serving import FitRequest, RefitRequest, ServingConfig, ServingEngine
@@ -804,32 +806,48 @@ def bench_serving(args):
             eng.fit(FitRequest(X, X[:, 0].contiguous(), max_depth=2, tree_id=f"warm{d}"))
             eng.evict(f"warm{d}")
 
-    cfg = ServingConfig(hbm_budget_bytes=200e9)
-    with ServingEngine(cfg) as eng:
-        warm(eng)
-        t0 = time.perf_counter()
-        futs = []
-        for i, e in enumerate(events):
-            delay = t0 + e.arrival - time.perf_counter()
-            if delay > 0:
-                time.sleep(delay)
-            futs.append(eng.submit(request(i, "b")))
-        res = []
-        for i, f in enumerate(futs):
-            res.append(f.result())
-            if (i + 1) % 100 == 0:
-                print(f"[bench] serving: {i + 1} results at {time.perf_counter() - t0:.1f} s",
-                      file=sys.stderr, flush=True)
-        makespan = time.perf_counter() - t0
-        steps = eng.steps
-        phases = {k: round(v, 4) for k, v in eng.phase_seconds.items()}
-    lat = np.array([r.timing.latency for r in res])
+    def serve(precision, tag):
+        """The stream as it arrives (continuous batching) at one precision."""
+        cfg = ServingConfig(hbm_budget_bytes=200e9, precision=precision)
+        with ServingEngine(cfg) as eng:
+            warm(eng)
+            t0 = time.perf_counter()
+            futs = []
+            for i, e in enumerate(events):
+                delay = t0 + e.arrival - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                futs.append(eng.submit(request(i, tag)))
+            res = []
+            for i, f in enumerate(futs):
+                res.append(f.result())
+                if (i + 1) % 100 == 0:
+                    print(f"[bench] serving ({precision}): {i + 1} results at "
+                          f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+            makespan = time.perf_counter() - t0
+            out = {"res": res, "makespan": makespan, "steps": eng.steps,
+                   "phases": {k: round(v, 4) for k, v in eng.phase_seconds.items()}}
+        lat = np.array([r.timing.latency for r in res])
+        out["lat"] = lat
+        print(f"[bench] serving ({precision}) {len(events)} requests in {makespan:.2f} s, p50 "
+              f"{np.percentile(lat, 50) * 1e3:.0f} ms p99 {np.percentile(lat, 99) * 1e3:.0f} ms",
+              file=sys.stderr, flush=True)
+        return out
+
+    mixed = serve("mixed", "b")
+    fp64 = serve("fp64", "f")
+    res, makespan, lat = mixed["res"], mixed["makespan"], mixed["lat"]
+    steps, phases = mixed["steps"], mixed["phases"]
+    same = all(np.array_equal(a.coefficients, b.coefficients) and
+               np.array_equal(a.status, b.status) for a, b in zip(res, fp64["res"]))
+    differ = [i for i, (a, b) in enumerate(zip(res, fp64["res"]))
+              if a.coefficients.shape != b.coefficients.shape]
+    coef_diff = max((float(np.max(np.abs(a.coefficients - b.coefficients), initial=0.0))
+                     for a, b in zip(res, fp64["res"])
+                     if a.coefficients.shape == b.coefficients.shape), default=0.0)
     leaves = [len(r.coefficients) for r, e in zip(res, events) if e.kind == "fit"]
     bad = sum(int((r.status != 0).sum()) for r in res)
-    print(f"[bench] serving {len(events)} requests in {makespan:.2f} s, p50 "
-          f"{np.percentile(lat, 50) * 1e3:.0f} ms p99 {np.percentile(lat, 99) * 1e3:.0f} ms",
-          file=sys.stderr, flush=True)
-    one = ServingConfig(hbm_budget_bytes=200e9, max_active=1)
+    one = ServingConfig(hbm_budget_bytes=200e9, max_active=1, precision="mixed")
     service = []
     with ServingEngine(one) as eng:
         warm(eng)
@@ -863,7 +881,15 @@ def bench_serving(args):
         "refits": sum(e.kind == "refit" for e in events), "rows": rows,
         "mean_leaves_per_tree": float(np.mean(leaves)) if leaves else 0.0,
         "failed_leaf_solves": bad,
-        "dtype": "f64",
+        "dtype": ("bf16 3-way split (6 MFMA products)/f32-acc statistics of open nodes "
+                  "(p <= 64, d <= 16); f64 leaf systems, leaf fits, split search, refits"),
+        "fp64": {"value": len(events) / fp64["makespan"], "unit": "requests/s",
+                 "latency_ms": {"p50": float(np.percentile(fp64["lat"], 50)) * 1e3,
+                                "p99": float(np.percentile(fp64["lat"], 99)) * 1e3},
+                 "engine_phase_s": fp64["phases"]},
+        "mixed_equals_fp64": {"same_leaf_models_and_status": bool(same),
+                              "requests_with_other_leaf_counts": len(differ),
+                              "max_abs_coefficient_difference": coef_diff},
         "one_at_a_time": {"value": len(events) / sum(service), "unit": "requests/s",
                           "service_ms": {"p50": float(np.percentile(service, 50)) * 1e3,
                                          "p99": float(np.percentile(service, 99)) * 1e3},

@@ -282,6 +282,19 @@ int art_split_init_ranges(const double* X, const int64_t* row_offsets, int d, co
 int art_leaf_gram(art_ridge_plan* plan, const double* X, const double* y, const double* w,
                   double weight_floor, double* grams, int* bad, hipStream_t stream);
 int art_gram_size(int p);
+/* art_leaf_gram with a precision and an optional leaf selection.
+ * precision ART_PRECISION_FP64 is art_leaf_gram's system; ART_PRECISION_MIXED
+ * (p <= 64, d <= 16) writes the split-bf16 MFMA system of the mixed fit in
+ * the same layout: entry (i, j) within 4e-7 of sqrt(G_ii G_jj) of the FP64
+ * system (the FP32 rounding of each scaled feature value), ~1e-8 on large
+ * leaves.
+ * mask (K,) int32 device array or NULL (every leaf): a leaf with mask[j] == 0
+ * is skipped and keeps what grams and bad held.  No reference counterpart:
+ * the serving loop forms split nodes' statistics at mixed precision and the
+ * systems it keeps for refits in FP64. */
+int art_leaf_gram_ex(art_ridge_plan* plan, const double* X, const double* y, const double* w,
+                     double weight_floor, int precision, const int* mask, double* grams,
+                     int* bad, hipStream_t stream);
 int art_gram_solve(const double* grams, const int* bad, int K, art_basis basis, double ridge,
                    double* beta, int* status, double* lambda, hipStream_t stream);
 

@@ -476,6 +476,72 @@ def test_resident_grams_solve_like_the_fp64_fit(torch_gpu, d, degree):
     np.testing.assert_allclose(lam.cpu().numpy(), l64.cpu().numpy(), rtol=1e-13)
 
 
+def _unpack_tiles(row, T):
+    """Full (16T, 16T) symmetric matrix of one resident system in tile layout
+    (tile pair (t1 <= t2, t1 outermost), element (a, b) = G[16 t1 + a][16 t2 + b])."""
+    G = np.zeros((16 * T, 16 * T))
+    q = 0
+    for t1 in range(T):
+        for t2 in range(t1, T):
+            blk = row[256 * q:256 * (q + 1)].reshape(16, 16)
+            G[16 * t1:16 * t1 + 16, 16 * t2:16 * t2 + 16] = blk
+            G[16 * t2:16 * t2 + 16, 16 * t1:16 * t1 + 16] = blk.T
+            q += 1
+    return G
+
+
+@pytest.mark.parametrize("d,degree", [(4, 2), (8, 2), (16, 1)])
+def test_masked_and_mixed_resident_grams(torch_gpu, d, degree):
+    """art_leaf_gram_ex: a mask forms only the selected leaves' systems (the
+    others keep what the output held) and bitwise what the unmasked call
+    forms; mixed-precision systems are within 4e-7 of the FP64 ones on the
+    Cauchy-Schwarz scale sqrt(G_ii G_jj), padding exactly zero.  The bound:
+    each operand v = sqrt(w) x_a x_b is formed in FP32 (up to five roundings,
+    <= 2.5 * 2^-24 relative), so a product v_i v_j is off by <= 3e-7 of
+    |v_i v_j|, and the sum over rows by <= 3e-7 of sum |v_i v_j| <= the scale;
+    a one-row leaf reaches it, large leaves average down to ~1e-8."""
+    from art import LeafRidgeBatch
+
+    torch, rng = torch_gpu, np.random.default_rng(90 + d)
+    sizes = np.array([5000, 1, 64, 0, 333, 9000, 17, 2048])
+    offsets = np.concatenate([[0], np.cumsum(sizes)])
+    N = offsets[-1]
+    X = rng.uniform(-1, 1, size=(N, d))
+    y, w = np.cos(3 * X[:, 0]) * X[:, -1], rng.uniform(0.05, 1.0, size=N)
+    Xd, yd, wd = dev(torch, X), dev(torch, y), dev(torch, w)
+    sel = np.array([1, 0, 1, 1, 0, 1, 0, 1], dtype=np.int32)
+    mask = torch.as_tensor(sel, device="cuda")
+    with LeafRidgeBatch(offsets, d, degree=degree) as batch:
+        T = (batch.p + 1 + 15) // 16
+        full64 = [t.clone() for t in batch.grams(Xd, yd, wd)]
+        fullmx = [t.clone() for t in batch.grams(Xd, yd, wd, precision="mixed")]
+        outs = {}
+        for prec in ("fp64", "mixed"):
+            g = torch.full_like(full64[0], 7.0)
+            b = torch.full_like(full64[1], 5)
+            batch.grams(Xd, yd, wd, precision=prec, mask=mask, out=(g, b))
+            outs[prec] = (g, b)
+        torch.cuda.synchronize()
+        with pytest.raises(ValueError):
+            batch.grams(Xd, yd, wd, precision="bf16")
+    for prec, full in (("fp64", full64), ("mixed", fullmx)):
+        g, b = (t.cpu().numpy() for t in outs[prec])
+        fg, fb = (t.cpu().numpy() for t in full)
+        np.testing.assert_array_equal(g[sel == 1], fg[sel == 1])
+        np.testing.assert_array_equal(b[sel == 1], fb[sel == 1])
+        assert (g[sel == 0] == 7.0).all() and (b[sel == 0] == 5).all()
+    g64, gmx = full64[0].cpu().numpy(), fullmx[0].cpu().numpy()
+    worst = 0.0
+    for j in range(len(sizes)):
+        A, M = _unpack_tiles(g64[j], T), _unpack_tiles(gmx[j], T)
+        scale = np.sqrt(np.outer(np.diag(A), np.diag(A)))
+        pad = scale == 0
+        assert (M[pad] == 0).all() and (A[pad] == 0).all()
+        if (~pad).any():
+            worst = max(worst, float(np.max(np.abs(M - A)[~pad] / scale[~pad])))
+    assert worst < 4e-7, worst
+
+
 def test_resident_grams_of_disjoint_rows_add(torch_gpu):
     """Rank-k refits: the systems of two disjoint row sets of each leaf sum to
     the system of their union, and its solve matches the oracle on all rows."""

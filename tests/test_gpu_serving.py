@@ -160,6 +160,77 @@ def test_results_are_the_same_alone_and_in_the_mix(torch_gpu):
     np.testing.assert_array_equal(alone_refit.coefficients, mixed_refit.coefficients)
 
 
+def test_mixed_precision_serving_equals_fp64_alone_and_in_the_mix(torch_gpu):
+    """precision="mixed": split-bf16 statistics for open nodes, FP64 systems
+    for leaves.  On bases it covers (d = 2, 4, 8 quadratic; 16 linear) and one
+    it leaves in FP64 (d = 12 quadratic), with error thresholds that stop
+    nodes by their MSE, every served tree equals the FP64 engine's (same
+    splits, leaf models and kept systems, so equal refits), alone and in a
+    mix; the small fit also matches the oracle."""
+    from art.serving import RefitRequest, ServingConfig, ServingEngine
+
+    rng = np.random.default_rng(12)
+    reqs = [_request(rng, n, d, seed=s, max_depth=dep, min_leaf=64, error_threshold=thr,
+                     tree_id=f"t{s}")
+            for n, d, s, dep, thr in ((20000, 4, 1, 6, 2e-4), (6000, 2, 2, 5, 1e-4),
+                                      (30000, 8, 3, 5, 5e-4), (9000, 16, 4, 4, 1e-3),
+                                      (7000, 12, 5, 3, 0.0))]
+    Xn = rng.uniform(-1, 1, (900, 4))
+    yn = np.sin(Xn.sum(axis=1))
+    base = dict(split_rows=2048)
+    results = {}
+    for prec in ("fp64", "mixed"):
+        with ServingEngine(ServingConfig(precision=prec, **base)) as eng:
+            futs = [eng.submit(r) for r in reqs]
+            fits = [f.result() for f in futs]
+            refit = eng.submit(RefitRequest("t1", Xn, yn)).result()
+        results[prec] = (fits, refit)
+    with ServingEngine(ServingConfig(precision="mixed", **base)) as eng:
+        alone = eng.fit(reqs[0])
+        alone_refit = eng.submit(RefitRequest("t1", Xn, yn)).result()
+    stopped_by_mse = 0
+    for req, a, b in zip(reqs, results["fp64"][0], results["mixed"][0]):
+        ta, tb = _tree_arrays(a), _tree_arrays(b)
+        assert [s[0] for s in ta[0]] == [s[0] for s in tb[0]]
+        for (_, w1, z1), (_, w2, z2) in zip(ta[0], tb[0]):
+            np.testing.assert_array_equal(w1, w2)
+            assert z1 == z2
+        np.testing.assert_array_equal(ta[1], tb[1])
+        np.testing.assert_array_equal(a.status, b.status)
+        for leaf in b.tree.iter_nodes():
+            if hasattr(leaf, "w"):
+                continue
+            meta = leaf.metadata
+            if (leaf.depth < req.max_depth and meta["n_rows"] >= 128
+                    and meta["mse"] <= req.error_threshold):
+                stopped_by_mse += 1
+    assert stopped_by_mse > 0
+    np.testing.assert_array_equal(results["fp64"][1].coefficients,
+                                  results["mixed"][1].coefficients)
+    ta, tb = _tree_arrays(alone), _tree_arrays(results["mixed"][0][0])
+    assert [s[0] for s in ta[0]] == [s[0] for s in tb[0]] and len(ta[0]) > 3
+    np.testing.assert_array_equal(ta[1], tb[1])
+    np.testing.assert_array_equal(alone_refit.coefficients, results["mixed"][1].coefficients)
+
+    small = _request(rng, 3000, 2, seed=11, max_depth=3, min_leaf=100)
+    cfg = ServingConfig(precision="mixed", split_rows=1024, max_iters=30)
+    with ServingEngine(cfg) as eng:
+        res = eng.fit(small)
+    s = S.FitSettings(max_depth=3, min_leaf=100, degree=2, seed=11, split_rows=1024,
+                      max_iters=30, ridge=cfg.ridge, split_ridge=cfg.split_ridge,
+                      min_side_fraction=cfg.min_side_fraction)
+    nodes = S.fit_tree(small.X, small.y, None, s)
+    got = {node.node_id: node for node in res.tree.iter_nodes()}
+    assert len(got) == len(nodes)
+    for h, ref in nodes.items():
+        node = got["/".join(["root"] + ["L" if b == "0" else "R" for b in bin(h)[3:]])]
+        if "split" in ref:
+            np.testing.assert_allclose(np.r_[node.w, node.z], np.r_[ref["split"]], rtol=1e-9,
+                                       atol=1e-10)
+        else:
+            np.testing.assert_allclose(node.model.coef_, ref["coef"], rtol=1e-8, atol=1e-10)
+
+
 def test_refit_equals_the_fit_on_all_rows(torch_gpu):
     from art.serving import FitRequest, RefitRequest, ServingConfig, ServingEngine
 
