@@ -339,17 +339,105 @@ __device__ __attribute__((always_inline)) inline void chain_run(const art_har_ar
           e[3 * DM + 1 + j] = nz ? (up ? bhi[j] : blo[j]) : INFINITY;
         }
       e[2 * DM] = u;
+      // Per row, 1 / (A_i.d) on the side of the chord it bounds and NaN on
+      // the other (a row with A_i.d = 0 bounds neither): the replay's
+      // slack * reciprocal is then the row's chord end or NaN, and fmax /
+      // fmin (IEEE maxNum / minNum) drop the NaNs.
       double* ea = e + 4 * DM + 1;
       for (int i = 0; i < m; ++i) {
         double ad = 0.0;
 #pragma unroll
         for (int j = 0; j < DM; ++j)
           if (EXACT || j < d) ad = fma(A[i * d + j], dir[j], ad);
-        ea[i] = ad;
-        ea[m + i] = rcp2(ad);
+        const double r = rcp2(ad);
+        ea[i] = ad < 0.0 ? r : NAN;
+        ea[m + i] = ad > 0.0 ? r : NAN;
       }
     }
     __syncthreads();
+    if constexpr (M >= 0) {
+      // Rows in registers: the replay of move k works from registers only.
+      // Move k + 1's entry is read from LDS while move k runs (its reads do
+      // not depend on the chain), and the chord ends are fmax / fmin trees
+      // over the candidates (exact, so the values of a running max / min,
+      // without a compare and select per candidate).
+      constexpr int NC = DM + MR;
+      struct Entry {
+        double dir[DM], r[DM], flo[DM], fhi[DM], u, rlo[MR], rhi[MR];
+      };
+      auto read = [&](int k, Entry& q) {
+        const double* e = chain_lds + (size_t)k * E;
+#pragma unroll
+        for (int j = 0; j < DM; ++j) {
+          const bool on = EXACT || j < d;
+          q.dir[j] = on ? e[j] : 0.0;
+          q.r[j] = on ? e[DM + j] : 1.0;
+          q.flo[j] = on ? e[2 * DM + 1 + j] : -INFINITY;
+          q.fhi[j] = on ? e[3 * DM + 1 + j] : INFINITY;
+        }
+        q.u = e[2 * DM];
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+          q.rlo[i] = e[4 * DM + 1 + i];
+          q.rhi[i] = e[4 * DM + 1 + M + i];
+        }
+      };
+      Entry cur;
+      read(0, cur);
+      for (int k = 0; k < cnt; ++k) {
+        Entry nxt;
+        read(k + 1 < cnt ? k + 1 : k, nxt);
+        double lo[NC], hi[NC];
+#pragma unroll
+        for (int j = 0; j < DM; ++j) {
+          if (box_lo && (EXACT || j < d)) {
+            lo[j] = (cur.flo[j] - x[j]) * cur.r[j];
+            hi[j] = (cur.fhi[j] - x[j]) * cur.r[j];
+          } else {
+            lo[j] = -INFINITY;
+            hi[j] = INFINITY;
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < MR; ++i) {
+          if (i < M) {
+            double ax = 0.0;
+#pragma unroll
+            for (int j = 0; j < DM; ++j)
+              if (EXACT || j < d) ax = fma(Ar[i][j], x[j], ax);
+            const double slack = br[i] - ax;
+            lo[DM + i] = slack * cur.rlo[i];
+            hi[DM + i] = slack * cur.rhi[i];
+          } else {
+            lo[DM + i] = -INFINITY;
+            hi[DM + i] = INFINITY;
+          }
+        }
+#pragma unroll
+        for (int w = 1; w < NC; w *= 2)
+#pragma unroll
+          for (int i = 0; i + w < NC; i += 2 * w) {
+            lo[i] = fmax(lo[i], lo[i + w]);
+            hi[i] = fmin(hi[i], hi[i + w]);
+          }
+        const double t = chord_point(lo[0], hi[0], cur.u);
+#pragma unroll
+        for (int j = 0; j < DM; ++j) x[j] += cur.dir[j] * t;
+        if (--until_record == 0) {  // moves first, first + stride, ...
+          if (lane == 0) {
+            double* o = out + (size_t)recorded * d;
+#pragma unroll
+            for (int j = 0; j < DM; ++j)
+              if (EXACT || j < d) o[j] = x[j];
+          }
+          ++recorded;
+          until_record = a.stride;
+        }
+        cur = nxt;
+      }
+      __syncthreads();
+      continue;
+    }
     for (int k = 0; k < cnt; ++k) {
       const double* e = chain_lds + (size_t)k * E;
       double tlo = -INFINITY, thi = INFINITY;
@@ -367,29 +455,16 @@ __device__ __attribute__((always_inline)) inline void chain_run(const art_har_ar
           }
       }
       const double* ea = e + 4 * DM + 1;
-      if constexpr (M >= 0) {
-#pragma unroll
-        for (int i = 0; i < M; ++i) {
-          double ax = 0.0;
-#pragma unroll
-          for (int j = 0; j < DM; ++j)
-            if (EXACT || j < d) ax = fma(Ar[i][j], x[j], ax);
-          const double ad = ea[i];
-          const double ratio = (br[i] - ax) * ea[M + i];
-          tlo = (ad < 0.0 && ratio > tlo) ? ratio : tlo;
-          thi = (ad > 0.0 && ratio < thi) ? ratio : thi;
-        }
-      } else {
+      {  // any row count (M < 0): rows read per move
 #pragma unroll 2
         for (int i = 0; i < m; ++i) {
           double ax = 0.0;
 #pragma unroll
           for (int j = 0; j < DM; ++j)
             if (EXACT || j < d) ax = fma(A[i * d + j], x[j], ax);
-          const double ad = ea[i];
-          const double ratio = (b[i] - ax) * ea[m + i];
-          tlo = (ad < 0.0 && ratio > tlo) ? ratio : tlo;
-          thi = (ad > 0.0 && ratio < thi) ? ratio : thi;
+          const double slack = b[i] - ax;
+          tlo = fmax(tlo, slack * ea[i]);
+          thi = fmin(thi, slack * ea[m + i]);
         }
       }
       const double t = chord_point(tlo, thi, e[2 * DM]);
