@@ -53,15 +53,31 @@ __device__ inline double gate(double score) {
   return 1.0 / (1.0 + exp(-fmin(50.0, fmax(-50.0, score))));
 }
 
-// Rows staged in LDS per pass step and their stride (odd, in doubles).  With
-// one feature tile (at most 16 non-constant child features, so d <= 16) a
-// chunk is 256 rows of stride 17: one row per thread in the per-row phases;
-// two tiles (d <= 31) keep 128 rows of stride 33 within the LDS budget.
+// Workgroup and staged-chunk geometry.  Rows are staged in LDS with an odd
+// stride of at least d + 1 doubles (x_0..x_{d-1}, y; odd, so a column read
+// by consecutive rows spreads over the banks), as many rows per chunk as the
+// table holds, up to one row per thread in the per-row phases.  One feature
+// tile (at most 16 non-constant child features, d <= 16): 16 waves and up
+// to 1024 rows (1024 at d <= 8, 512 at d = 16), so a node's pass is a few
+// chunks with four waves per SIMD to hide each other's latency; two tiles
+// (d <= 31): 4 waves and 128 rows.
 template <int T> struct ChunkShape;
-template <> struct ChunkShape<1> { static constexpr int rows = 256, stride = 17, maxd = 16; };
-template <> struct ChunkShape<2> { static constexpr int rows = 128, stride = 33, maxd = 31; };
-template <int T> constexpr int kChunkT = ChunkShape<T>::rows;
-template <int T> constexpr int kXStrideT = ChunkShape<T>::stride;
+template <> struct ChunkShape<1> {
+  static constexpr int waves = 8, maxrows = 512, table = 512 * 9, maxd = 16;
+};
+template <> struct ChunkShape<2> {
+  static constexpr int waves = 4, maxrows = 128, table = 128 * 33, maxd = 31;
+};
+template <int T> constexpr int kWavesT = ChunkShape<T>::waves;
+template <int T> constexpr int kThreadsT = 64 * ChunkShape<T>::waves;
+template <int T> constexpr int kMaxRowsT = ChunkShape<T>::maxrows;
+
+__host__ __device__ inline int row_stride(int d) { return (d + 1) | 1; }
+template <int T>
+__host__ __device__ inline int chunk_rows(int d) {
+  const int r = ChunkShape<T>::table / row_stride(d) / 64 * 64;
+  return r < ChunkShape<T>::maxrows ? r : ChunkShape<T>::maxrows;
+}
 
 template <int T>
 struct SplitShared {
@@ -73,15 +89,15 @@ struct SplitShared {
   int status[2];
   double geometry[4];
   double packet[7];
-  double red[4 * 256];        // MFMA tile reduction; reused by block reductions
+  double red[kWavesT<T> * 256];  // MFMA tile reduction; reused by block reductions
   double M[2][(kSplitMaxP + 1) * (kSplitMaxP + 2) / 2];
   double piv[2];
   double bc[2];
-  int ired[4];
-  double dred[4 * 64];
+  int ired[kWavesT<T>];
+  double dred[kWavesT<T> * 64];
   double tot[64];
-  double xs[kChunkT<T> * kXStrideT<T>];  // staged rows: x_0..x_{d-1}, y
-  double qs[2][kChunkT<T>];  // per-row side weights (fit) or gradient terms (stats)
+  double xs[ChunkShape<T>::table];  // staged rows: x_0..x_{d-1}, y, stride row_stride(d)
+  double qs[2][kMaxRowsT<T>];  // per-row side weights (fit) or gradient terms (stats)
   signed char fa[kSplitMaxP];    // feature f = x[fa] * x[fb]; -1 stands for 1
   signed char fb[kSplitMaxP];
   signed char core[kSplitMaxP];  // basis column of non-constant feature k
@@ -93,6 +109,7 @@ struct SplitCtx {
   const double* X;
   const double* y;
   int n, d, p;
+  int rows, sd;  // chunk rows and staged row stride
   Basis B;
   double T;
   double ridge;
@@ -118,14 +135,14 @@ __device__ inline void init_basis_tables(const Basis& B, SplitShared<T>& S) {
 }
 
 // Row staging: a chunk's x values are one contiguous block of rows * d
-// doubles (rows are contiguous in X), loaded element-wise by the 256 threads
-// (coalesced; ceil(rows d / 256) loads per thread, one for d = 2), and its
+// doubles (rows are contiguous in X), loaded element-wise by the workgroup
+// (coalesced; ceil(rows d / threads) loads per thread, two for d = 2), and its
 // y values one per thread.  A thread's loads for the next chunk are issued
 // before the current chunk is processed (register double buffering: the
 // barriers wait on LDS only, not on these loads), so the memory round trip
 // hides behind a chunk of compute.  S.xs holds x_0..x_{d-1}, y per row with
-// stride kXStride.
-template <int T> constexpr int kStageT = (kChunkT<T> * ChunkShape<T>::maxd + 255) / 256;
+// stride C.sd.
+template <int T> constexpr int kStageT = (ChunkShape<T>::table + kThreadsT<T> - 1) / kThreadsT<T>;
 
 template <int T>
 struct Staged {
@@ -139,7 +156,7 @@ __device__ inline void load_rows(const SplitCtx& C, int base, int rows, Staged<T
   const double* src = C.X + (size_t)base * C.d;
 #pragma unroll
   for (int i = 0; i < kStageT<T>; ++i) {
-    const int e = tid + 256 * i;
+    const int e = tid + kThreadsT<T> * i;
     v.x[i] = e < nx ? src[e] : 0.0;
   }
   v.y = tid < rows ? C.y[base + tid] : 0.0;
@@ -151,20 +168,20 @@ __device__ inline void store_rows(const SplitCtx& C, SplitShared<T>& S, int rows
   const int tid = threadIdx.x, d = C.d, nx = rows * d;
 #pragma unroll
   for (int i = 0; i < kStageT<T>; ++i) {
-    const int e = tid + 256 * i;
+    const int e = tid + kThreadsT<T> * i;
     if (e < nx) {
       const int r = e / d;
-      S.xs[r * kXStrideT<T> + (e - r * d)] = v.x[i];
+      S.xs[r * C.sd + (e - r * d)] = v.x[i];
     }
   }
-  if (tid < rows) S.xs[tid * kXStrideT<T> + d] = v.y;
+  if (tid < rows) S.xs[tid * C.sd + d] = v.y;
 }
 
 // Iterates the node's rows in staged chunks: body(rows) runs with chunk rows
 // [base, base + rows) in S.xs while the next chunk's loads are in flight.
 template <int T, class Body>
 __device__ inline void for_each_chunk(const SplitCtx& C, SplitShared<T>& S, Body body) {
-  constexpr int kChunk = kChunkT<T>;
+  const int kChunk = C.rows;
   Staged<T> v;
   load_rows(C, 0, min(kChunk, C.n), v);
   for (int base = 0; base < C.n; base += kChunk) {
@@ -198,9 +215,12 @@ __device__ inline void block_sums(const double (&v)[K], SplitShared<T>& S, int c
       if (lane == 0) S.dred[wave * 64 + k] = s;
     }
   __syncthreads();
-  if ((int)threadIdx.x < count)
-    S.tot[threadIdx.x] = ((S.dred[threadIdx.x] + S.dred[64 + threadIdx.x]) +
-                          S.dred[128 + threadIdx.x]) + S.dred[192 + threadIdx.x];
+  if ((int)threadIdx.x < count) {
+    double t = S.dred[threadIdx.x];
+#pragma unroll
+    for (int w = 1; w < kWavesT<T>; ++w) t += S.dred[64 * w + threadIdx.x];
+    S.tot[threadIdx.x] = t;
+  }
   __syncthreads();
 }
 
@@ -221,7 +241,7 @@ __device__ void fit_sides(const SplitCtx& C, SplitShared<T>& S, bool hard, doubl
   constexpr int NP = T * (T + 1) / 2;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int col = lane & 15, kq = lane >> 4;
-  const int d = C.d, p = C.p, n = C.n, sd = kXStrideT<T>, nf = S.nf;
+  const int d = C.d, p = C.p, n = C.n, sd = C.sd, nf = S.nf;
   int fa[T], fb[T];
   bool live[T];
 #pragma unroll
@@ -245,7 +265,7 @@ __device__ void fit_sides(const SplitCtx& C, SplitShared<T>& S, bool hard, doubl
   int bad = 0;
   const double z = S.theta[d];
   for_each_chunk(C, S, [&](int rows) {
-    if (tid < kChunkT<T>) {
+    if (tid < C.rows) {
       double q0 = 0.0, q1 = 0.0;
       if (tid < rows) {
         const double* xr = S.xs + tid * sd;
@@ -276,7 +296,7 @@ __device__ void fit_sides(const SplitCtx& C, SplitShared<T>& S, bool hard, doubl
       S.qs[1][tid] = q1;
     }
     __syncthreads();
-    for (int st = wave; st < kChunkT<T> / 4; st += 4) {
+    for (int st = wave; st < C.rows / 4; st += kWavesT<T>) {
       const int r = 4 * st + kq;
       const bool valid = r < rows;
       const double* xr = S.xs + r * sd;
@@ -329,9 +349,14 @@ __device__ void fit_sides(const SplitCtx& C, SplitShared<T>& S, bool hard, doubl
         for (int r = 0; r < 4; ++r)
           S.red[wave * 256 + (kq + 4 * r) * 16 + col] = side == 0 ? acc0[q][r] : acc1[q][r];
         __syncthreads();
-        const double v = ((S.red[tid] + S.red[256 + tid]) + S.red[512 + tid]) + S.red[768 + tid];
+        double v = 0.0;
+        if (tid < 256) {
+          v = S.red[tid];
+#pragma unroll
+          for (int w = 1; w < kWavesT<T>; ++w) v += S.red[256 * w + tid];
+        }
         const int ka = 16 * t1 + (tid >> 4), kb = 16 * t2 + (tid & 15);
-        if (ka < nf && kb < nf && (t1 != t2 || ka <= kb)) {
+        if (tid < 256 && ka < nf && kb < nf && (t1 != t2 || ka <= kb)) {
           const int ca = S.core[ka], cb = S.core[kb];
           S.M[side][ca >= cb ? tri(ca, cb) : tri(cb, ca)] = v;
         }
@@ -352,7 +377,9 @@ __device__ void fit_sides(const SplitCtx& C, SplitShared<T>& S, bool hard, doubl
     const int k = tid >> 4, c = tid & 15;
     const int t = k >> 2, kind = k & 3;
     const int feat = 16 * t + c;
-    const double s = ((S.red[tid] + S.red[256 + tid]) + S.red[512 + tid]) + S.red[768 + tid];
+    double s = S.red[tid];
+#pragma unroll
+    for (int w = 1; w < kWavesT<T>; ++w) s += S.red[256 * w + tid];
     if (feat < nf) {
       const int cf = S.core[feat];
       const int side = kind >> 1;
@@ -430,14 +457,14 @@ __device__ inline void side_predictions(const SplitShared<T>& S, const double* x
 template <int T>
 __device__ void stats_and_direction(const SplitCtx& C, SplitShared<T>& S) {
   const int tid = threadIdx.x;
-  const int d = C.d, p = C.p, n = C.n, sd = kXStrideT<T>;
+  const int d = C.d, p = C.p, n = C.n, sd = C.sd;
   const double z = S.theta[d];
   const double inv = 1.0 / (n * C.T);
   const int gj = tid & 31, gg = tid >> 5, groups = (int)(blockDim.x >> 5);
   double sums[3] = {0.0, 0.0, 0.0};  // loss, -sum c, count(pi >= 1/2)
   double g = 0.0;                   // partial of (X^T c)[gj]
   for_each_chunk(C, S, [&](int rows) {
-    if (tid < kChunkT<T>) {
+    if (tid < C.rows) {
       double c = 0.0;
       if (tid < rows) {
         const double* xr = S.xs + tid * sd;
@@ -515,7 +542,7 @@ template <int T>
 __device__ void trial_losses(const SplitCtx& C, SplitShared<T>& S, const double* alphas, int count,
                              double* out) {
   const int tid = threadIdx.x;
-  const int d = C.d, p = C.p, n = C.n, sd = kXStrideT<T>;
+  const int d = C.d, p = C.p, n = C.n, sd = C.sd;
   const double z = S.theta[d], temp = C.T;
   const double g0 = S.geometry[0], g1 = S.geometry[1], g2 = S.geometry[2], g3 = S.geometry[3];
   double f[kSplitMaxH];
@@ -566,7 +593,7 @@ template <int T>
 __device__ void hard_stats(const SplitCtx& C, SplitShared<T>& S, bool refit, double* out,
                            bool all_right = false) {
   const int tid = threadIdx.x;
-  const int d = C.d, p = C.p, n = C.n, sd = kXStrideT<T>;
+  const int d = C.d, p = C.p, n = C.n, sd = C.sd;
   const double z = S.theta[d];
   double sums[3] = {0.0, 0.0, 0.0};  // SSE left, SSE right, count right
   int bad = 0;
@@ -615,7 +642,7 @@ __device__ inline double clip_alpha(const art_split_args* a, double v) {
 }
 
 template <int T>
-__global__ __launch_bounds__(256) void split_optimize_kernel(art_split_args args) {
+__global__ __launch_bounds__(kThreadsT<T>) void split_optimize_kernel(art_split_args args) {
   __shared__ SplitShared<T> S;
   __shared__ double alphas[kSplitMaxH];
   __shared__ double losses[kSplitMaxH];
@@ -636,6 +663,8 @@ __global__ __launch_bounds__(256) void split_optimize_kernel(art_split_args args
   C.p = a->basis.p;
   C.T = a->temperature[node];
   C.ridge = a->ridge;
+  C.sd = row_stride(d);
+  C.rows = chunk_rows<T>(d);
   const int p = C.p;
   const int MI = a->max_iters;
   double* hist = a->history + (size_t)prob * 5 * (MI + 1);
@@ -903,9 +932,9 @@ extern "C" int art_split_optimize(const art_split_args* args, int problems, hipS
     return fail(ART_ERR_ARG, "art_split_optimize: null device pointer");
   // Gram tiles cover the non-constant features only.
   const int nf = p - (args->basis.bias ? 1 : 0);
-  const dim3 grid(problems), block(256);
-  if (nf <= 16) split_optimize_kernel<1><<<grid, block, 0, stream>>>(*args);
-  else split_optimize_kernel<2><<<grid, block, 0, stream>>>(*args);
+  const dim3 grid(problems);
+  if (nf <= 16) split_optimize_kernel<1><<<grid, dim3(kThreadsT<1>), 0, stream>>>(*args);
+  else split_optimize_kernel<2><<<grid, dim3(kThreadsT<2>), 0, stream>>>(*args);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? ART_OK : hip_fail(e, "split_optimize_kernel launch");
 }
