@@ -414,6 +414,7 @@ class ServingEngine:
         self.steps = 0
         self.completed = 0
         self.phase_seconds = collections.defaultdict(float)
+        self._retired_plans = []       # leaf plans to close after the level's searches
         self._thread = None
         self.stream = None
         self._pool = ThreadPoolExecutor(max_workers=8, thread_name_prefix="art-search")
@@ -507,6 +508,10 @@ class ServingEngine:
                     self._close_plan(g)
                 except Exception:  # the failure is already reported to every request
                     pass
+            try:
+                self._close_retired_plans()
+            except Exception:
+                pass
 
     def _fail_all(self, exc):
         with self._cv:
@@ -659,9 +664,12 @@ class ServingEngine:
     def _step(self):
         """One level of every basis group; refits first (they finish in one level).
 
-        Node fits run group by group on the engine stream; the split searches
-        of all groups then run at once, each group on its own stream from its
-        own host thread (the kernels of different bases overlap on the device).
+        Node fits run group by group on the engine stream.  A group's split
+        search starts, on the group's own stream from a pool thread, as soon
+        as its node fits are decided, so it runs on the device while the
+        engine thread partitions and fits the next groups.  Leaf plans freed
+        during the level (a free synchronizes the device) are closed after
+        the searches, so that no close waits on a search.
         """
         self.steps += 1
         if self._refits:
@@ -677,19 +685,22 @@ class ServingEngine:
             if g.entries:
                 idx = self._level(g)
                 if idx:
-                    searches.append((g, idx))
+                    searches.append((g, idx, self._pool.submit(self._search, g, idx)))
             if not g.entries and not g.new_jobs and g.pending is None:
                 del self._groups[key]
         if searches:
             t = time.perf_counter()
-            if len(searches) == 1:
-                found = [self._search(*searches[0])]
-            else:
-                found = list(self._pool.map(lambda a: self._search(*a), searches))
+            found = [(g, idx, fut.result()) for g, idx, fut in searches]
             t = self._clock("split", t)
-            for (g, idx), f in zip(searches, found):
+            for g, idx, f in found:
                 self._apply_splits(g, idx, *f)
             self._clock("decide", t)
+        self._close_retired_plans()
+
+    def _close_retired_plans(self):
+        plans, self._retired_plans = self._retired_plans, []
+        for plan in plans:
+            plan.close()
 
     def _build_level(self, g):
         """Next level buffer: children of last level's splits, then new roots."""
@@ -773,12 +784,13 @@ class ServingEngine:
         g.offsets = offsets
         if self._mixed(d, p):
             return self._level_mixed(g, degree, p, t)
-        with LeafRidgeBatch(offsets, d, degree=degree, stream=self.stream) as lb:
-            grams, bad = lb.grams(g.X, g.y, g.w, cfg.weight_floor)
-            coef, status, _ = solve_grams(grams, bad, d, cfg.ridge, degree=degree,
-                                          stream=self.stream)
-            sse, wsum = gram_sse(grams, coef, p, bias_col=0, stream=self.stream)
-            self.stream.synchronize()
+        lb = LeafRidgeBatch(offsets, d, degree=degree, stream=self.stream)
+        self._retired_plans.append(lb)  # closed after the level's searches
+        grams, bad = lb.grams(g.X, g.y, g.w, cfg.weight_floor)
+        coef, status, _ = solve_grams(grams, bad, d, cfg.ridge, degree=degree,
+                                      stream=self.stream)
+        sse, wsum = gram_sse(grams, coef, p, bias_col=0, stream=self.stream)
+        self.stream.synchronize()
         coef_h, status_h = coef.cpu().numpy(), status.cpu().numpy()
         sse_h, wsum_h, bad_h = sse.cpu().numpy(), wsum.cpu().numpy(), bad.cpu().numpy()
         t = self._clock("node_fits", t)
@@ -935,7 +947,7 @@ class ServingEngine:
 
     def _close_plan(self, g):
         if g.plan is not None:
-            g.plan.close()
+            self._retired_plans.append(g.plan)
             g.plan = None
 
     def _search(self, g, idx):
