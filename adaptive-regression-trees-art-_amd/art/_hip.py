@@ -169,3 +169,16 @@ def stream_handle(stream) -> int:
 
 def ptr(t) -> int:
     return int(t.data_ptr())
+
+
+def upload(a, device, dtype=None):
+    """Small host array -> device tensor, enqueued on the current stream with
+    no host wait: the values are staged in pinned host memory (torch's
+    caching host allocator keeps the staging block until the copy has run),
+    so a later change to ``a`` does not reach the copy.  A pageable copy
+    would first wait for every kernel already queued on the stream."""
+    import numpy as np
+    import torch
+
+    t = torch.from_numpy(np.ascontiguousarray(a, dtype=dtype))
+    return t.pin_memory().to(device, non_blocking=True)

@@ -222,9 +222,10 @@ def gather_strided(X, y, w, begin, end, counts, *, stream=None):
     Xo = torch.empty((total, d), dtype=torch.float64, device=dev)
     yo = torch.empty(total, dtype=torch.float64, device=dev)
     wo = None if w is None else torch.empty(total, dtype=torch.float64, device=dev)
-    b = torch.from_numpy(np.asarray(begin, dtype=np.int64)).to(dev)
-    e = torch.from_numpy(np.asarray(end, dtype=np.int64)).to(dev)
-    dd = torch.from_numpy(dst).to(dev)
+    table = _hip.upload(np.concatenate([np.asarray(begin, np.int64), np.asarray(end, np.int64),
+                                        dst]), dev)
+    K = len(counts)
+    b, e, dd = table[:K], table[K:2 * K], table[2 * K:]
     stream = torch.cuda.current_stream() if stream is None else stream
     _hip.check(lib.art_gather_strided(
         _hip.ptr(X), _hip.ptr(y), 0 if w is None else _hip.ptr(w), d, len(counts), _hip.ptr(b),
@@ -249,12 +250,12 @@ def partition_rows(X, y, w, begin, end, node_w, node_z, out, out_base, *, stream
     nblk = (end - begin + rows - 1) // rows
     blk = np.concatenate([[0], np.cumsum(nblk)]).astype(np.int64)
     blocks = int(blk[-1])
-    table = torch.from_numpy(np.concatenate([begin, end, blk, np.asarray(out_base, np.int64)]))
-    table = table.to(dev)
+    table = _hip.upload(np.concatenate([begin, end, blk, np.asarray(out_base, np.int64)]), dev)
     tb, te = table[:K], table[K:2 * K]
     tblk, tdst = table[2 * K:3 * K + 1], table[3 * K + 1:]
-    nw = torch.from_numpy(np.ascontiguousarray(node_w, dtype=np.float64)).to(dev)
-    nz = torch.from_numpy(np.ascontiguousarray(node_z, dtype=np.float64)).to(dev)
+    planes = _hip.upload(np.concatenate([np.asarray(node_w, np.float64).reshape(-1),
+                                         np.asarray(node_z, np.float64).reshape(-1)]), dev)
+    nw, nz = planes[:K * d], planes[K * d:]
     flags = torch.empty(int(X.shape[0]), dtype=torch.uint8, device=dev)
     blk_right = torch.empty(max(blocks, 1), dtype=torch.int32, device=dev)
     node_right = torch.empty(K, dtype=torch.int64, device=dev)
@@ -270,18 +271,32 @@ def partition_rows(X, y, w, begin, end, node_w, node_z, out, out_base, *, stream
 
 # --------------------------------------------------------------------------- engine state
 
-@dataclass
+class _Fits:
+    """Host results of one node-fit launch, row i for the launch's node i:
+    coefficients (K, p), solve status and weighted MSE.  A node's fit is the
+    pair (record, i), so a level costs three arrays, not a record per node."""
+
+    __slots__ = ("coef", "status", "mse", "bad")
+
+    def __init__(self, coef, status, sse, wsum, bad=None):
+        self.coef, self.status, self.bad = coef, status, bad
+        with np.errstate(divide="ignore", invalid="ignore"):
+            self.mse = np.where(wsum > 0, sse / wsum, np.inf)
+
+
+@dataclass(slots=True)
 class _Node:
     heap: int
     depth: int
     n_rows: int
     split: Optional[tuple] = None      # (w, z, metadata)
-    leaf: Optional[dict] = None        # coef, status, mse, n_rows
+    leaf: Optional[tuple] = None       # (_Fits, row) of a leaf's model
     gram: Any = None                   # (level systems tensor, row) of a leaf
     bad: int = 0
-    leaf_candidate: Optional[dict] = None   # the node's own fit while its split is open
+    leaf_candidate: Optional[tuple] = None   # the node's own fit while its split is open
     gram_candidate: Any = None
     index: int = -1                    # row of the node in its level's tables
+    err: Optional[str] = None          # why the split search failed, for a leaf
 
 
 @dataclass
@@ -320,7 +335,11 @@ class _Resident:
         self.tree_id, self.flat, self.grams, self.bad = tree_id, flat, grams, bad
         self.d, self.degree, self.p, self.ridge = d, degree, p, ridge
         self.splits = splits      # [(heap, depth, w, z, metadata)] in heap order
-        self.leaves = leaves      # [(heap, depth, metadata)] in heap order (= flat leaf order)
+        # Leaves in heap order (= flat leaf order): heaps and depths (lists),
+        # status, MSE and row counts (arrays), {leaf: split-search error}.
+        self.leaf_heap, self.leaf_depth, self.status, self.mse, self.leaf_rows, \
+            self.errors = leaves
+        self._meta = None         # leaf metadata dicts, made with the tree
         self.coef = None          # host (L, p), set by the owner
         self.n_rows = n_rows
         self.bytes = 0.0
@@ -340,8 +359,17 @@ class _Resident:
         from .tree import LeafNode, RegressionTree, SplitNode
 
         split_of = {h: (depth, w, z, meta) for h, depth, w, z, meta in self.splits}
-        leaf_of = {h: (j, depth, meta) for j, (h, depth, meta) in enumerate(self.leaves)}
-        self._models = [None] * len(self.leaves)
+        metas = []
+        for j in range(len(self.leaf_heap)):
+            meta = {"status": int(self.status[j]), "mse": float(self.mse[j]),
+                    "n_rows": int(self.leaf_rows[j])}
+            if j in self.errors:
+                meta["split_error"] = self.errors[j]
+            metas.append(meta)
+        self._meta = metas
+        leaf_of = {h: (j, depth, metas[j])
+                   for j, (h, depth) in enumerate(zip(self.leaf_heap, self.leaf_depth))}
+        self._models = [None] * len(self.leaf_heap)
 
         def build(h):
             if h in split_of:
@@ -352,8 +380,8 @@ class _Resident:
             m = PolynomialRidgeModel(self.degree, ridge=self.ridge)
             m.coef_ = self.coef[j].copy()
             self._models[j] = m
-            # The leaf shares its metadata dict with the record: refit
-            # statuses show through.
+            # The leaf shares its metadata dict with the resident record:
+            # refit statuses show through.
             return LeafNode(m, depth=depth, node_id=_heap_path(h), metadata=meta)
 
         return RegressionTree(build(1), metadata={"n_rows": self.n_rows, "d": self.d,
@@ -364,8 +392,10 @@ class _Resident:
         updated by the caller)."""
         with self._lock:
             self.coef = coef
-            for j, (_, _, meta) in enumerate(self.leaves):
-                meta["status"] = int(status[j])
+            self.status = np.asarray(status, dtype=np.int64).copy()
+            if self._meta is not None:
+                for j, meta in enumerate(self._meta):
+                    meta["status"] = int(self.status[j])
             if self._tree is not None:
                 for j, m in enumerate(self._models):
                     m.coef_ = coef[j].copy()
@@ -415,6 +445,7 @@ class ServingEngine:
         self.completed = 0
         self.phase_seconds = collections.defaultdict(float)
         self._retired_plans = []       # leaf plans to close after the level's searches
+        self._diag_cache = {}
         self._thread = None
         self.stream = None
         self._pool = ThreadPoolExecutor(max_workers=8, thread_name_prefix="art-search")
@@ -774,6 +805,14 @@ class ServingEngine:
     def _mixed(self, d, p):
         return self.config.precision == "mixed" and p <= 64 and d <= 16
 
+    def _diag_index(self, p):
+        """Device offsets of the diagonal entries of a packed (p+1)-system."""
+        t = self._diag_cache.get(p)
+        if t is None:
+            t = _hip.upload([_diag_offset(c, p) for c in range(p + 1)], self.device, np.int64)
+            self._diag_cache[p] = t
+        return t
+
     def _level(self, g):
         from .leaf_ridge import LeafRidgeBatch, solve_grams
 
@@ -791,25 +830,23 @@ class ServingEngine:
                                       stream=self.stream)
         sse, wsum = gram_sse(grams, coef, p, bias_col=0, stream=self.stream)
         self.stream.synchronize()
-        coef_h, status_h = coef.cpu().numpy(), status.cpu().numpy()
-        sse_h, wsum_h, bad_h = sse.cpu().numpy(), wsum.cpu().numpy(), bad.cpu().numpy()
+        rec = _Fits(coef.cpu().numpy(), status.cpu().numpy(), sse.cpu().numpy(),
+                    wsum.cpu().numpy(), bad.cpu().numpy())
+        mse_h, bad_h = rec.mse, rec.bad
         t = self._clock("node_fits", t)
         split_idx = []
         for i, (job, node) in enumerate(g.entries):
             if job is None:
                 continue
             req = job.req
-            mse = sse_h[i] / wsum_h[i] if wsum_h[i] > 0 else float("inf")
-            leaf = {"coef": coef_h[i], "status": int(status_h[i]), "mse": float(mse),
-                    "n_rows": node.n_rows}
             stop = (node.depth >= req.max_depth or node.n_rows < 2 * req.min_leaf
-                    or node.n_rows < 2 * (d + 2) or mse <= req.error_threshold)
+                    or node.n_rows < 2 * (d + 2) or mse_h[i] <= req.error_threshold)
             if stop:
-                node.leaf, node.gram, node.bad = leaf, (grams, i), int(bad_h[i])
+                node.leaf, node.gram, node.bad = (rec, i), (grams, i), int(bad_h[i])
                 job.open -= 1
                 self._maybe_finish(job)
             else:
-                node.leaf_candidate, node.gram_candidate = leaf, (grams, i)
+                node.leaf_candidate, node.gram_candidate = (rec, i), (grams, i)
                 node.bad = int(bad_h[i])
                 node.index = i
                 split_idx.append(i)
@@ -853,46 +890,47 @@ class ServingEngine:
         if cand.any():
             mg = torch.zeros((K, size), **f64)
             mb = torch.zeros(K, dtype=torch.int32, device=self.device)
-            mask = torch.from_numpy(cand.astype(np.int32)).to(self.device)
+            mask = _hip.upload(cand, self.device, np.int32)
             g.plan.grams(g.X, g.y, g.w, cfg.weight_floor, precision="mixed", mask=mask,
                          out=(mg, mb))
             coef, status, _ = solve_grams(mg, mb, d, cfg.ridge, degree=degree, stream=self.stream)
             sse, wsum = gram_sse(mg, coef, p, bias_col=0, stream=self.stream)
-            diag = torch.tensor([_diag_offset(c, p) for c in range(p + 1)], device=self.device)
+            diag = self._diag_index(p)
             root = mg.index_select(1, diag).clamp_min(0.0).sqrt()
             scale = (coef.abs() * root[:, :p]).sum(dim=1) + root[:, p]
             scale = scale * scale
             self.stream.synchronize()
-            coef_h, status_h = coef.cpu().numpy(), status.cpu().numpy()
-            sse_h, wsum_h, s2_h = sse.cpu().numpy(), wsum.cpu().numpy(), scale.cpu().numpy()
+            wsum_h = wsum.cpu().numpy()
+            mrec = _Fits(coef.cpu().numpy(), status.cpu().numpy(), sse.cpu().numpy(), wsum_h)
+            s2_h = scale.cpu().numpy()
             with np.errstate(divide="ignore", invalid="ignore"):
-                mse = np.where(wsum_h > 0, sse_h / wsum_h, np.inf)
                 margin = np.where(wsum_h > 0, _MIXED_SSE_MARGIN * s2_h / wsum_h, np.inf)
-            clear = cand & (status_h == 0) & np.isfinite(mse) & (mse > thr + margin)
-        fits = self._fp64_collect(g, final, *early) if early is not None else {}
+            mse = mrec.mse
+            clear = cand & (mrec.status == 0) & np.isfinite(mse) & (mse > thr + margin)
+        frec = self._fp64_collect(g, *early) if early is not None else None
         self._clock("node_fits", t)
         near = cand & ~clear
-        if near.any():
-            fits.update(self._fp64_fits(g, degree, p, near))
+        nrec = self._fp64_fits(g, degree, p, near) if near.any() else None
         t = time.perf_counter()
         split_idx = []
-        for i in np.flatnonzero(live):
+        grams = g.grams
+        for i in np.flatnonzero(live).tolist():
             job, node = g.entries[i]
-            node.index = int(i)
+            node.index = i
             if clear[i]:
-                node.leaf_candidate = {"coef": coef_h[i], "status": int(status_h[i]),
-                                       "mse": float(mse[i]), "n_rows": node.n_rows}
+                node.leaf_candidate = (mrec, i)
                 node.gram_candidate = None   # formed in FP64 only if the node ends as a leaf
-                split_idx.append(int(i))
+                split_idx.append(i)
                 continue
-            leaf, gram, bad = fits[i]
-            if final[i] or leaf["mse"] <= job.req.error_threshold:
-                node.leaf, node.gram, node.bad = leaf, gram, bad
+            rec = frec if final[i] else nrec
+            if final[i] or rec.mse[i] <= job.req.error_threshold:
+                node.leaf, node.gram, node.bad = (rec, i), (grams, i), int(rec.bad[i])
                 job.open -= 1
                 self._maybe_finish(job)
             else:
-                node.leaf_candidate, node.gram_candidate, node.bad = leaf, gram, bad
-                split_idx.append(int(i))
+                node.leaf_candidate, node.gram_candidate = (rec, i), (grams, i)
+                node.bad = int(rec.bad[i])
+                split_idx.append(i)
         self._clock("decide", t)
         return split_idx
 
@@ -903,7 +941,7 @@ class ServingEngine:
         from .leaf_ridge import solve_grams
 
         torch, cfg, d = self.torch, self.config, g.d
-        mask = torch.from_numpy(sel.astype(np.int32)).to(self.device)
+        mask = _hip.upload(sel, self.device, np.int32)
         g.plan.grams(g.X, g.y, g.w, cfg.weight_floor, mask=mask, out=(g.grams, g.bad))
         coef, status, _ = solve_grams(g.grams, g.bad, d, cfg.ridge, degree=degree,
                                       stream=self.stream)
@@ -911,25 +949,19 @@ class ServingEngine:
         return coef, status, sse, wsum
 
     def _fp64_fits(self, g, degree, p, sel):
-        """_fp64_launch, then its host records (see _fp64_collect)."""
+        """_fp64_launch, then its host record (see _fp64_collect)."""
         t = time.perf_counter()
-        out = self._fp64_collect(g, sel, *self._fp64_launch(g, degree, p, sel))
+        out = self._fp64_collect(g, *self._fp64_launch(g, degree, p, sel))
         self._clock("node_fits", t)
         return out
 
-    def _fp64_collect(self, g, sel, coef, status, sse, wsum):
-        """{i: (leaf record, kept system reference, weight flag)} of the nodes
-        sel of an FP64 launch (synchronizes the engine stream)."""
+    def _fp64_collect(self, g, coef, status, sse, wsum):
+        """The host record (_Fits, with the group's weight flags) of an FP64
+        launch; rows of the launch's nodes hold their fits (synchronizes the
+        engine stream)."""
         self.stream.synchronize()
-        coef_h, status_h = coef.cpu().numpy(), status.cpu().numpy()
-        sse_h, wsum_h, bad_h = sse.cpu().numpy(), wsum.cpu().numpy(), g.bad.cpu().numpy()
-        out = {}
-        for i in np.flatnonzero(sel):
-            node = g.entries[i][1]
-            mse = sse_h[i] / wsum_h[i] if wsum_h[i] > 0 else float("inf")
-            out[int(i)] = ({"coef": coef_h[i], "status": int(status_h[i]), "mse": float(mse),
-                            "n_rows": node.n_rows}, (g.grams, int(i)), int(bad_h[i]))
-        return out
+        return _Fits(coef.cpu().numpy(), status.cpu().numpy(), sse.cpu().numpy(),
+                     wsum.cpu().numpy(), g.bad.cpu().numpy())
 
     def _late_leaves(self, g, nodes):
         """Mixed precision: nodes split on mixed statistics that end as leaves
@@ -941,9 +973,10 @@ class ServingEngine:
         sel = np.zeros(len(g.entries), dtype=bool)
         sel[[n.index for n in todo]] = True
         degree, p = self._basis(g.d, g.degree)
-        fits = self._fp64_fits(g, degree, p, sel)
+        rec = self._fp64_fits(g, degree, p, sel)
         for n in todo:
-            n.leaf_candidate, n.gram_candidate, n.bad = fits[n.index]
+            i = n.index
+            n.leaf_candidate, n.gram_candidate, n.bad = (rec, i), (g.grams, i), int(rec.bad[i])
 
     def _close_plan(self, g):
         if g.plan is not None:
@@ -992,7 +1025,7 @@ class ServingEngine:
             job, node = g.entries[i]
             if not ok[k]:
                 node.leaf, node.gram = node.leaf_candidate, node.gram_candidate
-                node.leaf["split_error"] = STOP_REASONS[int(reason[k])]
+                node.err = STOP_REASONS[int(reason[k])]
                 node.leaf_candidate = node.gram_candidate = None
                 job.open -= 1
                 self._maybe_finish(job)
@@ -1019,14 +1052,13 @@ class ServingEngine:
 
         torch, d, degree = self.torch, job.d, job.degree
         _, p = self._basis(d, degree)
-        heaps = sorted(job.nodes)
-        split_h = [h for h in heaps if job.nodes[h].split is not None]
-        leaf_h = [h for h in heaps if job.nodes[h].split is None]
-        sidx = {h: i for i, h in enumerate(split_h)}
-        lidx = {h: j for j, h in enumerate(leaf_h)}
-
-        def link(h):
-            return sidx[h] if h in sidx else -1 - lidx[h]
+        nodes = job.nodes
+        split_h, leaf_h = [], []
+        for h in sorted(nodes):
+            (split_h if nodes[h].split is not None else leaf_h).append(h)
+        # Child link of every node: its split row, or -1 - its leaf index.
+        link = {h: i for i, h in enumerate(split_h)}
+        link.update((h, -1 - j) for j, h in enumerate(leaf_h))
 
         S = max(len(split_h), 1)
         w = np.zeros((S, d))
@@ -1034,31 +1066,52 @@ class ServingEngine:
         child = np.zeros((S, 2), dtype=np.int32)
         splits = []
         for i, h in enumerate(split_h):
-            node = job.nodes[h]
-            w[i], z[i] = node.split[0], node.split[1]
-            child[i] = link(2 * h), link(2 * h + 1)
-            splits.append((h, node.depth, node.split[0], node.split[1], node.split[2]))
-        leaves, refs, bads = [], [], []
-        coef = np.empty((len(leaf_h), p))
+            node = nodes[h]
+            sw, sz, meta = node.split
+            w[i], z[i] = sw, sz
+            child[i, 0], child[i, 1] = link[2 * h], link[2 * h + 1]
+            splits.append((h, node.depth, sw, sz, meta))
+        # Leaf fits and systems are rows of their launches' records and level
+        # tensors: gather them launch by launch.
+        L = len(leaf_h)
+        depth, rows, bads, errors = [0] * L, [0] * L, [0] * L, {}
+        by_rec, by_gram = {}, {}
         for j, h in enumerate(leaf_h):
-            node = job.nodes[h]
-            coef[j] = node.leaf["coef"]
-            leaves.append((h, node.depth, {k: v for k, v in node.leaf.items() if k != "coef"}))
-            refs.append(node.gram)
-            bads.append(node.bad)
-        flat = FlatTree.from_tables(w, z, child, link(1), coef, d, (degree, True, False))
-        # Gather each leaf's system from the level tensor that holds it.
-        grams = torch.empty((len(refs), refs[0][0].shape[1]), dtype=torch.float64,
-                            device=self.device)
-        by_level = collections.defaultdict(list)
-        for j, (t, i) in enumerate(refs):
-            by_level[id(t)].append((j, i, t))
-        for lst in by_level.values():
-            dst = torch.tensor([j for j, _, _ in lst], device=self.device)
-            src = torch.tensor([i for _, i, _ in lst], device=self.device)
-            grams.index_copy_(0, dst, lst[0][2].index_select(0, src))
-        bad = torch.tensor(bads, dtype=torch.int32, device=self.device)
-        status = np.array([meta["status"] for _, _, meta in leaves])
+            node = nodes[h]
+            rec, i = node.leaf
+            e = by_rec.get(id(rec))
+            if e is None:
+                e = by_rec[id(rec)] = (rec, [], [])
+            e[1].append(j)
+            e[2].append(i)
+            t, i = node.gram
+            e = by_gram.get(id(t))
+            if e is None:
+                e = by_gram[id(t)] = (t, [], [])
+            e[1].append(j)
+            e[2].append(i)
+            depth[j], rows[j], bads[j] = node.depth, node.n_rows, node.bad
+            if node.err is not None:
+                errors[j] = node.err
+        coef = np.empty((L, p))
+        status = np.empty(L, dtype=np.int64)
+        mse = np.empty(L)
+        for rec, js, rs in by_rec.values():
+            coef[js], status[js], mse[js] = rec.coef[rs], rec.status[rs], rec.mse[rs]
+        flat = FlatTree.from_tables(w, z, child, link[1], coef, d, (degree, True, False),
+                                    non_blocking=True)
+        lists = list(by_gram.values())
+        grams = torch.empty((L, lists[0][0].shape[1]), dtype=torch.float64, device=self.device)
+        idx = _hip.upload(np.array([j for lst in lists for j in lst[1]]
+                                   + [i for lst in lists for i in lst[2]] + bads,
+                                   dtype=np.int64), self.device)
+        at = 0
+        for t, js, _ in lists:
+            n = len(js)
+            grams.index_copy_(0, idx[at:at + n], t.index_select(0, idx[L + at:L + at + n]))
+            at += n
+        bad = idx[2 * L:].to(torch.int32)
+        leaves = (leaf_h, depth, status.copy(), mse, np.asarray(rows, dtype=np.int64), errors)
         res = _Resident(job.tree_id, flat, grams, bad, d, degree, p, self.config.ridge, splits,
                         leaves, job.n_rows)
         res.coef = coef
