@@ -182,3 +182,39 @@ def upload(a, device, dtype=None):
 
     t = torch.from_numpy(np.ascontiguousarray(a, dtype=dtype))
     return t.pin_memory().to(device, non_blocking=True)
+
+
+class PinnedRing:
+    """Pinned host staging for small uploads that one thread enqueues on one
+    stream: a bump allocator over one pinned buffer (no allocation per
+    upload).  When it wraps, the stream is synchronized first, so every copy
+    of the previous lap has run before its bytes are overwritten.  Uploads
+    larger than a quarter of the ring go through ``upload``."""
+
+    ALIGN = 256
+
+    def __init__(self, stream, nbytes=8 << 20):
+        import torch
+
+        self.torch, self.stream = torch, stream
+        self.buf = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
+        self.host = self.buf.numpy()
+        self.size, self.at = nbytes, 0
+
+    def upload(self, a, device, dtype=None):
+        import numpy as np
+
+        torch = self.torch
+        a = np.ascontiguousarray(a, dtype=dtype)
+        n = a.nbytes
+        if 4 * n > self.size:
+            return upload(a, device)
+        if self.at + n > self.size:
+            self.stream.synchronize()
+            self.at = 0
+        at = self.at
+        self.host[at:at + n] = a.reshape(-1).view(np.uint8)
+        self.at = (at + n + self.ALIGN - 1) // self.ALIGN * self.ALIGN
+        src = self.buf[at:at + n].view(torch.from_numpy(a[:0]).dtype).view(a.shape)
+        with torch.cuda.stream(self.stream):
+            return src.to(device, non_blocking=True)

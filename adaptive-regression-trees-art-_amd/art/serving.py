@@ -211,8 +211,9 @@ def gram_sse(grams, beta, p, *, bias_col=0, stream=None):
     return sse, wsum
 
 
-def gather_strided(X, y, w, begin, end, counts, *, stream=None):
-    """Strided row samples of ranges [begin[k], end[k]) with counts[k] rows each."""
+def gather_strided(X, y, w, begin, end, counts, *, stream=None, staging=None):
+    """Strided row samples of ranges [begin[k], end[k]) with counts[k] rows
+    each (staging: a PinnedRing on the stream for the range table)."""
     torch = _hip.torch_device()
     lib = _hip.load()
     dev, d = X.device, int(X.shape[1])
@@ -222,8 +223,9 @@ def gather_strided(X, y, w, begin, end, counts, *, stream=None):
     Xo = torch.empty((total, d), dtype=torch.float64, device=dev)
     yo = torch.empty(total, dtype=torch.float64, device=dev)
     wo = None if w is None else torch.empty(total, dtype=torch.float64, device=dev)
-    table = _hip.upload(np.concatenate([np.asarray(begin, np.int64), np.asarray(end, np.int64),
-                                        dst]), dev)
+    put = _hip.upload if staging is None else staging.upload
+    table = put(np.concatenate([np.asarray(begin, np.int64), np.asarray(end, np.int64), dst]),
+                dev)
     K = len(counts)
     b, e, dd = table[:K], table[K:2 * K], table[2 * K:]
     stream = torch.cuda.current_stream() if stream is None else stream
@@ -234,10 +236,12 @@ def gather_strided(X, y, w, begin, end, counts, *, stream=None):
     return Xo, yo, wo, dst
 
 
-def partition_rows(X, y, w, begin, end, node_w, node_z, out, out_base, *, stream=None):
+def partition_rows(X, y, w, begin, end, node_w, node_z, out, out_base, *, stream=None,
+                   staging=None):
     """Stable partition of ranges [begin[k], end[k]) by hyperplanes (node_w, node_z)
     into ``out`` = (Xo, yo, wo) at rows out_base[k] + ...; returns the right
-    counts (host int64 array, synchronizes)."""
+    counts (host int64 array, synchronizes).  staging: a PinnedRing on the
+    stream for the tables."""
     torch = _hip.torch_device()
     lib = _hip.load()
     dev, d = X.device, int(X.shape[1])
@@ -250,11 +254,12 @@ def partition_rows(X, y, w, begin, end, node_w, node_z, out, out_base, *, stream
     nblk = (end - begin + rows - 1) // rows
     blk = np.concatenate([[0], np.cumsum(nblk)]).astype(np.int64)
     blocks = int(blk[-1])
-    table = _hip.upload(np.concatenate([begin, end, blk, np.asarray(out_base, np.int64)]), dev)
+    put = _hip.upload if staging is None else staging.upload
+    table = put(np.concatenate([begin, end, blk, np.asarray(out_base, np.int64)]), dev)
     tb, te = table[:K], table[K:2 * K]
     tblk, tdst = table[2 * K:3 * K + 1], table[3 * K + 1:]
-    planes = _hip.upload(np.concatenate([np.asarray(node_w, np.float64).reshape(-1),
-                                         np.asarray(node_z, np.float64).reshape(-1)]), dev)
+    planes = put(np.concatenate([np.asarray(node_w, np.float64).reshape(-1),
+                                 np.asarray(node_z, np.float64).reshape(-1)]), dev)
     nw, nz = planes[:K * d], planes[K * d:]
     flags = torch.empty(int(X.shape[0]), dtype=torch.uint8, device=dev)
     blk_right = torch.empty(max(blocks, 1), dtype=torch.int32, device=dev)
@@ -413,6 +418,7 @@ class _Group:
         self.new_jobs = []
         self.offsets = None      # row offsets of entries in the level buffer
         self.stream = None       # the group's split-search stream
+        self.staging = None      # pinned upload ring on that stream
         self.plan = None         # mixed precision: the level's leaf plan, kept for late leaves
         self.grams = None        # mixed precision: the level's FP64 systems (leaves only)
         self.bad = None
@@ -458,6 +464,7 @@ class ServingEngine:
         if self._thread is not None:
             return
         self.stream = self.torch.cuda.Stream(device=self.device)
+        self.staging = _hip.PinnedRing(self.stream)
         self._thread = threading.Thread(target=self._run, name="art-serving", daemon=True)
         self._thread.start()
 
@@ -755,7 +762,8 @@ class ServingEngine:
             end = [e for _, _, _, e, _, _ in pend]
             base = np.concatenate([[0], np.cumsum(np.subtract(end, begin))[:-1]])
             right = partition_rows(g.X, g.y, g.w, begin, end, np.stack([p[4] for p in pend]),
-                                   [p[5] for p in pend], (X, y, w), base, stream=self.stream)
+                                   [p[5] for p in pend], (X, y, w), base, stream=self.stream,
+                                   staging=self.staging)
             if g.plan is not None:
                 # A partition that leaves one side empty makes its node a leaf
                 # (below); its FP64 system comes from the previous level buffer.
@@ -809,7 +817,8 @@ class ServingEngine:
         """Device offsets of the diagonal entries of a packed (p+1)-system."""
         t = self._diag_cache.get(p)
         if t is None:
-            t = _hip.upload([_diag_offset(c, p) for c in range(p + 1)], self.device, np.int64)
+            t = self.staging.upload([_diag_offset(c, p) for c in range(p + 1)], self.device,
+                                    np.int64)
             self._diag_cache[p] = t
         return t
 
@@ -890,7 +899,7 @@ class ServingEngine:
         if cand.any():
             mg = torch.zeros((K, size), **f64)
             mb = torch.zeros(K, dtype=torch.int32, device=self.device)
-            mask = _hip.upload(cand, self.device, np.int32)
+            mask = self.staging.upload(cand, self.device, np.int32)
             g.plan.grams(g.X, g.y, g.w, cfg.weight_floor, precision="mixed", mask=mask,
                          out=(mg, mb))
             coef, status, _ = solve_grams(mg, mb, d, cfg.ridge, degree=degree, stream=self.stream)
@@ -941,7 +950,7 @@ class ServingEngine:
         from .leaf_ridge import solve_grams
 
         torch, cfg, d = self.torch, self.config, g.d
-        mask = _hip.upload(sel, self.device, np.int32)
+        mask = self.staging.upload(sel, self.device, np.int32)
         g.plan.grams(g.X, g.y, g.w, cfg.weight_floor, mask=mask, out=(g.grams, g.bad))
         coef, status, _ = solve_grams(g.grams, g.bad, d, cfg.ridge, degree=degree,
                                       stream=self.stream)
@@ -1007,8 +1016,11 @@ class ServingEngine:
         R, count = cfg.n_restarts, len(idx)
         keys = [start_key(g.entries[i][0].req.seed, g.entries[i][1].heap, r)
                 for r in range(R) for i in idx]
+        if g.staging is None:
+            g.staging = _hip.PinnedRing(g.stream, 1 << 20)
         with torch.cuda.stream(g.stream):
-            Xs, ys, _, dst = gather_strided(g.X, g.y, None, begin, end, counts, stream=g.stream)
+            Xs, ys, _, dst = gather_strided(g.X, g.y, None, begin, end, counts, stream=g.stream,
+                                            staging=g.staging)
             with RaggedOptimizerWorkspace(splitter, Xs, ys, dst, temperature=temps) as ws:
                 slots = [k for _ in range(R) for k in range(count)]
                 theta0 = ws._device_starts(slots, keys=keys).reshape(R, count, d + 1)
@@ -1099,10 +1111,10 @@ class ServingEngine:
         for rec, js, rs in by_rec.values():
             coef[js], status[js], mse[js] = rec.coef[rs], rec.status[rs], rec.mse[rs]
         flat = FlatTree.from_tables(w, z, child, link[1], coef, d, (degree, True, False),
-                                    non_blocking=True)
+                                    upload=self.staging.upload)
         lists = list(by_gram.values())
         grams = torch.empty((L, lists[0][0].shape[1]), dtype=torch.float64, device=self.device)
-        idx = _hip.upload(np.array([j for lst in lists for j in lst[1]]
+        idx = self.staging.upload(np.array([j for lst in lists for j in lst[1]]
                                    + [i for lst in lists for i in lst[2]] + bads,
                                    dtype=np.int64), self.device)
         at = 0

@@ -123,22 +123,22 @@ class FlatTree:
         self._upload(w, z, child, links[id(root)], coef, d, basis, len(splits))
 
     @classmethod
-    def from_tables(cls, w, z, child, root, coef, d, basis, *, non_blocking=False):
+    def from_tables(cls, w, z, child, root, coef, d, basis, *, upload=None):
         """Device tables from host arrays already in this layout: w (S, d),
         z (S,), child links (S, 2) int32, root link, leaf coefficients (L, p)
         and basis (degree, bias, bias_last); S >= 1 rows (a placeholder row
-        when the tree is one leaf).  non_blocking: one pinned copy enqueued
-        on the current stream, for callers that use the tables on that
-        stream only."""
+        when the tree is one leaf).  upload(array, device, dtype): one
+        non-blocking staged copy (``_hip.upload`` or a ``PinnedRing``'s), for
+        callers that use the tables on the stream it enqueues on."""
         self = cls.__new__(cls)
         self.leaf_nodes = None
         n_splits = int(np.asarray(child).shape[0]) if root >= 0 else 0
         self._upload(np.asarray(w, dtype=float), np.asarray(z, dtype=float),
                      np.asarray(child, dtype=np.int32), int(root), np.asarray(coef, dtype=float),
-                     int(d), basis, n_splits, non_blocking)
+                     int(d), basis, n_splits, upload)
         return self
 
-    def _upload(self, w, z, child, root, coef, d, basis, n_splits, non_blocking=False):
+    def _upload(self, w, z, child, root, coef, d, basis, n_splits, upload=None):
         import torch
 
         from . import _hip
@@ -146,7 +146,7 @@ class FlatTree:
         degree, bias, bias_last = basis
         self.torch, self.lib = _hip.torch_device(), _hip.load()
         dev = torch.device("cuda", torch.cuda.current_device())
-        if non_blocking:
+        if upload is not None:
             # One staging copy: the float tables, then the int32 links as
             # float64-sized slots reinterpreted on the device.
             parts = [np.ascontiguousarray(w).reshape(-1), np.ascontiguousarray(z).reshape(-1),
@@ -155,7 +155,7 @@ class FlatTree:
             pad = np.zeros(links.size + (links.size & 1), dtype=np.int32)
             pad[:links.size] = links
             parts.append(pad.view(np.float64))
-            blob = _hip.upload(np.concatenate(parts), dev, np.float64)
+            blob = upload(np.concatenate(parts), dev, np.float64)
             nw, nz, nc = w.size, z.size, coef.size
             self.w = blob[:nw].view(w.shape)
             self.z = blob[nw:nw + nz]
