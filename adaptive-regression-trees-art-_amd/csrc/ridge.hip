@@ -1379,6 +1379,10 @@ struct art_ridge_plan {
   int* d_roff = nullptr;
   LeafChunk* d_rchunks = nullptr;
   int nrchunks = 0;
+  // Leaf plans: one device allocation holds the tables and the Gram work
+  // buffers (the pointers above that fall inside it are not freed alone).
+  unsigned char* arena = nullptr;
+  size_t arena_bytes = 0;
   double refine_tol = ART_MIXED_REFINE_TOL;
   int refine_steps = ART_MIXED_REFINE_STEPS;  // extra steps on flagged leaves
   double* d_dnorm = nullptr;
@@ -1443,22 +1447,27 @@ int art_check_finite(const double* a, const double* y, int64_t groups, int64_t r
 
 static void plan_free(art_ridge_plan* P) {
   if (!P) return;
-  (void)hipFree(P->d_chunks);
-  (void)hipFree(P->d_sys);
-  (void)hipFree(P->d_partials);
-  (void)hipFree(P->d_flags);
-  (void)hipFree(P->d_scratch);
+  auto release = [P](void* ptr) {
+    const auto* b = static_cast<const unsigned char*>(ptr);
+    if (ptr != nullptr && !(b >= P->arena && b < P->arena + P->arena_bytes)) (void)hipFree(ptr);
+  };
+  release(P->d_chunks);
+  release(P->d_sys);
+  release(P->d_partials);
+  release(P->d_flags);
+  release(P->d_scratch);
   if (P->h_sys_pinned) (void)hipHostFree(P->h_sys_pinned);
   if (P->sys_ready) (void)hipEventDestroy(P->sys_ready);
   for (hipEvent_t e : P->tev) (void)hipEventDestroy(e);
-  (void)hipFree(P->d_factor);
-  (void)hipFree(P->d_rpart);
-  (void)hipFree(P->d_tpart);
-  (void)hipFree(P->d_need);
-  (void)hipFree(P->d_dnorm);
-  (void)hipFree(P->d_goff);
-  (void)hipFree(P->d_roff);
-  (void)hipFree(P->d_rchunks);
+  release(P->d_factor);
+  release(P->d_rpart);
+  release(P->d_tpart);
+  release(P->d_need);
+  release(P->d_dnorm);
+  release(P->d_goff);
+  release(P->d_roff);
+  release(P->d_rchunks);
+  if (P->arena) (void)hipFree(P->arena);
   delete P;
 }
 
@@ -1547,32 +1556,47 @@ int art_leaf_plan_create(const int64_t* offsets, int K, int d, int degree, int i
   if (rchunks.empty()) rchunks.push_back(LeafChunk{0, 0, 0, 0});
   P->nrchunks = (int)rchunks.size();
   launch_order(rchunks);
-  if (hipMalloc(&P->d_goff, (K + 1) * sizeof(int)) != hipSuccess ||
-      hipMalloc(&P->d_roff, (K + 1) * sizeof(int)) != hipSuccess ||
-      hipMalloc(&P->d_rchunks, rchunks.size() * sizeof(LeafChunk)) != hipSuccess ||
-      hipMemcpy(P->d_goff, goff.data(), (K + 1) * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(P->d_roff, roff.data(), (K + 1) * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(P->d_rchunks, rchunks.data(), rchunks.size() * sizeof(LeafChunk),
-                hipMemcpyHostToDevice) != hipSuccess) {
+  // One allocation: the tables (uploaded by one copy), then the chunk weight
+  // flags and traces, the chunk partial systems and, for systems too wide
+  // for LDS, the solve scratch; every part starts on a 256-byte boundary.
+  auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t b_off = (size_t)(K + 1) * sizeof(int);
+  const size_t b_rch = rchunks.size() * sizeof(LeafChunk);
+  const size_t b_ch = chunks.size() * sizeof(LeafChunk), b_sys = sys.size() * sizeof(SystemDesc);
+  const size_t o_goff = 0, o_roff = up(o_goff + b_off), o_rch = up(o_roff + b_off);
+  const size_t o_ch = up(o_rch + b_rch), o_sys = up(o_ch + b_ch), tables = up(o_sys + b_sys);
+  const size_t o_flags = tables, o_tpart = up(o_flags + (size_t)P->nchunks * sizeof(int));
+  const size_t o_part = up(o_tpart + (size_t)P->nchunks * sizeof(double));
+  const size_t nel = (size_t)(P->p + 1) * (P->p + 2) / 2;
+  const size_t b_part = (size_t)P->nchunks * P->npairs * 256 * sizeof(double);
+  const bool scratch = nel * sizeof(double) > ART_SOLVE_LDS_BYTES;
+  const size_t o_scr = up(o_part + b_part);
+  const size_t total = scratch ? o_scr + (size_t)K * nel * sizeof(double) : o_scr;
+  std::vector<unsigned char> host(tables, 0);
+  std::memcpy(host.data() + o_goff, goff.data(), b_off);
+  std::memcpy(host.data() + o_roff, roff.data(), b_off);
+  std::memcpy(host.data() + o_rch, rchunks.data(), b_rch);
+  std::memcpy(host.data() + o_ch, chunks.data(), b_ch);
+  std::memcpy(host.data() + o_sys, sys.data(), b_sys);
+  if (hipMalloc(&P->arena, total) != hipSuccess) {
+    P->arena = nullptr;
     plan_free(P);
-    return fail(ART_ERR_HIP, "art_leaf_plan_create: residual table upload failed");
+    return fail(ART_ERR_ALLOC, "art_leaf_plan_create: hipMalloc failed");
   }
-  int rc;
-  if (hipMalloc(&P->d_chunks, chunks.size() * sizeof(LeafChunk)) != hipSuccess ||
-      hipMalloc(&P->d_sys, sys.size() * sizeof(SystemDesc)) != hipSuccess) {
+  P->arena_bytes = total;
+  unsigned char* A = P->arena;
+  P->d_goff = reinterpret_cast<int*>(A + o_goff);
+  P->d_roff = reinterpret_cast<int*>(A + o_roff);
+  P->d_rchunks = reinterpret_cast<LeafChunk*>(A + o_rch);
+  P->d_chunks = reinterpret_cast<LeafChunk*>(A + o_ch);
+  P->d_sys = reinterpret_cast<SystemDesc*>(A + o_sys);
+  P->d_flags = reinterpret_cast<int*>(A + o_flags);
+  P->d_tpart = reinterpret_cast<double*>(A + o_tpart);
+  P->d_partials = reinterpret_cast<double*>(A + o_part);
+  if (scratch) P->d_scratch = reinterpret_cast<double*>(A + o_scr);
+  if (hipMemcpy(A, host.data(), tables, hipMemcpyHostToDevice) != hipSuccess) {
     plan_free(P);
-    return fail(ART_ERR_HIP, "art_leaf_plan_create: hipMalloc failed");
-  }
-  if ((rc = plan_alloc_common(P, 1)) != ART_OK) {
-    plan_free(P);
-    return rc;
-  }
-  if (hipMemcpy(P->d_chunks, chunks.data(), chunks.size() * sizeof(LeafChunk),
-                hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(P->d_sys, sys.data(), sys.size() * sizeof(SystemDesc), hipMemcpyHostToDevice) !=
-          hipSuccess) {
-    plan_free(P);
-    return fail(ART_ERR_HIP, "art_leaf_plan_create: upload failed");
+    return fail(ART_ERR_HIP, "art_leaf_plan_create: table upload failed");
   }
   *out = P;
   return ART_OK;
@@ -1728,7 +1752,8 @@ static int leaf_fit_mixed(art_ridge_plan* P, const double* X, const double* y, c
     ART_HIP(hipMalloc(&P->d_factor, (size_t)P->K * (p * (p + 1) / 2) * sizeof(double)));
     const size_t rp = std::max((size_t)P->nchunks * 16 * T, (size_t)P->nrchunks * p);
     ART_HIP(hipMalloc(&P->d_rpart, rp * sizeof(double)));
-    ART_HIP(hipMalloc(&P->d_tpart, (size_t)P->nchunks * sizeof(double)));
+    if (P->d_tpart == nullptr)
+      ART_HIP(hipMalloc(&P->d_tpart, (size_t)P->nchunks * sizeof(double)));
     ART_HIP(hipMalloc(&P->d_need, (size_t)P->K * sizeof(int)));
     ART_HIP(hipMalloc(&P->d_dnorm, (size_t)P->K * sizeof(double)));
   }
