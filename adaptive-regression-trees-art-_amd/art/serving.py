@@ -294,7 +294,7 @@ class _Node:
     heap: int
     depth: int
     n_rows: int
-    split: Optional[tuple] = None      # (w, z, metadata)
+    split: Optional[tuple] = None      # (w, z, sample rows)
     leaf: Optional[tuple] = None       # (_Fits, row) of a leaf's model
     gram: Any = None                   # (level systems tensor, row) of a leaf
     bad: int = 0
@@ -339,7 +339,7 @@ class _Resident:
     def __init__(self, tree_id, flat, grams, bad, d, degree, p, ridge, splits, leaves, n_rows):
         self.tree_id, self.flat, self.grams, self.bad = tree_id, flat, grams, bad
         self.d, self.degree, self.p, self.ridge = d, degree, p, ridge
-        self.splits = splits      # [(heap, depth, w, z, metadata)] in heap order
+        self.splits = splits      # [(heap, depth, w, z, sample rows)] in heap order
         # Leaves in heap order (= flat leaf order): heaps and depths (lists),
         # status, MSE and row counts (arrays), {leaf: split-search error}.
         self.leaf_heap, self.leaf_depth, self.status, self.mse, self.leaf_rows, \
@@ -363,7 +363,7 @@ class _Resident:
         from .models import PolynomialRidgeModel
         from .tree import LeafNode, RegressionTree, SplitNode
 
-        split_of = {h: (depth, w, z, meta) for h, depth, w, z, meta in self.splits}
+        split_of = {h: (depth, w, z, {"sample_rows": n}) for h, depth, w, z, n in self.splits}
         metas = []
         for j in range(len(self.leaf_heap)):
             meta = {"status": int(self.status[j]), "mse": float(self.mse[j]),
@@ -744,7 +744,7 @@ class ServingEngine:
         """Next level buffer: children of last level's splits, then new roots."""
         torch, d = self.torch, g.d
         pend = g.pending or []
-        child_rows = sum(int(e - b) for _, _, b, e, _, _ in pend)
+        child_rows = sum(e - b for _, _, b, e, _, _ in pend)
         root_rows = sum(job.n_rows for job in g.new_jobs)
         total = child_rows + root_rows
         if total == 0:
@@ -764,34 +764,33 @@ class ServingEngine:
             right = partition_rows(g.X, g.y, g.w, begin, end, np.stack([p[4] for p in pend]),
                                    [p[5] for p in pend], (X, y, w), base, stream=self.stream,
                                    staging=self.staging)
+            right = right.tolist()
             if g.plan is not None:
                 # A partition that leaves one side empty makes its node a leaf
                 # (below); its FP64 system comes from the previous level buffer.
                 self._late_leaves(g, [node for (_, node, b, e, _, _), nr in zip(pend, right)
-                                      if int(nr) == 0 or int(nr) == int(e - b)])
+                                      if nr == 0 or nr == e - b])
             self._close_plan(g)
             for (job, node, b, e, _, _), nr in zip(pend, right):
-                n = int(e - b)
-                nl = n - int(nr)
-                for side, cnt in ((0, nl), (1, int(nr))):
-                    if nl == 0 or nr == 0:
-                        # Degenerate partition (sample and rows disagree at the
-                        # boundary): the node becomes a leaf, its rows unused.
-                        entries.append((None, None))
-                    else:
-                        child = _Node(2 * node.heap + side, node.depth + 1, cnt)
-                        job.nodes[child.heap] = child
-                        job.open += 1
-                        entries.append((job, child))
-                    sizes.append(cnt)
+                nl = e - b - nr
                 if nl == 0 or nr == 0:
+                    # Degenerate partition (sample and rows disagree at the
+                    # boundary): the node becomes a leaf, its rows unused.
+                    entries += ((None, None), (None, None))
+                    sizes += (nl, nr)
                     node.split = None
                     node.leaf = node.leaf_candidate
                     node.gram = node.gram_candidate
-                else:
-                    node.leaf_candidate = node.gram_candidate = None
-                job.open -= 1
-                self._maybe_finish(job)
+                    job.open -= 1
+                    self._maybe_finish(job)
+                    continue
+                h, depth, nodes = 2 * node.heap, node.depth + 1, job.nodes
+                left = nodes[h] = _Node(h, depth, nl)
+                rgt = nodes[h + 1] = _Node(h + 1, depth, nr)
+                entries += ((job, left), (job, rgt))
+                sizes += (nl, nr)
+                node.leaf_candidate = node.gram_candidate = None
+                job.open += 1   # two children open, the node itself decided
         self._close_plan(g)
         off = child_rows
         for job in g.new_jobs:
@@ -1031,10 +1030,16 @@ class ServingEngine:
         from .batched_optimizer import STOP_REASONS
 
         pending = []
+        ok = np.asarray(ok, dtype=bool).tolist()
         if g.plan is not None:
             self._late_leaves(g, [g.entries[i][1] for k, i in enumerate(idx) if not ok[k]])
+        # Each split's weights are a row of one copy of the batch's planes.
+        W = np.array(theta[:, :-1], dtype=np.float64)
+        Z, C = theta[:, -1].tolist(), np.asarray(counts).tolist()
+        B, E = np.asarray(begin).tolist(), np.asarray(end).tolist()
+        entries = g.entries
         for k, i in enumerate(idx):
-            job, node = g.entries[i]
+            job, node = entries[i]
             if not ok[k]:
                 node.leaf, node.gram = node.leaf_candidate, node.gram_candidate
                 node.err = STOP_REASONS[int(reason[k])]
@@ -1042,9 +1047,9 @@ class ServingEngine:
                 job.open -= 1
                 self._maybe_finish(job)
                 continue
-            node.split = (theta[k, :-1].copy(), float(theta[k, -1]),
-                          {"sample_rows": int(counts[k])})
-            pending.append((job, node, int(begin[k]), int(end[k]), node.split[0], node.split[1]))
+            wk, zk = W[k], Z[k]
+            node.split = (wk, zk, C[k])
+            pending.append((job, node, B[k], E[k], wk, zk))
         g.pending = pending or None
 
     # ----------------------------------------------------------------- completion
@@ -1079,10 +1084,10 @@ class ServingEngine:
         splits = []
         for i, h in enumerate(split_h):
             node = nodes[h]
-            sw, sz, meta = node.split
+            sw, sz, n = node.split
             w[i], z[i] = sw, sz
             child[i, 0], child[i, 1] = link[2 * h], link[2 * h + 1]
-            splits.append((h, node.depth, sw, sz, meta))
+            splits.append((h, node.depth, sw, sz, n))
         # Leaf fits and systems are rows of their launches' records and level
         # tensors: gather them launch by launch.
         L = len(leaf_h)
