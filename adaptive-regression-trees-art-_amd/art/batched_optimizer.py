@@ -24,7 +24,8 @@ from .batched_ridge import _dimensions, _node_ids
 from .features import feature_count
 from .models import AffineRidgeModel, PolynomialRidgeModel, RidgeSolveResult, model_basis
 from .optimizers import AdaptiveAlpha
-from .splitters import SplitNotFoundError, SplitResult, _partition_result, project_unit_w
+from .splitters import (SplitNotFoundError, SplitResult, _partition_result, project_unit_w,
+                        project_unit_w_rows)
 
 STOP_REASONS = ("max_iters", "gradient_tolerance", "non_descent_direction", "line_search_failed",
                 "max_line_search_failures", "alpha_min_reached", "cholesky_failure",
@@ -388,9 +389,14 @@ class BatchedOptimizerWorkspace:
         if starts.shape != (R, count, self.d + 1) or not np.isfinite(starts).all():
             raise ValueError("theta0 must be finite with shape (R, k, d+1).")
         nodes = np.tile(self.all_ids[:count], R)
-        thetas = np.stack([project_unit_w(t) for t in starts.reshape(-1, self.d + 1)])
+        thetas = project_unit_w_rows(starts.reshape(-1, self.d + 1))
         theta, _, hard, info, _, _ = self._launch(nodes, thetas, 0, self.s.min_side_fallback,
                                                   True, timing=timing)
+        if R == 1:  # one start per node: its split, or its stop reason
+            ok = info[:count, 6] == 0
+            out = np.where(ok[:, None], theta[:count], 0.0)
+            reason = np.where(ok, -1, info[:count, 0]).astype(np.int64)
+            return out, ok, reason
         out = np.zeros((count, self.d + 1))
         ok = np.zeros(count, dtype=bool)
         reason = np.full(count, -1, dtype=np.int64)
@@ -645,7 +651,8 @@ class RaggedOptimizerWorkspace(BatchedOptimizerWorkspace):
             raise ValueError("Temperatures must be finite and positive.")
         self._setup(splitter, K, int(sizes.max()), X.shape[1], reorder=True)
         self.sizes = sizes
-        self.required_k = [splitter._required_side_points(int(n)) for n in sizes]
+        self.required_k = np.maximum(int(splitter.min_side_points),
+                                     np.ceil(splitter.min_side_fraction * sizes).astype(np.int64))
         torch = self.torch
         with torch.cuda.stream(self.stream):
             self.x = X if on_device else torch.from_numpy(X).to(self.device)
@@ -664,7 +671,7 @@ class RaggedOptimizerWorkspace(BatchedOptimizerWorkspace):
         return int(self.sizes[j])
 
     def _required(self, j):
-        return self.required_k[j]
+        return int(self.required_k[j])
 
     def _ragged_args(self, a):
         a.n = 0

@@ -170,6 +170,26 @@ def start_key(seed, heap, restart):
     return _splitmix64((int(seed) & _MASK64) ^ _splitmix64((int(heap) << 8) | int(restart)))
 
 
+def _splitmix64_u64(x):
+    """_splitmix64 of a uint64 array (numpy's uint64 arithmetic wraps mod 2^64)."""
+    u = np.uint64
+    x = x + u(0x9E3779B97F4A7C15)
+    x = (x ^ (x >> u(30))) * u(0xBF58476D1CE4E5B9)
+    x = (x ^ (x >> u(27))) * u(0x94D049BB133111EB)
+    return x ^ (x >> u(31))
+
+
+def start_keys(seeds, heaps, restarts):
+    """start_key of every (seed, heap, restart) triple of equal-length
+    integer arrays, as a uint64 array."""
+    u = np.uint64
+    seeds = np.asarray(seeds, dtype=np.uint64)   # nonnegative seeds below 2^64
+    heaps = np.asarray(heaps, dtype=np.uint64)
+    restarts = np.asarray(restarts, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        return _splitmix64_u64(seeds ^ _splitmix64_u64((heaps << u(8)) | restarts))
+
+
 # Mixed-precision stop test: a node goes to the split search on its mixed
 # statistics only when its mixed MSE exceeds the error threshold by
 # _MIXED_SSE_MARGIN * S^2 / sum(w), S = sum_i |b_i| sqrt(G_ii) + sqrt(y'Wy);
@@ -289,23 +309,13 @@ class _Fits:
             self.mse = np.where(wsum > 0, sse / wsum, np.inf)
 
 
-@dataclass(slots=True)
-class _Node:
-    heap: int
-    depth: int
-    n_rows: int
-    split: Optional[tuple] = None      # (w, z, sample rows)
-    leaf: Optional[tuple] = None       # (_Fits, row) of a leaf's model
-    gram: Any = None                   # (level systems tensor, row) of a leaf
-    bad: int = 0
-    leaf_candidate: Optional[tuple] = None   # the node's own fit while its split is open
-    gram_candidate: Any = None
-    index: int = -1                    # row of the node in its level's tables
-    err: Optional[str] = None          # why the split search failed, for a leaf
-
-
 @dataclass
 class _FitJob:
+    """An admitted fit.  Its decided nodes accumulate as blocks of arrays, one
+    per decision batch: leaf blocks (heap, depth, rows, record, record rows,
+    FP64 systems tensor, tensor rows, weight flags, split-search errors or
+    None) and split blocks (heap, depth, w (k, d), z, sample rows)."""
+
     req: FitRequest
     future: Future
     timing: RequestTiming
@@ -317,8 +327,10 @@ class _FitJob:
     X: Any = None
     y: Any = None
     w: Any = None
-    nodes: dict = field(default_factory=dict)
     open: int = 1                       # nodes not yet decided
+    slot: int = -1                      # the job's slot in its group
+    leaves: list = field(default_factory=list)
+    splits: list = field(default_factory=list)
 
 
 @dataclass
@@ -339,7 +351,7 @@ class _Resident:
     def __init__(self, tree_id, flat, grams, bad, d, degree, p, ridge, splits, leaves, n_rows):
         self.tree_id, self.flat, self.grams, self.bad = tree_id, flat, grams, bad
         self.d, self.degree, self.p, self.ridge = d, degree, p, ridge
-        self.splits = splits      # [(heap, depth, w, z, sample rows)] in heap order
+        self.splits = splits      # (heap, depth, w (S, d), z, sample rows) arrays, heap order
         # Leaves in heap order (= flat leaf order): heaps and depths (lists),
         # status, MSE and row counts (arrays), {leaf: split-search error}.
         self.leaf_heap, self.leaf_depth, self.status, self.mse, self.leaf_rows, \
@@ -363,7 +375,10 @@ class _Resident:
         from .models import PolynomialRidgeModel
         from .tree import LeafNode, RegressionTree, SplitNode
 
-        split_of = {h: (depth, w, z, {"sample_rows": n}) for h, depth, w, z, n in self.splits}
+        H, D, W, Z, C = self.splits
+        split_of = {h: (depth, w, z, {"sample_rows": n})
+                    for h, depth, w, z, n in zip(H.tolist(), D.tolist(), W, Z.tolist(),
+                                                 C.tolist())}
         metas = []
         for j in range(len(self.leaf_heap)):
             meta = {"status": int(self.status[j]), "mse": float(self.mse[j]),
@@ -373,7 +388,8 @@ class _Resident:
             metas.append(meta)
         self._meta = metas
         leaf_of = {h: (j, depth, metas[j])
-                   for j, (h, depth) in enumerate(zip(self.leaf_heap, self.leaf_depth))}
+                   for j, (h, depth) in enumerate(zip(self.leaf_heap.tolist(),
+                                                      self.leaf_depth.tolist()))}
         self._models = [None] * len(self.leaf_heap)
 
         def build(h):
@@ -407,14 +423,24 @@ class _Resident:
 
 
 class _Group:
-    """Level buffer of every fit request with basis (d, degree)."""
+    """Level buffer of every fit request with basis (d, degree).
+
+    The level's entries are row ranges of the buffer, as arrays: the job
+    slot (into ``jobs``; -1 for a range no node uses), the node's heap index
+    (root 1, children 2h and 2h+1), depth and row count.  Slots keep their
+    request's stop rules and search settings as arrays."""
+
+    _PARAMS = ("max_depth", "min_leaf", "thr", "temp", "seed")
 
     def __init__(self, d, degree):
         self.d, self.degree = d, degree
         self.X = self.y = self.w = None
-        self.entries = []        # (job, node) in buffer order; node None = unused range
-        self.sizes = []
-        self.pending = None      # splits to apply: [(job, node, begin, end, w, z)]
+        empty = np.zeros(0, dtype=np.int64)
+        self.jid, self.heap, self.depth, self.sizes = empty, empty, empty, empty
+        self.jobs = []           # job per slot (None once finished)
+        self._params = {k: [] for k in self._PARAMS}
+        self._param_arrays = None
+        self.pending = None      # splits to apply: dict of arrays (idx, begin, end, W, Z, counts)
         self.new_jobs = []
         self.offsets = None      # row offsets of entries in the level buffer
         self.stream = None       # the group's split-search stream
@@ -422,6 +448,30 @@ class _Group:
         self.plan = None         # mixed precision: the level's leaf plan, kept for late leaves
         self.grams = None        # mixed precision: the level's FP64 systems (leaves only)
         self.bad = None
+        # The searched entries' own fits, used if they end as leaves: record
+        # c_recs[c_sel[i]] row i; FP64 system row i of c_gram_t when c_gram[i]
+        # (mixed: formed late, from the rows still in the level buffer).
+        self.c_recs, self.c_sel, self.c_gram, self.c_gram_t, self.c_bad = [], None, None, None, None
+
+    def add_job(self, job):
+        req = job.req
+        job.slot = len(self.jobs)
+        self.jobs.append(job)
+        for k, v in zip(self._PARAMS, (req.max_depth, req.min_leaf, req.error_threshold,
+                                       req.temperature, int(req.seed) & _MASK64)):
+            self._params[k].append(v)
+        self._param_arrays = None
+
+    def params(self):
+        """Per-slot arrays of the stop rules and search settings."""
+        if self._param_arrays is None:
+            P = {k: np.asarray(v) for k, v in self._params.items()}
+            P["seed"] = np.asarray(self._params["seed"], dtype=np.uint64)
+            self._param_arrays = P
+        return self._param_arrays
+
+    def live_jobs(self):
+        return [j for j in self.jobs if j is not None]
 
 
 class ServingEngine:
@@ -560,7 +610,7 @@ class ServingEngine:
             if not fut.done():
                 fut.set_exception(exc)
         for g in self._groups.values():
-            for job in {id(j): j for j, _ in g.entries if j is not None}.values():
+            for job in g.live_jobs():
                 if not job.future.done():
                     job.future.set_exception(exc)
             for job in g.new_jobs:
@@ -663,12 +713,13 @@ class ServingEngine:
                 job.X = self._to_device(req.X, (n, d), "X")
                 job.y = self._to_device(req.y, (n,), "y")
                 job.w = self._to_device(req.w, (n,), "w")
-                job.nodes[1] = _Node(1, 0, n)
                 job.timing.admitted = time.perf_counter()
                 self._inflight_bytes += est
                 self._active += 1
                 self._fitting.add(tree_id)
-                self._groups.setdefault((d, degree), _Group(d, degree)).new_jobs.append(job)
+                g = self._groups.setdefault((d, degree), _Group(d, degree))
+                g.add_job(job)
+                g.new_jobs.append(job)
             except Exception as exc:  # a bad request fails alone
                 fut.set_exception(exc)
         self._waiting = keep
@@ -720,11 +771,11 @@ class ServingEngine:
             t = time.perf_counter()
             self._build_level(g)
             t = self._clock("partition", t)
-            if g.entries:
+            if g.sizes.size:
                 idx = self._level(g)
-                if idx:
+                if idx.size:
                     searches.append((g, idx, self._pool.submit(self._search, g, idx)))
-            if not g.entries and not g.new_jobs and g.pending is None:
+            if not g.sizes.size and not g.new_jobs and g.pending is None:
                 del self._groups[key]
         if searches:
             t = time.perf_counter()
@@ -743,71 +794,70 @@ class ServingEngine:
     def _build_level(self, g):
         """Next level buffer: children of last level's splits, then new roots."""
         torch, d = self.torch, g.d
-        pend = g.pending or []
-        child_rows = sum(e - b for _, _, b, e, _, _ in pend)
+        pend = g.pending
+        child_rows = int((pend["end"] - pend["begin"]).sum()) if pend is not None else 0
         root_rows = sum(job.n_rows for job in g.new_jobs)
         total = child_rows + root_rows
         if total == 0:
             self._close_plan(g)
-            g.entries, g.sizes, g.pending = [], [], None
+            empty = np.zeros(0, dtype=np.int64)
+            g.jid, g.heap, g.depth, g.sizes = empty, empty, empty, empty
+            g.pending = None
             g.X = g.y = g.w = None
             return
         f64 = dict(dtype=torch.float64, device=self.device)
         X = torch.empty((total, d), **f64)
         y = torch.empty(total, **f64)
         w = torch.empty(total, **f64)
-        entries, sizes = [], []
-        if pend:
-            begin = [b for _, _, b, _, _, _ in pend]
-            end = [e for _, _, _, e, _, _ in pend]
-            base = np.concatenate([[0], np.cumsum(np.subtract(end, begin))[:-1]])
-            right = partition_rows(g.X, g.y, g.w, begin, end, np.stack([p[4] for p in pend]),
-                                   [p[5] for p in pend], (X, y, w), base, stream=self.stream,
-                                   staging=self.staging)
-            right = right.tolist()
-            if g.plan is not None:
-                # A partition that leaves one side empty makes its node a leaf
-                # (below); its FP64 system comes from the previous level buffer.
-                self._late_leaves(g, [node for (_, node, b, e, _, _), nr in zip(pend, right)
-                                      if nr == 0 or nr == e - b])
+        parts = []
+        if pend is not None:
+            idx, begin, end = pend["idx"], pend["begin"], pend["end"]
+            n = end - begin
+            base = np.concatenate([[0], np.cumsum(n)[:-1]])
+            right = partition_rows(g.X, g.y, g.w, begin, end, pend["W"], pend["Z"],
+                                   (X, y, w), base, stream=self.stream, staging=self.staging)
+            left = n - right
+            # A partition that leaves one side empty (sample and rows disagree
+            # at the boundary) makes its node a leaf; its rows go unused, and
+            # under mixed precision its FP64 system comes from the previous
+            # level buffer.
+            degen = (left == 0) | (right == 0)
+            if degen.any():
+                if g.plan is not None:
+                    self._late_leaves(g, idx[degen])
+                self._close_plan(g)
+                self._candidate_leaves(g, idx[degen], None)
             self._close_plan(g)
-            for (job, node, b, e, _, _), nr in zip(pend, right):
-                nl = e - b - nr
-                if nl == 0 or nr == 0:
-                    # Degenerate partition (sample and rows disagree at the
-                    # boundary): the node becomes a leaf, its rows unused.
-                    entries += ((None, None), (None, None))
-                    sizes += (nl, nr)
-                    node.split = None
-                    node.leaf = node.leaf_candidate
-                    node.gram = node.gram_candidate
-                    job.open -= 1
-                    self._maybe_finish(job)
-                    continue
-                h, depth, nodes = 2 * node.heap, node.depth + 1, job.nodes
-                left = nodes[h] = _Node(h, depth, nl)
-                rgt = nodes[h + 1] = _Node(h + 1, depth, nr)
-                entries += ((job, left), (job, rgt))
-                sizes += (nl, nr)
-                node.leaf_candidate = node.gram_candidate = None
-                job.open += 1   # two children open, the node itself decided
+            keep = ~degen
+            self._add_splits(g, idx[keep], pend["W"][keep], pend["Z"][keep],
+                             pend["counts"][keep])
+            h = g.heap[idx]
+            parts.append((np.repeat(np.where(degen, -1, g.jid[idx]), 2),
+                          np.stack([2 * h, 2 * h + 1], axis=1).reshape(-1),
+                          np.repeat(g.depth[idx] + 1, 2),
+                          np.stack([left, right], axis=1).reshape(-1)))
         self._close_plan(g)
-        off = child_rows
-        for job in g.new_jobs:
-            n = job.n_rows
-            X[off:off + n].copy_(job.X)
-            y[off:off + n].copy_(job.y)
-            if job.w is None:
-                w[off:off + n].fill_(1.0)
-            else:
-                w[off:off + n].copy_(job.w)
-            job.X = job.y = job.w = None   # the level buffer holds the rows now
-            entries.append((job, job.nodes[1]))
-            sizes.append(n)
-            off += n
+        if g.new_jobs:
+            off = child_rows
+            for job in g.new_jobs:
+                n = job.n_rows
+                X[off:off + n].copy_(job.X)
+                y[off:off + n].copy_(job.y)
+                if job.w is None:
+                    w[off:off + n].fill_(1.0)
+                else:
+                    w[off:off + n].copy_(job.w)
+                job.X = job.y = job.w = None   # the level buffer holds the rows now
+                off += n
+            k = len(g.new_jobs)
+            parts.append((np.array([job.slot for job in g.new_jobs], dtype=np.int64),
+                          np.ones(k, dtype=np.int64), np.zeros(k, dtype=np.int64),
+                          np.array([job.n_rows for job in g.new_jobs], dtype=np.int64)))
         g.new_jobs = []
         g.X, g.y, g.w = X, y, w
-        g.entries, g.sizes, g.pending = entries, sizes, None
+        g.jid, g.heap, g.depth, g.sizes = (np.concatenate(c).astype(np.int64)
+                                           for c in zip(*parts))
+        g.pending = None
 
     def _mixed(self, d, p):
         return self.config.precision == "mixed" and p <= 64 and d <= 16
@@ -840,31 +890,26 @@ class ServingEngine:
         self.stream.synchronize()
         rec = _Fits(coef.cpu().numpy(), status.cpu().numpy(), sse.cpu().numpy(),
                     wsum.cpu().numpy(), bad.cpu().numpy())
-        mse_h, bad_h = rec.mse, rec.bad
         t = self._clock("node_fits", t)
-        split_idx = []
-        for i, (job, node) in enumerate(g.entries):
-            if job is None:
-                continue
-            req = job.req
-            stop = (node.depth >= req.max_depth or node.n_rows < 2 * req.min_leaf
-                    or node.n_rows < 2 * (d + 2) or mse_h[i] <= req.error_threshold)
-            if stop:
-                node.leaf, node.gram, node.bad = (rec, i), (grams, i), int(bad_h[i])
-                job.open -= 1
-                self._maybe_finish(job)
-            else:
-                node.leaf_candidate, node.gram_candidate = (rec, i), (grams, i)
-                node.bad = int(bad_h[i])
-                node.index = i
-                split_idx.append(i)
+        live, final, thr = self._stop_rules(g)
+        stop = live & (final | (rec.mse <= thr))
+        K = g.sizes.size
+        g.c_recs, g.c_sel = [rec], np.zeros(K, dtype=np.int64)
+        g.c_gram, g.c_gram_t, g.c_bad = np.ones(K, dtype=bool), grams, rec.bad
+        self._add_leaves(g, np.flatnonzero(stop), rec, grams, rec.bad)
         self._clock("decide", t)
-        return split_idx
+        return np.flatnonzero(live & ~stop)
 
-    def _final_by_size(self, job, node, d):
-        req = job.req
-        return (node.depth >= req.max_depth or node.n_rows < 2 * req.min_leaf
-                or node.n_rows < 2 * (d + 2))
+    def _stop_rules(self, g):
+        """Per entry: live (a node uses it), final (its depth or size stops
+        it) and its request's error threshold."""
+        P, jid, d = g.params(), g.jid, g.d
+        live = jid >= 0
+        j = np.where(live, jid, 0)
+        n = g.sizes
+        final = live & ((g.depth >= P["max_depth"][j]) | (n < 2 * P["min_leaf"][j])
+                        | (n < 2 * (d + 2)))
+        return live, final, np.where(live, P["thr"][j], 0.0)
 
     def _level_mixed(self, g, degree, p, t):
         """One level at mixed precision.  Nodes the depth and size rules leave
@@ -879,19 +924,16 @@ class ServingEngine:
         from .leaf_ridge import LeafRidgeBatch, solve_grams
 
         torch, cfg, d = self.torch, self.config, g.d
-        K = len(g.entries)
-        live = np.array([job is not None for job, _ in g.entries], dtype=bool)
-        final = np.array([job is not None and self._final_by_size(job, node, d)
-                          for job, node in g.entries], dtype=bool)
+        K = g.sizes.size
+        live, final, thr = self._stop_rules(g)
         cand = live & ~final
-        thr = np.array([job.req.error_threshold if job is not None else 0.0
-                        for job, _ in g.entries])
         g.plan = LeafRidgeBatch(g.offsets, d, degree=degree, stream=self.stream)
         size = self.lib.art_gram_size(p)
         f64 = dict(dtype=torch.float64, device=self.device)
         g.grams = torch.zeros((K, size), **f64)
         g.bad = torch.zeros(K, dtype=torch.int32, device=self.device)
         clear = np.zeros(K, dtype=bool)
+        mrec = None
         # FP64 systems of the nodes the depth and size rules stop go first, on
         # the same stream, so that both passes complete under one synchronization.
         early = self._fp64_launch(g, degree, p, final) if final.any() else None
@@ -920,27 +962,23 @@ class ServingEngine:
         near = cand & ~clear
         nrec = self._fp64_fits(g, degree, p, near) if near.any() else None
         t = time.perf_counter()
-        split_idx = []
-        grams = g.grams
-        for i in np.flatnonzero(live).tolist():
-            job, node = g.entries[i]
-            node.index = i
-            if clear[i]:
-                node.leaf_candidate = (mrec, i)
-                node.gram_candidate = None   # formed in FP64 only if the node ends as a leaf
-                split_idx.append(i)
-                continue
-            rec = frec if final[i] else nrec
-            if final[i] or rec.mse[i] <= job.req.error_threshold:
-                node.leaf, node.gram, node.bad = (rec, i), (grams, i), int(rec.bad[i])
-                job.open -= 1
-                self._maybe_finish(job)
-            else:
-                node.leaf_candidate, node.gram_candidate = (rec, i), (grams, i)
-                node.bad = int(rec.bad[i])
-                split_idx.append(i)
+        # Searched on their mixed statistics: their fit is the mixed one, and
+        # their FP64 system is formed only if they end as leaves.  Near the
+        # threshold: the FP64 fit decides (leaf, or searched with its FP64
+        # system in hand).  Stopped by depth or size: FP64 leaves.
+        g.c_recs = [mrec, nrec]
+        g.c_sel = np.where(clear, 0, 1)
+        g.c_gram, g.c_gram_t = near.copy(), g.grams
+        g.c_bad = np.zeros(K, dtype=np.int64) if nrec is None else np.where(near, nrec.bad, 0)
+        if frec is not None:
+            self._add_leaves(g, np.flatnonzero(final), frec, g.grams, frec.bad)
+        split = clear.copy()
+        if nrec is not None:
+            stop = near & (nrec.mse <= thr)
+            self._add_leaves(g, np.flatnonzero(stop), nrec, g.grams, nrec.bad)
+            split |= near & ~stop
         self._clock("decide", t)
-        return split_idx
+        return np.flatnonzero(split)
 
     def _fp64_launch(self, g, degree, p, sel):
         """Enqueue the FP64 systems and fits of the level's nodes sel (bool
@@ -971,20 +1009,21 @@ class ServingEngine:
         return _Fits(coef.cpu().numpy(), status.cpu().numpy(), sse.cpu().numpy(),
                      wsum.cpu().numpy(), g.bad.cpu().numpy())
 
-    def _late_leaves(self, g, nodes):
-        """Mixed precision: nodes split on mixed statistics that end as leaves
-        (failed search, one-sided partition) take their FP64 system and fit
-        from the rows still in their level buffer."""
-        todo = [n for n in nodes if n.gram_candidate is None]
-        if not todo:
+    def _late_leaves(self, g, idx):
+        """Mixed precision: entries idx split on mixed statistics that end as
+        leaves (failed search, one-sided partition) take their FP64 system
+        and fit from the rows still in their level buffer."""
+        todo = idx[~g.c_gram[idx]]
+        if not todo.size:
             return
-        sel = np.zeros(len(g.entries), dtype=bool)
-        sel[[n.index for n in todo]] = True
+        sel = np.zeros(g.sizes.size, dtype=bool)
+        sel[todo] = True
         degree, p = self._basis(g.d, g.degree)
         rec = self._fp64_fits(g, degree, p, sel)
-        for n in todo:
-            i = n.index
-            n.leaf_candidate, n.gram_candidate, n.bad = (rec, i), (g.grams, i), int(rec.bad[i])
+        g.c_recs.append(rec)
+        g.c_sel[todo] = len(g.c_recs) - 1
+        g.c_gram[todo] = True
+        g.c_bad = np.where(sel, rec.bad, g.c_bad)
 
     def _close_plan(self, g):
         if g.plan is not None:
@@ -1003,18 +1042,19 @@ class ServingEngine:
         if g.stream is None:
             g.stream = torch.cuda.Stream(device=self.device)
         g.stream.wait_stream(self.stream)
-        offsets = g.offsets
+        offsets, P, jid = g.offsets, g.params(), g.jid[idx]
         begin = offsets[idx]
-        end = offsets[np.asarray(idx) + 1]
+        end = offsets[idx + 1]
         counts = np.minimum(end - begin, cfg.split_rows)
-        temps = np.array([g.entries[i][0].req.temperature for i in idx])
+        temps = P["temp"][jid].astype(float)
         splitter = SoftObliqueSplitter(AffineRidgeModel(cfg.split_ridge), float(temps[0]),
                                        max_iters=cfg.max_iters, min_side_points=d + 2,
                                        min_side_fraction=cfg.min_side_fraction,
                                        n_restarts=cfg.n_restarts, random_state=0)
         R, count = cfg.n_restarts, len(idx)
-        keys = [start_key(g.entries[i][0].req.seed, g.entries[i][1].heap, r)
-                for r in range(R) for i in idx]
+        seeds, heaps = P["seed"][jid], g.heap[idx].astype(np.uint64)
+        keys = start_keys(np.tile(seeds, R), np.tile(heaps, R),
+                          np.repeat(np.arange(R, dtype=np.uint64), count))
         if g.staging is None:
             g.staging = _hip.PinnedRing(g.stream, 1 << 20)
         with torch.cuda.stream(g.stream):
@@ -1029,106 +1069,138 @@ class ServingEngine:
     def _apply_splits(self, g, idx, begin, end, counts, theta, ok, reason):
         from .batched_optimizer import STOP_REASONS
 
-        pending = []
-        ok = np.asarray(ok, dtype=bool).tolist()
-        if g.plan is not None:
-            self._late_leaves(g, [g.entries[i][1] for k, i in enumerate(idx) if not ok[k]])
+        ok = np.asarray(ok, dtype=bool)
+        fail = idx[~ok]
+        if fail.size:
+            if g.plan is not None:
+                self._late_leaves(g, fail)
+            self._candidate_leaves(g, fail, [STOP_REASONS[int(r)] for r in reason[~ok]])
+        if not ok.any():
+            g.pending = None
+            return
         # Each split's weights are a row of one copy of the batch's planes.
-        W = np.array(theta[:, :-1], dtype=np.float64)
-        Z, C = theta[:, -1].tolist(), np.asarray(counts).tolist()
-        B, E = np.asarray(begin).tolist(), np.asarray(end).tolist()
-        entries = g.entries
-        for k, i in enumerate(idx):
-            job, node = entries[i]
-            if not ok[k]:
-                node.leaf, node.gram = node.leaf_candidate, node.gram_candidate
-                node.err = STOP_REASONS[int(reason[k])]
-                node.leaf_candidate = node.gram_candidate = None
-                job.open -= 1
-                self._maybe_finish(job)
-                continue
-            wk, zk = W[k], Z[k]
-            node.split = (wk, zk, C[k])
-            pending.append((job, node, B[k], E[k], wk, zk))
-        g.pending = pending or None
+        theta = np.asarray(theta, dtype=np.float64)
+        g.pending = dict(idx=idx[ok], begin=np.asarray(begin)[ok], end=np.asarray(end)[ok],
+                         W=np.array(theta[ok, :-1]), Z=theta[ok, -1].copy(),
+                         counts=np.asarray(counts)[ok])
+
+    # ----------------------------------------------------------------- decisions
+
+    def _job_runs(self, g, idx):
+        """Entries idx grouped by job: (job, positions in idx) per job."""
+        jid = g.jid[idx]
+        order = np.argsort(jid, kind="stable")
+        js = jid[order]
+        cuts = np.flatnonzero(js[1:] != js[:-1]) + 1
+        starts = np.concatenate([[0], cuts]).tolist()
+        ends = np.concatenate([cuts, [js.size]]).tolist()
+        return [(g.jobs[int(js[a])], order[a:b]) for a, b in zip(starts, ends)]
+
+    def _add_leaves(self, g, idx, rec, gram_t, bad, errs=None):
+        """Entries idx end as leaves with fits rec rows idx, FP64 systems
+        gram_t rows idx and weight flags bad[idx] (errs: per entry, the split
+        search's stop reason or None); finishes every job left with no open
+        node."""
+        if not idx.size:
+            return
+        heap, depth, rows, flags = g.heap[idx], g.depth[idx], g.sizes[idx], bad[idx]
+        done = []
+        for job, pos in self._job_runs(g, idx):
+            e = None if errs is None else [errs[k] for k in pos.tolist()]
+            job.leaves.append((heap[pos], depth[pos], rows[pos], rec, idx[pos], gram_t, idx[pos],
+                               flags[pos], e))
+            job.open -= pos.size
+            if job.open == 0:
+                done.append(job)
+        for job in done:
+            g.jobs[job.slot] = None
+            t0 = time.perf_counter()
+            self._finish(job)
+            self.phase_seconds["assemble"] += time.perf_counter() - t0
+
+    def _candidate_leaves(self, g, idx, errs):
+        """Searched entries idx end as leaves on their own fits (the
+        candidate state of their level)."""
+        sel = g.c_sel[idx]
+        for s in np.unique(sel).tolist():
+            m = sel == s
+            self._add_leaves(g, idx[m], g.c_recs[s], g.c_gram_t, g.c_bad,
+                             None if errs is None else [e for e, k in zip(errs, m) if k])
+
+    def _add_splits(self, g, idx, W, Z, counts):
+        """Entries idx split (planes W, Z; search samples of counts rows):
+        each adds two open children and closes itself."""
+        if not idx.size:
+            return
+        heap, depth = g.heap[idx], g.depth[idx]
+        for job, pos in self._job_runs(g, idx):
+            job.splits.append((heap[pos], depth[pos], W[pos], Z[pos], counts[pos]))
+            job.open += pos.size
 
     # ----------------------------------------------------------------- completion
 
-    def _maybe_finish(self, job):
-        if job.open > 0:
-            return
-        t0 = time.perf_counter()
-        self._finish(job)
-        self.phase_seconds["assemble"] += time.perf_counter() - t0
-
     def _finish(self, job):
-        """Flatten a finished fit straight from its node records: splits and
+        """Flatten a finished fit from its leaf and split blocks: splits and
         leaves in heap order (the breadth-first order of ``FlatTree``), leaf
-        systems gathered from the level tensors that hold them."""
+        fits gathered from their records and leaf systems from the level
+        tensors that hold them."""
         from .tree import FlatTree
 
         torch, d, degree = self.torch, job.d, job.degree
         _, p = self._basis(d, degree)
-        nodes = job.nodes
-        split_h, leaf_h = [], []
-        for h in sorted(nodes):
-            (split_h if nodes[h].split is not None else leaf_h).append(h)
-        # Child link of every node: its split row, or -1 - its leaf index.
-        link = {h: i for i, h in enumerate(split_h)}
-        link.update((h, -1 - j) for j, h in enumerate(leaf_h))
-
-        S = max(len(split_h), 1)
-        w = np.zeros((S, d))
-        z = np.zeros(S)
-        child = np.zeros((S, 2), dtype=np.int32)
-        splits = []
-        for i, h in enumerate(split_h):
-            node = nodes[h]
-            sw, sz, n = node.split
-            w[i], z[i] = sw, sz
-            child[i, 0], child[i, 1] = link[2 * h], link[2 * h + 1]
-            splits.append((h, node.depth, sw, sz, n))
-        # Leaf fits and systems are rows of their launches' records and level
-        # tensors: gather them launch by launch.
-        L = len(leaf_h)
-        depth, rows, bads, errors = [0] * L, [0] * L, [0] * L, {}
-        by_rec, by_gram = {}, {}
-        for j, h in enumerate(leaf_h):
-            node = nodes[h]
-            rec, i = node.leaf
-            e = by_rec.get(id(rec))
-            if e is None:
-                e = by_rec[id(rec)] = (rec, [], [])
-            e[1].append(j)
-            e[2].append(i)
-            t, i = node.gram
-            e = by_gram.get(id(t))
-            if e is None:
-                e = by_gram[id(t)] = (t, [], [])
-            e[1].append(j)
-            e[2].append(i)
-            depth[j], rows[j], bads[j] = node.depth, node.n_rows, node.bad
-            if node.err is not None:
-                errors[j] = node.err
+        if job.splits:
+            sh, sd, W, Z, C = (np.concatenate(c) for c in zip(*job.splits))
+            o = np.argsort(sh, kind="stable")
+            sh, sd, W, Z, C = sh[o], sd[o], W[o], Z[o], C[o]
+        else:
+            sh = sd = C = np.zeros(0, dtype=np.int64)
+            W, Z = np.zeros((0, d)), np.zeros(0)
+        blocks = job.leaves
+        lh = np.concatenate([b[0] for b in blocks])
+        order = np.argsort(lh, kind="stable")   # heap order of the leaves
+        L = lh.size
+        at = np.empty(L, dtype=np.int64)        # leaf j of the blocks -> its heap-order slot
+        at[order] = np.arange(L)
         coef = np.empty((L, p))
         status = np.empty(L, dtype=np.int64)
         mse = np.empty(L)
-        for rec, js, rs in by_rec.values():
-            coef[js], status[js], mse[js] = rec.coef[rs], rec.status[rs], rec.mse[rs]
-        flat = FlatTree.from_tables(w, z, child, link[1], coef, d, (degree, True, False),
+        errors = {}
+        k = 0
+        for h, _, _, rec, rr, _, _, _, e in blocks:
+            dst = at[k:k + h.size]
+            coef[dst], status[dst], mse[dst] = rec.coef[rr], rec.status[rr], rec.mse[rr]
+            if e is not None:
+                errors.update((int(dst[i]), v) for i, v in enumerate(e) if v is not None)
+            k += h.size
+        lh = lh[order]
+        ldepth = np.concatenate([b[1] for b in blocks])[order]
+        lrows = np.concatenate([b[2] for b in blocks])[order]
+        bads = np.concatenate([b[7] for b in blocks])[order]
+        # Child link of every split: its children's split rows, or -1 - leaf index.
+        S = sh.size
+        ch = np.stack([2 * sh, 2 * sh + 1], axis=1)
+        ks = np.minimum(np.searchsorted(sh, ch), max(S - 1, 0))
+        is_split = (S > 0) & (sh[ks] == ch) if S else np.zeros(ch.shape, dtype=bool)
+        kl = np.searchsorted(lh, ch)
+        child = np.where(is_split, ks, -1 - kl).astype(np.int32)
+        root = 0 if S and sh[0] == 1 else -1
+        if not S:
+            W, Z, child = np.zeros((1, d)), np.zeros(1), np.zeros((1, 2), dtype=np.int32)
+        flat = FlatTree.from_tables(W, Z, child, root, coef, d, (degree, True, False),
                                     upload=self.staging.upload)
-        lists = list(by_gram.values())
-        grams = torch.empty((L, lists[0][0].shape[1]), dtype=torch.float64, device=self.device)
-        idx = self.staging.upload(np.array([j for lst in lists for j in lst[1]]
-                                   + [i for lst in lists for i in lst[2]] + bads,
-                                   dtype=np.int64), self.device)
-        at = 0
-        for t, js, _ in lists:
-            n = len(js)
-            grams.index_copy_(0, idx[at:at + n], t.index_select(0, idx[L + at:L + at + n]))
-            at += n
-        bad = idx[2 * L:].to(torch.int32)
-        leaves = (leaf_h, depth, status.copy(), mse, np.asarray(rows, dtype=np.int64), errors)
+        # Leaf systems: gathered block by block into their heap-order rows.
+        grams = torch.empty((L, blocks[0][5].shape[1]), dtype=torch.float64, device=self.device)
+        gidx = self.staging.upload(np.concatenate([at, np.concatenate([b[6] for b in blocks]),
+                                                   bads.astype(np.int64)]), self.device,
+                                   np.int64)
+        k = 0
+        for b in blocks:
+            n = b[0].size
+            grams.index_copy_(0, gidx[k:k + n], b[5].index_select(0, gidx[L + k:L + k + n]))
+            k += n
+        bad = gidx[2 * L:].to(torch.int32)
+        splits = (sh, sd, W[:S], Z[:S], C)
+        leaves = (lh, ldepth, status.copy(), mse, lrows, errors)
         res = _Resident(job.tree_id, flat, grams, bad, d, degree, p, self.config.ridge, splits,
                         leaves, job.n_rows)
         res.coef = coef
@@ -1142,7 +1214,7 @@ class ServingEngine:
             self._inflight_bytes -= job.est_bytes
             self._active -= 1
             self._fitting.discard(job.tree_id)
-        job.nodes.clear()
+        job.leaves, job.splits = [], []
         job.timing.finished = time.perf_counter()
         self.completed += 1
         job.future.set_result(FitResult(job.tree_id, coef.copy(), status, job.n_rows, job.timing,

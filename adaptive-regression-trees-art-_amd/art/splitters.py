@@ -50,6 +50,20 @@ def project_unit_w(theta):
     return out
 
 
+def project_unit_w_rows(thetas):
+    """project_unit_w of every row of thetas (n, d+1) at once."""
+    thetas = np.asarray(thetas, dtype=float)
+    if thetas.ndim != 2 or thetas.shape[1] < 2:
+        raise ValueError("theta must contain at least one w component and one z component.")
+    w = thetas[:, :-1]
+    n = np.sqrt(np.einsum("ij,ij->i", w, w))
+    if not np.all(n > 1e-12):
+        raise ValueError("Cannot project theta with near-zero w.")
+    out = thetas.copy()
+    out[:, :-1] = w / n[:, None]
+    return out
+
+
 def project_unit_w_gradient(theta, grad):
     """Tangent projection of the w-gradient (art/splitters.py:86-98)."""
     theta = project_unit_w(theta)
