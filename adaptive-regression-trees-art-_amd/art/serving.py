@@ -499,6 +499,7 @@ class ServingEngine:
         self._error = None
         self.steps = 0
         self.completed = 0
+        self.idle_periods = collections.deque(maxlen=100000)  # (start, end) with no work
         self.phase_seconds = collections.defaultdict(float)
         self._retired_plans = []       # leaf plans to close after the level's searches
         self._diag_cache = {}
@@ -578,8 +579,11 @@ class ServingEngine:
             with torch.cuda.stream(self.stream):
                 while True:
                     with self._cv:
-                        while not self._queue and not self._closing and not self._busy():
-                            self._cv.wait()
+                        if not self._queue and not self._closing and not self._busy():
+                            t0 = time.perf_counter()
+                            while not self._queue and not self._closing and not self._busy():
+                                self._cv.wait()
+                            self.idle_periods.append((t0, time.perf_counter()))
                         incoming = list(self._queue)
                         self._queue.clear()
                         if self._closing and not incoming and not self._busy():

@@ -39,11 +39,12 @@ def parse():
     ap.add_argument("--chunk-rows", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--only", choices=("all", "ridge", "ridge_kernels", "sampler", "split",
-                                       "tree", "serving"),
+                                       "tree", "serving", "serving_stream"),
                     default="all",
                     help="run one leg (profiling); the JSON line keeps the ridge metric; "
                          "ridge_kernels runs only --warmup + --steps fits of --precision "
-                         "(the counter and trace driver)")
+                         "(the counter and trace driver); serving_stream serves the "
+                         "--requests stream once at --precision (the serving trace driver)")
     ap.add_argument("--sampler-steps", type=int, default=3)
     ap.add_argument("--tree-steps", type=int, default=3)
     ap.add_argument("--polytopes", type=int, default=65536)
@@ -824,8 +825,12 @@ def bench_serving(args):
                 if (i + 1) % 100 == 0:
                     print(f"[bench] serving ({precision}): {i + 1} results at "
                           f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
-            makespan = time.perf_counter() - t0
-            out = {"res": res, "makespan": makespan, "steps": eng.steps,
+            t1 = time.perf_counter()
+            makespan = t1 - t0
+            # Engine time with no request to work on, inside [t0, t1]: the
+            # stream's work took makespan - idle seconds of the engine.
+            idle = sum(max(0.0, min(b, t1) - max(a, t0)) for a, b in list(eng.idle_periods))
+            out = {"res": res, "makespan": makespan, "steps": eng.steps, "idle": idle,
                    "phases": {k: round(v, 4) for k, v in eng.phase_seconds.items()}}
         lat = np.array([r.timing.latency for r in res])
         out["lat"] = lat
@@ -834,6 +839,16 @@ def bench_serving(args):
               file=sys.stderr, flush=True)
         return out
 
+    if args.only == "serving_stream":
+        # Trace driver: the stream once, continuously batched, at --precision.
+        out = serve(args.precision, "s")
+        lat = out["lat"]
+        return {"leg": "serving_stream", "precision": args.precision, "requests": len(events),
+                "makespan_s": out["makespan"], "value": len(events) / out["makespan"],
+                "unit": "requests/s", "engine_idle_s": out["idle"],
+                "latency_ms": {"p50": float(np.percentile(lat, 50)) * 1e3,
+                               "p99": float(np.percentile(lat, 99)) * 1e3},
+                "engine_levels": out["steps"], "engine_phase_s": out["phases"]}
     mixed = serve("mixed", "b")
     fp64 = serve("fp64", "f")
     res, makespan, lat = mixed["res"], mixed["makespan"], mixed["lat"]
@@ -872,7 +887,14 @@ def bench_serving(args):
         "metric": "served requests/s and p50/p99 request latency",
         "value": len(events) / makespan, "unit": "requests/s",
         "arrival_rate": args.rate,
-        "keeps_up_with_arrivals": bool(len(events) / makespan >= args.rate),
+        "keeps_up_with_arrivals": bool(len(events) / (makespan - mixed["idle"]) >= args.rate),
+        "service_capacity": {
+            "value": len(events) / (makespan - mixed["idle"]), "unit": "requests/s",
+            "engine_idle_s": mixed["idle"],
+            "definition": "requests / (makespan - engine idle time in the window); the engine "
+                          "keeps up when this reaches the arrival rate"},
+        "drain_after_last_arrival_s": makespan - events[-1].arrival,
+        "latency_ms_by_quarter": [float(np.mean(q)) * 1e3 for q in np.array_split(lat, 4)],
         "latency_ms": {"p50": float(np.percentile(lat, 50)) * 1e3,
                        "p99": float(np.percentile(lat, 99)) * 1e3,
                        "max": float(lat.max()) * 1e3},
@@ -916,7 +938,7 @@ def main():
         sys.exit(2)
     sys.path.insert(0, str(ROOT))
     background = None
-    if not args.no_cpu and args.only != "ridge_kernels":
+    if not args.no_cpu and args.only not in ("ridge_kernels", "serving_stream"):
         results, background = run_cpu_baselines(args)
         _CPU_RESULTS.update(results)
     try:
@@ -977,7 +999,7 @@ def run_legs(args):
     if args.only == "tree":
         _LINE["line"] = bench_tree(args)
         return
-    if args.only == "serving":
+    if args.only in ("serving", "serving_stream"):
         _LINE["line"] = bench_serving(args)
         return
     wl = _CPU.get("ridge_workload") or leaf_ridge_workload(n_leaves=args.leaves, seed=2)
