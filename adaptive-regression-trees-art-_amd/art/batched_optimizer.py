@@ -229,11 +229,12 @@ class BatchedOptimizerWorkspace:
         return out.cpu().numpy()
 
     def _launch(self, nodes, theta0, mode, fallback, hard_refit, anneal=None, timing=None,
-                batched=False, child_precision=_hip.PRECISION_FP64):
+                batched=False, child_precision=_hip.PRECISION_FP64, lean=False):
         """One device launch over problems (nodes[i], theta0[i]); host copies out.
 
         anneal = (iters, t0, t1, step) for mode 1; timing, a list, receives
-        the launch's device milliseconds."""
+        the launch's device milliseconds.  lean: only theta, the hard-refit
+        results and info come back (beta, histories and scalars are None)."""
         torch, s, P = self.torch, self.s, len(nodes)
         mi = int(s.max_iters) if anneal is None else int(anneal[0])
         dev = dict(device=self.device)
@@ -244,7 +245,10 @@ class BatchedOptimizerWorkspace:
         out_beta = torch.empty((P, 2, self.p), **f64)
         hard = torch.zeros((P, 8), **f64)
         info = torch.zeros((P, 8), dtype=torch.int32, **dev)
-        hist = torch.full((P, 5, mi + 1), float("nan"), **f64)
+        if lean:  # written by the kernel, never read back
+            hist = torch.empty((P, 5, mi + 1), **f64)
+        else:
+            hist = torch.full((P, 5, mi + 1), float("nan"), **f64)
         scal = torch.zeros((P, SCALARS), **f64)
         m = s.model_template
         a = _hip.SplitArgs()
@@ -282,6 +286,9 @@ class BatchedOptimizerWorkspace:
         self.stream.synchronize()
         if timing is not None:
             timing.append(ev[0].elapsed_time(ev[1]))
+        if lean:
+            return out_theta.cpu().numpy(), None, hard.cpu().numpy(), info.cpu().numpy(), None, \
+                None
         return (out_theta.cpu().numpy(), out_beta.cpu().numpy(), hard.cpu().numpy(),
                 info.cpu().numpy(), hist.cpu().numpy(), scal.cpu().numpy())
 
@@ -391,7 +398,7 @@ class BatchedOptimizerWorkspace:
         nodes = np.tile(self.all_ids[:count], R)
         thetas = project_unit_w_rows(starts.reshape(-1, self.d + 1))
         theta, _, hard, info, _, _ = self._launch(nodes, thetas, 0, self.s.min_side_fallback,
-                                                  True, timing=timing)
+                                                  True, timing=timing, lean=True)
         if R == 1:  # one start per node: its split, or its stop reason
             ok = info[:count, 6] == 0
             out = np.where(ok[:, None], theta[:count], 0.0)

@@ -305,7 +305,8 @@ class _Fits:
 
     def __init__(self, coef, status, sse, wsum, bad=None):
         self.coef, self.status, self.bad = coef, status, bad
-        with np.errstate(divide="ignore", invalid="ignore"):
+        # Rows of nodes a launch did not form hold unread values: no warnings.
+        with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
             self.mse = np.where(wsum > 0, sse / wsum, np.inf)
 
 
@@ -934,7 +935,9 @@ class ServingEngine:
         g.plan = LeafRidgeBatch(g.offsets, d, degree=degree, stream=self.stream)
         size = self.lib.art_gram_size(p)
         f64 = dict(dtype=torch.float64, device=self.device)
-        g.grams = torch.zeros((K, size), **f64)
+        # Rows are formed for the nodes that need them (masked launches); the
+        # others are never read.
+        g.grams = torch.empty((K, size), **f64)
         g.bad = torch.zeros(K, dtype=torch.int32, device=self.device)
         clear = np.zeros(K, dtype=bool)
         mrec = None
@@ -942,7 +945,7 @@ class ServingEngine:
         # the same stream, so that both passes complete under one synchronization.
         early = self._fp64_launch(g, degree, p, final) if final.any() else None
         if cand.any():
-            mg = torch.zeros((K, size), **f64)
+            mg = torch.empty((K, size), **f64)
             mb = torch.zeros(K, dtype=torch.int32, device=self.device)
             mask = self.staging.upload(cand, self.device, np.int32)
             g.plan.grams(g.X, g.y, g.w, cfg.weight_floor, precision="mixed", mask=mask,
