@@ -926,6 +926,9 @@ def bench_serving(args):
     }
 
 
+TUNE_CANDIDATES = (2048, 4096, 8192, 16384)
+
+
 def rel_err(a, b):
     return np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-300)
 
@@ -1007,15 +1010,32 @@ def run_legs(args):
     X = torch.from_numpy(wl.X).to(dev)
     y = torch.from_numpy(wl.y).to(dev)
     w = torch.from_numpy(wl.w).to(dev)
-    batch = LeafRidgeBatch(wl.offsets, 8, degree=2, chunk_rows=args.chunk_rows)
+    from art import autotune
+
+    # Chunk sizes: each form at the size the autotuner measures fastest on this
+    # workload (before any timing), unless --chunk-rows pins one.
+    precisions = ["fp64"] + ([args.precision] if args.precision != "fp64" else [])
+    chunk, tuned = {}, {}
+    for prec in precisions:
+        if args.chunk_rows:
+            chunk[prec], tuned[prec] = args.chunk_rows, None
+        else:
+            chunk[prec], tuned[prec] = autotune.tune_chunk_rows(
+                wl.offsets, 8, X, y, w, wl.ridge, precision=prec, force=True,
+                candidates=TUNE_CANDIDATES, reps=5)
+        print(f"[bench] {prec} chunk rows {chunk[prec]} ({tuned[prec]})", file=sys.stderr,
+              flush=True)
+    plain_batch = LeafRidgeBatch(wl.offsets, 8, degree=2, chunk_rows=chunk["fp64"])
+    batch = plain_batch if args.precision == "fp64" else \
+        LeafRidgeBatch(wl.offsets, 8, degree=2, chunk_rows=chunk[args.precision])
     p, N, K = batch.p, wl.n_rows, wl.n_leaves
     lib = batch.lib
     print(f"[bench] leaves={K} rows={N} p={p} chunks={batch.n_chunks}", file=sys.stderr, flush=True)
 
     # FP64 plain form: the accuracy reference and the comparison point.
-    plain = timed_fits(batch, lib, X, y, w, wl.ridge, "fp64", args.steps, args.warmup)
-    ref = batch.coefficients.cpu().numpy()
-    ref_status = batch.status.cpu().numpy()
+    plain = timed_fits(plain_batch, lib, X, y, w, wl.ridge, "fp64", args.steps, args.warmup)
+    ref = plain_batch.coefficients.cpu().numpy()
+    ref_status = plain_batch.status.cpu().numpy()
     print(f"[bench] fp64 plain {plain['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     run = plain
     if args.precision == "mixed":
@@ -1026,12 +1046,6 @@ def run_legs(args):
     graphed = graph_replays(batch, X, y, w, wl.ridge, args.precision, args.steps, coef)
     print(f"[bench] graph replay {graphed['ms_per_step']:.3f} ms/step, equal to eager: "
           f"{graphed['equal_to_eager_bitwise']}", file=sys.stderr, flush=True)
-    from art import autotune
-
-    tuned_chunk, tuned_ms = autotune.tune_chunk_rows(
-        wl.offsets, 8, X, y, w, wl.ridge, precision=args.precision, force=True)
-    print(f"[bench] autotuned chunk rows {tuned_chunk} ({tuned_ms})", file=sys.stderr,
-          flush=True)
     ok = (st == 0) & (ref_status == 0)
     err = rel_err(coef[ok], ref[ok])
     fallback = batch.fallback_count() if args.precision == "mixed" else 0
@@ -1075,7 +1089,7 @@ def run_legs(args):
             "workload": ("batched weighted ridge: 4096 leaves, d=8, quadratic p=45, "
                          "log-uniform 64..16384 rows, ridge 1e-6, seed 2"),
             "leaves": K, "rows": N, "p": p, "precision": args.precision,
-            "chunk_rows": args.chunk_rows or leaf_ridge.DEFAULT_CHUNK_ROWS, "chunks": batch.n_chunks,
+            "chunk_rows": chunk, "chunks": batch.n_chunks,
         },
         "latency_ms": {"p50": float(np.percentile(run["step_ms"], 50)),
                        "p99": float(np.percentile(run["step_ms"], 99)),
@@ -1090,8 +1104,12 @@ def run_legs(args):
         "kernels_ms": {"first": kernel_ms, "rest": run["rest_ms"]},
         "failed_leaves": int((st != 0).sum()),
         "graph_replay": graphed,
-        "autotune": {"chunk_rows_ms": {str(k): v for k, v in tuned_ms.items()},
-                     "chosen_chunk_rows": tuned_chunk,
+        "autotune": {"chunk_rows_ms": {prec: (None if ms is None else
+                                              {str(k): v for k, v in ms.items()})
+                                       for prec, ms in tuned.items()},
+                     "chosen_chunk_rows": chunk,
+                     "note": "each form runs at its own fastest chunk size (median of 5 fits "
+                             "per candidate), chosen before timing",
                      "key": "leaf_fit: d, p, basis, precision, K and N power-of-two buckets"},
         "roofline": roofline,
     }
@@ -1107,6 +1125,7 @@ def run_legs(args):
     if args.copy_probe:
         copy_probe(torch)
     batch.close()
+    plain_batch.close()
     del X, y, w
     torch.cuda.empty_cache()
     result["measured_peaks"] = peak_probe(torch)
