@@ -235,6 +235,66 @@ __device__ inline void block_sums(const double (&v)[K], SplitShared<T>& S, int c
 // Gram of the non-constant features (lane (col, kq) feeds feature 16t + col
 // of row 4s + kq), and the same lanes accumulate the feature sums against
 // q and q y (the constant and y columns of the augmented system).
+// Child systems up to kWaveSolveP wide are solved by one wave each.
+constexpr int kWaveSolveP = 16;
+
+// Trace-scaled ridge, Cholesky and solve of one packed child system M
+// ((p+1) x (p+1): G and the row h) by one wave, rows in registers; the same
+// status contract as the workgroup form below (-1 bad gates, -2 non-finite,
+// k+1 first pivot <= 0, -3 non-finite solution).  col: 64 doubles of LDS for
+// this wave.  Writes beta[0..p) and lam; returns the status (uniform).
+__device__ inline int wave_side_solve(double* M, int p, double ridge, int bad, int lane,
+                                      double* col, double* beta, double& lam_out) {
+  constexpr int PMAX = kWaveSolveP;
+  const double tr = wave_sum(lane < p ? M[tri(lane, lane)] : 0.0);
+  const double lam = ridge == 0.0 ? 0.0 : ridge * (tr / p);
+  int st = bad ? -1 : ((!isfinite(tr) || !isfinite(lam)) ? -2 : 0);
+  double x = 0.0;
+  if (st == 0) {
+    // Each loop tests the lane against every column; a fresh opaque copy of
+    // the lane index per loop keeps those tests out of scalar registers.
+    double a[PMAX];
+    {
+      const int me = opaque_lane(lane);
+#pragma unroll
+      for (int j = 0; j < PMAX; ++j) a[j] = (j <= me && me < p) ? M[tri(me, j)] : 0.0;
+    }
+    const double h = lane < p ? M[tri(p, lane)] : 0.0;
+#pragma unroll
+    for (int j = 0; j < PMAX; ++j)
+      if (j == lane) a[j] += lam;
+    double diag;
+    st = wave_cholesky<PMAX>(a, p, lane, diag, col);
+    if (st == 0) {
+      const int me = opaque_lane(lane);
+      int nf = me < p && !isfinite(h);
+#pragma unroll
+      for (int j = 0; j < PMAX; ++j) nf |= (j <= me && me < p && !isfinite(a[j]));
+      if (wave_or(nf)) st = -2;
+    }
+    if (st == 0) {
+      const double dinv = 1.0 / diag;
+      const double z = wave_forward<PMAX>(a, dinv, p, lane, h);
+      // The factor's rows go back to M, from which every lane reads its
+      // column for the back substitution (one wave: its LDS operations
+      // complete in order; the fence keeps the reads after the writes).
+      const int me = opaque_lane(lane);
+#pragma unroll
+      for (int j = 0; j < PMAX; ++j)
+        if (j <= me && me < p) M[tri(me, j)] = a[j];
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      const int me2 = opaque_lane(lane);
+#pragma unroll
+      for (int k = 0; k < PMAX; ++k) a[k] = (k > me2 && k < p) ? M[tri(k, me2)] : 0.0;
+      x = wave_backward<PMAX>(a, dinv, p, lane, z);
+      if (wave_or(lane < p && !isfinite(x))) st = -3;
+    }
+  }
+  if (lane < p) beta[lane] = st == 0 ? x : __builtin_nan("");
+  if (lane == 0) lam_out = lam;
+  return st;
+}
+
 template <int T>
 __device__ void fit_sides(const SplitCtx& C, SplitShared<T>& S, bool hard, double floor,
                           bool half = false) {
@@ -403,6 +463,17 @@ __device__ void fit_sides(const SplitCtx& C, SplitShared<T>& S, bool hard, doubl
   }
   __syncthreads();
   bad = block_or(bad, S.ired);
+  if (p <= kWaveSolveP) {
+    // Small child systems: wave `side` factors and solves side `side` in
+    // registers (no barrier per column), both sides at once.
+    if (wave < 2) {
+      const int side = wave;
+      S.status[side] = wave_side_solve(S.M[side], p, C.ridge, bad, lane,
+                                       S.red + 64 * side, S.beta[side], S.lam[side]);
+    }
+    __syncthreads();
+    return;
+  }
   for (int side = 0; side < 2; ++side) {
     double* M = S.M[side];
     const double tr = block_sum(tid < p ? M[tri(tid, tid)] : 0.0, S.red);
