@@ -17,6 +17,7 @@ LIB_DIR = Path(__file__).resolve().with_name("_lib")
 LIB_PATH = LIB_DIR / "libart_hip.so"
 
 ART_OK = 0
+ERR_ARG, ERR_HIP = -1, -2
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 PRECISION_FP64_REFINED = 2
@@ -97,6 +98,9 @@ _PROTOTYPES = {
     "art_plan_timing_read": (_i, [_p, _p, _p, _i]),
     "art_plan_destroy": (None, [_p]),
     "art_split_optimize": (_i, [ctypes.POINTER(SplitArgs), _i, _p]),
+    "art_split_optimize_parts": (_i, [ctypes.POINTER(SplitArgs), _i, _i, _p, _p, _p, _p]),
+    "art_split_part_doubles": (_i, [_i, _i]),
+    "art_split_max_parts": (_i, [ctypes.POINTER(SplitArgs), _i]),
     "art_split_anneal_batched": (_i, [ctypes.POINTER(SplitArgs), _i, _p]),
     "art_split_init": (_i, [_p, _i, _i, _p, _i, ctypes.c_uint64, ctypes.c_uint64, _p, _p]),
     "art_split_init_ranges": (_i, [_p, _p, _i, _p, _i, ctypes.c_uint64, ctypes.c_uint64, _p,

@@ -245,6 +245,17 @@ __device__ inline void split3_pair(float a, float b, const NegPairs& c, unsigned
 
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
+// Device (agent) scope store and load of a double: global_store /
+// global_load with the device-scope cache policy, so that a workgroup on
+// another XCD reads the value once the store has completed (each XCD has its
+// own L2; no L2 write-back or invalidate is needed).
+__device__ inline void store_dev(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline double load_dev(const double* p) {
+  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Lane `src`'s value of v (src uniform).
 __device__ inline double lane_value(double v, int src) {
   const long long b = __double_as_longlong(v);

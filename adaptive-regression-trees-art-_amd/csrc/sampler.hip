@@ -310,6 +310,7 @@ __device__ __attribute__((always_inline)) inline void chain_run(const art_har_ar
   }
   int until_record = a.first, recorded = 0;
   constexpr int MR = M > 0 ? M : 1;
+  constexpr int MR0 = M > 0 ? M : 0;
   double Ar[MR][DM], br[MR];
   if constexpr (M > 0) {
 #pragma unroll
@@ -320,19 +321,24 @@ __device__ __attribute__((always_inline)) inline void chain_run(const art_har_ar
     }
   }
 
+  // Doubles per move entry: the launch's bound E for the per-move row
+  // reads; with M rows in registers, this polytope's own 4 DM + 1 + 2 M, a
+  // compile-time stride (the workgroup's entries are its own).
+  const int ES = M >= 0 ? 4 * DM + 1 + 2 * MR0 : E;
   for (int s0 = 0; s0 < moves; s0 += kChainBlock) {
     const int cnt = min(kChainBlock, moves - s0);
     if (lane < cnt) {
       double dir[DM], u;
       move_randoms<DM, EXACT>(a, shape, d, W, poly, stream_poly, 0, s0 + lane, k0, k1, dir, u);
-      double* e = chain_lds + (size_t)lane * E;
+      double* e = chain_lds + (size_t)lane * ES;
       // Per coordinate: the box face the chord leaves through on each side
       // is fixed by the direction's sign, so the move stores it (or +-inf
-      // with a unit scale when the coordinate does not move).
+      // with a unit scale when the coordinate does not move or there is no
+      // box: the replay's (face - x) * scale is then -inf / +inf).
 #pragma unroll
       for (int j = 0; j < DM; ++j)
         if (EXACT || j < d) {
-          const bool nz = dir[j] != 0.0, up = dir[j] > 0.0;
+          const bool nz = box_lo && dir[j] != 0.0, up = dir[j] > 0.0;
           e[j] = dir[j];
           e[DM + j] = nz ? rcp2(dir[j]) : 1.0;
           e[2 * DM + 1 + j] = nz ? (up ? blo[j] : bhi[j]) : -INFINITY;
@@ -366,7 +372,7 @@ __device__ __attribute__((always_inline)) inline void chain_run(const art_har_ar
         double dir[DM], r[DM], flo[DM], fhi[DM], u, rlo[MR], rhi[MR];
       };
       auto read = [&](int k, Entry& q) {
-        const double* e = chain_lds + (size_t)k * E;
+        const double* e = chain_lds + (size_t)k * ES;
 #pragma unroll
         for (int j = 0; j < DM; ++j) {
           const bool on = EXACT || j < d;
@@ -382,15 +388,11 @@ __device__ __attribute__((always_inline)) inline void chain_run(const art_har_ar
           q.rhi[i] = e[4 * DM + 1 + M + i];
         }
       };
-      Entry cur;
-      read(0, cur);
-      for (int k = 0; k < cnt; ++k) {
-        Entry nxt;
-        read(k + 1 < cnt ? k + 1 : k, nxt);
+      auto step = [&](const Entry& cur) __attribute__((always_inline)) {
         double lo[NC], hi[NC];
 #pragma unroll
         for (int j = 0; j < DM; ++j) {
-          if (box_lo && (EXACT || j < d)) {
+          if (EXACT || j < d) {
             lo[j] = (cur.flo[j] - x[j]) * cur.r[j];
             hi[j] = (cur.fhi[j] - x[j]) * cur.r[j];
           } else {
@@ -433,8 +435,19 @@ __device__ __attribute__((always_inline)) inline void chain_run(const art_har_ar
           ++recorded;
           until_record = a.stride;
         }
-        cur = nxt;
+      };
+      // Two entries in turn (no copy of a loop-carried entry per move): the
+      // next move's entry loads while the current move runs.
+      Entry e0, e1;
+      read(0, e0);
+      int k = 0;
+      for (; k + 2 <= cnt; k += 2) {
+        read(k + 1, e1);
+        step(e0);
+        read(k + 2 < cnt ? k + 2 : k + 1, e0);
+        step(e1);
       }
+      if (k < cnt) step(e0);
       __syncthreads();
       continue;
     }

@@ -25,6 +25,7 @@
 #include "art_common.h"
 #include "art_hip.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <new>
@@ -103,6 +104,7 @@ struct SplitShared {
   signed char core[kSplitMaxP];  // basis column of non-constant feature k
   int nf;                        // number of non-constant features
   int cbias;                     // basis column of the constant, or -1
+  int part_dead;                 // a multi-part node's exchange gave up waiting
 };
 
 struct SplitCtx {
@@ -110,10 +112,33 @@ struct SplitCtx {
   const double* y;
   int n, d, p;
   int rows, sd;  // chunk rows and staged row stride
+  int lo, hi;    // this workgroup's rows of the node: [lo, hi)
   Basis B;
   double T;
   double ridge;
 };
+
+// A node's rows split over `parts` workgroups (part 0..parts-1, consecutive
+// block indices).  Every pass's row sums are exchanged through `buf` (two
+// alternating slots of `stride` doubles per part and problem, stored and
+// read at device scope so that workgroups on other XCDs see them) after a
+// node barrier on `count[prob]`; each part then adds the parts' sums in part
+// order, so every part holds bitwise the same node state and takes the same
+// control path.  The barrier's wait is bounded: a part that waits longer
+// than kPartSpins polls marks the launch failed (`failed`), stops waiting
+// at later exchanges and finishes with meaningless results.
+struct PartSync {
+  int parts, part, prob, stride;
+  int* count;
+  double* buf;
+  int* failed;
+  int epoch, dead;
+};
+constexpr long kPartSpins = 1L << 22;
+constexpr int kSplitMaxParts = 8;
+static_assert(ChunkShape<1>::table >= 2 * ((kSplitMaxP + 1) * (kSplitMaxP + 2) / 2) + 1 &&
+                  ChunkShape<2>::table >= 2 * ((kSplitMaxP + 1) * (kSplitMaxP + 2) / 2) + 1,
+              "the row table stages a node's two exchanged systems");
 
 // Feature tables of the basis (once per kernel, before any pass).
 template <int T>
@@ -177,19 +202,20 @@ __device__ inline void store_rows(const SplitCtx& C, SplitShared<T>& S, int rows
   if (tid < rows) S.xs[tid * C.sd + d] = v.y;
 }
 
-// Iterates the node's rows in staged chunks: body(rows) runs with chunk rows
-// [base, base + rows) in S.xs while the next chunk's loads are in flight.
+// Iterates this workgroup's rows of the node [C.lo, C.hi) in staged chunks:
+// body(rows) runs with chunk rows [base, base + rows) in S.xs while the next
+// chunk's loads are in flight.
 template <int T, class Body>
 __device__ inline void for_each_chunk(const SplitCtx& C, SplitShared<T>& S, Body body) {
   const int kChunk = C.rows;
   Staged<T> v;
-  load_rows(C, 0, min(kChunk, C.n), v);
-  for (int base = 0; base < C.n; base += kChunk) {
-    const int rows = min(kChunk, C.n - base);
+  load_rows(C, C.lo, min(kChunk, C.hi - C.lo), v);
+  for (int base = C.lo; base < C.hi; base += kChunk) {
+    const int rows = min(kChunk, C.hi - base);
     __syncthreads();  // the previous chunk is consumed
     store_rows(C, S, rows, v);
     const int next = base + kChunk;
-    if (next < C.n) load_rows(C, next, min(kChunk, C.n - next), v);
+    if (next < C.hi) load_rows(C, next, min(kChunk, C.hi - next), v);
     __syncthreads();
     body(rows);
   }
@@ -220,6 +246,47 @@ __device__ inline void block_sums(const double (&v)[K], SplitShared<T>& S, int c
 #pragma unroll
     for (int w = 1; w < kWavesT<T>; ++w) t += S.dred[64 * w + threadIdx.x];
     S.tot[threadIdx.x] = t;
+  }
+  __syncthreads();
+}
+
+// Node-wide sums of an LDS vector v[0..len) over the problem's parts; every
+// part ends with the same sums.  All threads of every part call this at the
+// same point of the control flow.
+template <int T>
+__device__ inline void parts_sum(PartSync& P, SplitShared<T>& S, double* v, int len) {
+  if (P.parts == 1) return;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const size_t slot = ((size_t)P.prob * 2 + (P.epoch & 1)) * P.parts;
+  double* mine = P.buf + (slot + P.part) * P.stride;
+  __syncthreads();  // v is complete
+  for (int k = tid; k < len; k += nt) store_dev(&mine[k], v[k]);
+  // Every thread's stores complete before the barrier, so before the count.
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (tid == 0) {
+    int* cnt = P.count + P.prob;
+    __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int target = P.parts * (P.epoch + 1);
+    long spins = 0;
+    while (!P.dead && __hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(4);
+      if (++spins > kPartSpins) {
+        P.dead = 1;
+        __hip_atomic_store(P.failed, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    S.part_dead = P.dead;
+  }
+  __syncthreads();
+  P.dead = S.part_dead;
+  ++P.epoch;
+  const double* all = P.buf + slot * P.stride;
+  for (int k = tid; k < len; k += nt) {
+    double s = 0.0;
+    for (int w = 0; w < P.parts; ++w) s += load_dev(&all[(size_t)w * P.stride + k]);
+    v[k] = s;
   }
   __syncthreads();
 }
@@ -296,8 +363,8 @@ __device__ inline int wave_side_solve(double* M, int p, double ridge, int bad, i
 }
 
 template <int T>
-__device__ void fit_sides(const SplitCtx& C, SplitShared<T>& S, bool hard, double floor,
-                          bool half = false) {
+__device__ void fit_sides(const SplitCtx& C, SplitShared<T>& S, PartSync& P, bool hard,
+                          double floor, bool half = false) {
   constexpr int NP = T * (T + 1) / 2;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int col = lane & 15, kq = lane >> 4;
@@ -463,6 +530,23 @@ __device__ void fit_sides(const SplitCtx& C, SplitShared<T>& S, bool hard, doubl
   }
   __syncthreads();
   bad = block_or(bad, S.ired);
+  if (P.parts > 1) {
+    // The node's systems from every part's rows, staged contiguously in the
+    // row table (free between passes; at least 2 nel + 1 doubles).
+    double* v = S.xs;
+    for (int k = tid; k < nel; k += blockDim.x) {
+      v[k] = S.M[0][k];
+      v[nel + k] = S.M[1][k];
+    }
+    if (tid == 0) v[2 * nel] = bad;
+    parts_sum(P, S, v, 2 * nel + 1);
+    for (int k = tid; k < nel; k += blockDim.x) {
+      S.M[0][k] = v[k];
+      S.M[1][k] = v[nel + k];
+    }
+    bad = v[2 * nel] != 0.0;
+    __syncthreads();
+  }
   if (p <= kWaveSolveP) {
     // Small child systems: wave `side` factors and solves side `side` in
     // registers (no barrier per column), both sides at once.
@@ -526,7 +610,7 @@ __device__ inline void side_predictions(const SplitShared<T>& S, const double* x
 // The gradient X^T c is formed per chunk from the per-row terms c in LDS,
 // a (column, row group) thread grid over the staged rows.
 template <int T>
-__device__ void stats_and_direction(const SplitCtx& C, SplitShared<T>& S) {
+__device__ void stats_and_direction(const SplitCtx& C, SplitShared<T>& S, PartSync& P) {
   const int tid = threadIdx.x;
   const int d = C.d, p = C.p, n = C.n, sd = C.sd;
   const double z = S.theta[d];
@@ -560,7 +644,6 @@ __device__ void stats_and_direction(const SplitCtx& C, SplitShared<T>& S) {
       for (int r = gg; r < rows; r += groups) g = fma(S.qs[0][r], S.xs[r * sd + gj], g);
   });
   block_sums<3>(sums, S);
-  const double loss = S.tot[0] / n, gz = S.tot[1], cnt = S.tot[2];
   __syncthreads();
   S.red[tid] = g;
   __syncthreads();
@@ -570,6 +653,8 @@ __device__ void stats_and_direction(const SplitCtx& C, SplitShared<T>& S) {
     S.tot[3 + tid] = s;
   }
   __syncthreads();
+  parts_sum(P, S, S.tot, 3 + d);
+  const double loss = S.tot[0] / n, gz = S.tot[1], cnt = S.tot[2];
   if (tid == 0) {
     const double* w = S.theta;
     const double* gw = S.tot + 3;
@@ -610,8 +695,8 @@ __device__ void stats_and_direction(const SplitCtx& C, SplitShared<T>& S) {
 
 // Frozen-model trial losses of `count` (<= kSplitMaxH) candidates alphas[0..count).
 template <int T>
-__device__ void trial_losses(const SplitCtx& C, SplitShared<T>& S, const double* alphas, int count,
-                             double* out) {
+__device__ void trial_losses(const SplitCtx& C, SplitShared<T>& S, PartSync& P,
+                             const double* alphas, int count, double* out) {
   const int tid = threadIdx.x;
   const int d = C.d, p = C.p, n = C.n, sd = C.sd;
   const double z = S.theta[d], temp = C.T;
@@ -654,6 +739,7 @@ __device__ void trial_losses(const SplitCtx& C, SplitShared<T>& S, const double*
     }
   });
   block_sums<kSplitMaxH>(f, S, count);
+  parts_sum(P, S, S.tot, count);
   if (tid < count) out[tid] = S.tot[tid] / n;
   __syncthreads();
 }
@@ -661,8 +747,8 @@ __device__ void trial_losses(const SplitCtx& C, SplitShared<T>& S, const double*
 // Hard partition statistics after a hard fit (art/_cuda_batched_optimizer.cu:92-120):
 // [MSE, MSE_L, MSE_R, n_right, lambda_L, lambda_R, bad].
 template <int T>
-__device__ void hard_stats(const SplitCtx& C, SplitShared<T>& S, bool refit, double* out,
-                           bool all_right = false) {
+__device__ void hard_stats(const SplitCtx& C, SplitShared<T>& S, PartSync& P, bool refit,
+                           double* out, bool all_right = false) {
   const int tid = threadIdx.x;
   const int d = C.d, p = C.p, n = C.n, sd = C.sd;
   const double z = S.theta[d];
@@ -689,6 +775,11 @@ __device__ void hard_stats(const SplitCtx& C, SplitShared<T>& S, bool refit, dou
   });
   block_sums<3>(sums, S);
   bad = block_or(bad, S.ired);
+  if (P.parts > 1) {
+    if (tid == 0) S.tot[3] = bad;
+    parts_sum(P, S, S.tot, 4);
+    bad = S.tot[3] != 0.0;
+  }
   if (tid == 0) {
     const double sl = S.tot[0], sr = S.tot[1], cnt = S.tot[2];
     out[3] = cnt;
@@ -713,12 +804,17 @@ __device__ inline double clip_alpha(const art_split_args* a, double v) {
 }
 
 template <int T>
-__global__ __launch_bounds__(kThreadsT<T>) void split_optimize_kernel(art_split_args args) {
+__global__ __launch_bounds__(kThreadsT<T>) void split_optimize_kernel(art_split_args args,
+                                                                      PartSync P) {
   __shared__ SplitShared<T> S;
   __shared__ double alphas[kSplitMaxH];
   __shared__ double losses[kSplitMaxH];
   __shared__ int sflag[8];
-  const int prob = blockIdx.x;
+  const int prob = blockIdx.x / P.parts;
+  P.part = blockIdx.x - prob * P.parts;
+  P.prob = prob;
+  P.epoch = 0;
+  P.dead = 0;
   const int tid = threadIdx.x;
   const art_split_args* a = &args;
   const int node = a->node[prob];
@@ -736,6 +832,12 @@ __global__ __launch_bounds__(kThreadsT<T>) void split_optimize_kernel(art_split_
   C.ridge = a->ridge;
   C.sd = row_stride(d);
   C.rows = chunk_rows<T>(d);
+  {  // this part's rows: whole chunks, as even a share as they allow
+    const int chunks = (n + C.rows - 1) / C.rows;
+    const int per = (chunks + P.parts - 1) / P.parts;
+    C.lo = min(n, P.part * per * C.rows);
+    C.hi = min(n, C.lo + per * C.rows);
+  }
   const int p = C.p;
   const int MI = a->max_iters;
   double* hist = a->history + (size_t)prob * 5 * (MI + 1);
@@ -754,9 +856,9 @@ __global__ __launch_bounds__(kThreadsT<T>) void split_optimize_kernel(art_split_
     // Parent fit (art/_cuda_batched_optimizer.py:403-411): pi = 1/2 on both
     // sides is the unweighted fit (the factor cancels in G, h and the
     // trace-scaled ridge); its MSE over all rows is the parent loss.
-    fit_sides<T>(C, S, false, 0.0, true);
+    fit_sides<T>(C, S, P, false, 0.0, true);
     double* hard = a->hard + (size_t)prob * 8;
-    hard_stats(C, S, true, hard, true);
+    hard_stats(C, S, P, true, hard, true);
     if (tid < p) {
       a->beta_out[((size_t)prob * 2 + 0) * p + tid] = S.beta[1][tid];
       a->beta_out[((size_t)prob * 2 + 1) * p + tid] = S.beta[1][tid];
@@ -783,8 +885,8 @@ __global__ __launch_bounds__(kThreadsT<T>) void split_optimize_kernel(art_split_
     // Annealed fixed-step projected gradient: T_k geometric from t0 to t1.
     for (int k = 0; k < a->anneal_iters; ++k) {
       C.T = a->t0 * pow(a->t1 / a->t0, (double)k / (double)max(a->anneal_iters - 1, 1));
-      fit_sides<T>(C, S, false, a->floor);
-      stats_and_direction(C, S);
+      fit_sides<T>(C, S, P, false, a->floor);
+      stats_and_direction(C, S, P);
       if (tid == 0 && k <= MI) hist[k] = S.packet[0];
       if (S.packet[6] != 0.0) {
         error = kCholeskyFailure;
@@ -801,8 +903,8 @@ __global__ __launch_bounds__(kThreadsT<T>) void split_optimize_kernel(art_split_
     }
     C.T = a->temperature[node];
   } else {
-    fit_sides<T>(C, S, false, a->floor);
-    stats_and_direction(C, S);
+    fit_sides<T>(C, S, P, false, a->floor);
+    stats_and_direction(C, S, P);
     const double g0 = S.packet[1];
     const double tol = a->grad_atol + a->grad_rtol * g0;
     if (tid == 0) {
@@ -858,7 +960,7 @@ __global__ __launch_bounds__(kThreadsT<T>) void split_optimize_kernel(art_split_
             }
             for (int k = 0; k < c2; ++k) acur *= a->rho;
             __syncthreads();
-            trial_losses(C, S, alphas, c2, losses);
+            trial_losses(C, S, P, alphas, c2, losses);
             for (int k = 0; k < c2 && sel < 0; ++k) {
               value = losses[k];
               step = alphas[k];
@@ -909,8 +1011,8 @@ __global__ __launch_bounds__(kThreadsT<T>) void split_optimize_kernel(art_split_
         S.theta[d] += step * S.pdir[d];
       }
       __syncthreads();
-      fit_sides<T>(C, S, false, a->floor);
-      stats_and_direction(C, S);
+      fit_sides<T>(C, S, P, false, a->floor);
+      stats_and_direction(C, S, P);
       if (tid == 0) {
         hist[0 * (MI + 1) + it] = S.packet[0];
         hist[1 * (MI + 1) + it] = S.packet[1];
@@ -943,8 +1045,8 @@ __global__ __launch_bounds__(kThreadsT<T>) void split_optimize_kernel(art_split_
       for (int j = 0; j < d; ++j) S.theta[j] /= nw;
     }
     __syncthreads();
-    if (a->hard_refit) fit_sides<T>(C, S, true, 0.0);
-    hard_stats(C, S, a->hard_refit != 0, hard);
+    if (a->hard_refit) fit_sides<T>(C, S, P, true, 0.0);
+    hard_stats(C, S, P, a->hard_refit != 0, hard);
     if (tid == 0) {
       const double right2 = hard[3];
       if (!a->hard_refit && hard[6] != 0.0) error = kNonfiniteSplit;
@@ -1004,8 +1106,82 @@ extern "C" int art_split_optimize(const art_split_args* args, int problems, hipS
   // Gram tiles cover the non-constant features only.
   const int nf = p - (args->basis.bias ? 1 : 0);
   const dim3 grid(problems);
-  if (nf <= 16) split_optimize_kernel<1><<<grid, dim3(kThreadsT<1>), 0, stream>>>(*args);
-  else split_optimize_kernel<2><<<grid, dim3(kThreadsT<2>), 0, stream>>>(*args);
+  const PartSync one{1, 0, 0, 0, nullptr, nullptr, nullptr, 0, 0};
+  if (nf <= 16) split_optimize_kernel<1><<<grid, dim3(kThreadsT<1>), 0, stream>>>(*args, one);
+  else split_optimize_kernel<2><<<grid, dim3(kThreadsT<2>), 0, stream>>>(*args, one);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? ART_OK : hip_fail(e, "split_optimize_kernel launch");
+}
+
+// Exchange slot of one part: the longest exchanged vector (both sides'
+// packed augmented systems and a flag; the statistics pass's 3 + d sums; a
+// trial pass's losses), rounded up to 32 doubles.
+static int part_stride(int p) {
+  const int nel = (p + 1) * (p + 2) / 2;
+  const int longest = std::max(2 * nel + 1, std::max(3 + kSplitMaxD, kSplitMaxH));
+  return (longest + 31) / 32 * 32;
+}
+
+extern "C" int art_split_part_doubles(int p, int parts) {
+  if (p < 1 || p > kSplitMaxP || parts < 1) return -1;
+  return 2 * parts * part_stride(p);
+}
+
+extern "C" int art_split_max_parts(const art_split_args* args, int problems) {
+  if (!args || problems < 1) return 1;
+  const int p = args->basis.p;
+  if (p < 1 || p > kSplitMaxP) return 1;
+  const int nf = p - (args->basis.bias ? 1 : 0);  // the launch's kernel: see art_split_optimize
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return 1;
+  const hipError_t e =
+      nf <= 16 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, split_optimize_kernel<1>,
+                                                               kThreadsT<1>, 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, split_optimize_kernel<2>,
+                                                               kThreadsT<2>, 0);
+  if (e != hipSuccess || per_cu < 1) return 1;
+  const long capacity = (long)per_cu * cus;
+  const long parts = capacity / problems;
+  return (int)(parts < 1 ? 1 : parts > kSplitMaxParts ? kSplitMaxParts : parts);
+}
+
+extern "C" int art_split_optimize_parts(const art_split_args* args, int problems, int parts,
+                                        int* count, double* buf, int* failed,
+                                        hipStream_t stream) {
+  if (parts == 1) return art_split_optimize(args, problems, stream);
+  if (!args || problems < 1 || parts < 1 || parts > kSplitMaxParts || !count || !buf || !failed)
+    return fail(ART_ERR_ARG, "art_split_optimize_parts: bad arguments");
+  const int p = args->basis.p;
+  if (p < 1 || p > kSplitMaxP || args->d < 1 || args->d > kSplitMaxD)
+    return fail(ART_ERR_ARG, "art_split_optimize_parts: requires 1 <= p <= 32 and 1 <= d <= 31");
+  if ((!args->row_offsets && args->n < 1) || args->H < 1 || args->max_iters < 0 ||
+      args->max_backtracks < 0 || args->max_failures < 1 || args->mode < 0 || args->mode > 2 ||
+      args->anneal_iters < 0 || args->child_precision != ART_PRECISION_FP64)
+    return fail(ART_ERR_ARG, "art_split_optimize_parts: invalid sizes or controls");
+  if (!args->X || !args->y || !args->temperature || !args->theta0 || !args->node ||
+      !args->theta_out || !args->beta_out || !args->hard || !args->info || !args->history ||
+      !args->scalars)
+    return fail(ART_ERR_ARG, "art_split_optimize_parts: null device pointer");
+  hipError_t me = hipMemsetAsync(count, 0, (size_t)problems * sizeof(int), stream);
+  if (me == hipSuccess) me = hipMemsetAsync(failed, 0, sizeof(int), stream);
+  if (me != hipSuccess) return hip_fail(me, "art_split_optimize_parts: clearing the counts");
+  PartSync ps{parts, 0, 0, part_stride(p), count, buf, failed, 0, 0};
+  art_split_args a = *args;
+  void* params[] = {&a, &ps};
+  const int nf = p - (args->basis.bias ? 1 : 0);
+  // Every part of a problem waits for the others at each exchange, so the
+  // whole grid must be resident at once: a cooperative launch, which the
+  // runtime refuses (an error, nothing runs) when it cannot be.
+  const hipError_t e =
+      nf <= 16 ? hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&split_optimize_kernel<1>),
+                                            dim3(problems * parts), dim3(kThreadsT<1>), params, 0,
+                                            stream)
+               : hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&split_optimize_kernel<2>),
+                                            dim3(problems * parts), dim3(kThreadsT<2>), params, 0,
+                                            stream);
+  if (e == hipSuccess) return ART_OK;
+  (void)hipGetLastError();  // a refused launch leaves no error for the next call to find
+  return hip_fail(e, "split_optimize_kernel cooperative launch");
 }

@@ -237,6 +237,31 @@ typedef struct {
  * SoftObliqueSplitter._optimize_from_initialization (art/splitters.py:365-690). */
 int art_split_optimize(const art_split_args* args, int problems, hipStream_t stream);
 
+/* art_split_optimize with each problem's rows split over `parts` workgroups
+ * (1 <= parts <= 8): the same passes, every pass's row sums added over the
+ * parts (in part order, so all parts hold the same node state and make the
+ * same decisions), for cohorts too small to fill the GPU with one workgroup
+ * per problem.  Same outputs and controls as art_split_optimize (whose
+ * reference counterparts it shares: art/_cuda_batched_optimizer.py:331-526,
+ * art/splitters.py:365-690); the sums differ from it in rounding only.
+ * count: `problems` ints and failed: one int, both cleared by the call; buf:
+ * problems * art_split_part_doubles(p, parts) doubles of exchange space.
+ * The grid (problems * parts workgroups) goes through a cooperative launch
+ * and must be resident at once; the call returns ART_ERR_HIP, with nothing
+ * run, when the runtime refuses it.  *failed = 1 after the launch means a
+ * part waited too long for the others (the results are then invalid).
+ * parts = 1 is art_split_optimize. */
+int art_split_optimize_parts(const art_split_args* args, int problems, int parts, int* count,
+                             double* buf, int* failed, hipStream_t stream);
+
+/* Exchange doubles per problem of art_split_optimize_parts (-1 if invalid). */
+int art_split_part_doubles(int p, int parts);
+
+/* The most parts (<= 8) for which `problems` problems of this launch's
+ * kernel (args' basis picks it) fit on the current device at once (1 when
+ * none). */
+int art_split_max_parts(const art_split_args* args, int problems);
+
 /* Annealed mode only (mode and Armijo fields ignored), restart-batched:
  * problems come in blocks of 16 consecutive problems that share one node
  * (node[16b] is the block's node; pad a node's starts with duplicates), one

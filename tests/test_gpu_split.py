@@ -584,3 +584,43 @@ def test_keyed_starts_do_not_depend_on_position(torch_gpu):
         np.testing.assert_allclose(a[j], O.device_start(X[offsets[j]:offsets[j + 1]], 0, 0, 0,
                                                         key=int(keys[j])),
                                    rtol=1e-12, atol=1e-13)
+
+
+@pytest.mark.parametrize("degree,d", [(1, 3), (2, 3), (2, 5)])
+def test_node_rows_spread_over_parts_match_one_workgroup(torch_gpu, degree, d):
+    # Large nodes searched with each node's rows spread over several
+    # workgroups (art_split_optimize_parts) agree with one workgroup per node
+    # and with the oracle: the parts exchange every pass's row sums, so only
+    # the order of the partial sums differs.  d = 5 quadratic (p = 21) runs
+    # the wide kernel and the block Cholesky.
+    from art import AffineRidgeModel, PolynomialRidgeModel, SoftObliqueSplitter
+    from art.batched_optimizer import RaggedOptimizerWorkspace
+
+    rng = np.random.default_rng(90 + 10 * degree + d)
+    sizes = [23000, 9000, 2500, 17001]
+    X = rng.uniform(-1, 1, (sum(sizes), d))
+    u = X[:, 0] + .4 * X[:, 1] - .2 * X[:, 2]
+    y = np.where(u > .1, 1. + X[:, 1], -.5 + .3 * X[:, 0]) + .05 * rng.standard_normal(len(u))
+    offsets = np.concatenate([[0], np.cumsum(sizes)])
+    model = AffineRidgeModel(1e-6) if degree == 1 else PolynomialRidgeModel(ridge=1e-6)
+    s = SoftObliqueSplitter(model, .3, max_iters=12, min_side_points=5, min_side_fraction=.05)
+    theta0 = np.tile(np.r_[1., np.zeros(d - 1), 0.], (len(sizes), 1))
+    theta0[:, 1] = [.3, -.2, .5, .1]
+    runs = {}
+    with RaggedOptimizerWorkspace(s, X, y, offsets) as ws:
+        for parts in (1, 4, None):
+            ws.split_parts = parts
+            runs[parts] = (ws.optimize(theta0), ws.last_launch_parts)
+    assert runs[1][1] == 1 and runs[4][1] == 4
+    assert runs[None][1] > 1  # the largest node asks for MAX_PARTS
+    for parts in (4, None):
+        for a, b in zip(runs[1][0], runs[parts][0]):
+            np.testing.assert_allclose(np.r_[b.w, b.z], np.r_[a.w, a.z], rtol=1e-9, atol=1e-10)
+            np.testing.assert_allclose(b.metadata["soft_loss_history"],
+                                       a.metadata["soft_loss_history"], rtol=1e-10)
+            assert b.stop_reason == a.stop_reason and b.n_iters == a.n_iters
+            assert b.n_right == a.n_right
+            np.testing.assert_allclose(b.loss, a.loss, rtol=1e-10)
+    for j in (0, 3):
+        lo, hi = offsets[j], offsets[j + 1]
+        check_against_oracle(runs[4][0][j], s, X[lo:hi], y[lo:hi], theta0[j], .3)
