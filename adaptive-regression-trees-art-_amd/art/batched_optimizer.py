@@ -153,6 +153,7 @@ class BatchedOptimizerWorkspace:
         # an int fixes it (1 is one workgroup per problem).
         self.split_parts = None
         self.last_launch_parts = 1
+        self.part_retries = 0  # multi-part launches rerun on one workgroup per problem
         self._part_bufs = None
 
     def _check_open(self):
@@ -309,8 +310,20 @@ class BatchedOptimizerWorkspace:
         self.last_launch_parts = parts
         self.stream.synchronize()
         if parts > 1 and int(failed.item()):
-            raise RuntimeError("art_split_optimize_parts: a node's workgroups were not all "
-                               "running together (exchange wait expired); results discarded.")
+            # A part waited too long for the others (the device was shared with
+            # other work): discard the launch and run it on one workgroup per
+            # problem, from cleared outputs.
+            self.part_retries += 1
+            with torch.cuda.stream(self.stream):
+                hard.zero_()
+                info.zero_()
+                scal.zero_()
+                if not lean:
+                    hist.fill_(float("nan"))
+                _hip.check(self.lib.art_split_optimize(ctypes.byref(a), P,
+                                                       _hip.stream_handle(self.stream)))
+            self.last_launch_parts = 1
+            self.stream.synchronize()
         if timing is not None:
             timing.append(ev[0].elapsed_time(ev[1]))
         if lean:

@@ -1068,10 +1068,6 @@ class ServingEngine:
             Xs, ys, _, dst = gather_strided(g.X, g.y, None, begin, end, counts, stream=g.stream,
                                             staging=g.staging)
             with RaggedOptimizerWorkspace(splitter, Xs, ys, dst, temperature=temps) as ws:
-                # Searches of several groups run at once on their own streams:
-                # one workgroup per problem, no launch that needs the device
-                # to itself.
-                ws.split_parts = 1
                 slots = [k for _ in range(R) for k in range(count)]
                 theta0 = ws._device_starts(slots, keys=keys).reshape(R, count, d + 1)
                 theta, ok, reason = ws.hyperplanes(theta0)

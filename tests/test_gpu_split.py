@@ -611,6 +611,7 @@ def test_node_rows_spread_over_parts_match_one_workgroup(torch_gpu, degree, d):
         for parts in (1, 4, None):
             ws.split_parts = parts
             runs[parts] = (ws.optimize(theta0), ws.last_launch_parts)
+        assert ws.part_retries == 0
     assert runs[1][1] == 1 and runs[4][1] == 4
     assert runs[None][1] > 1  # the largest node asks for MAX_PARTS
     for parts in (4, None):
