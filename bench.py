@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--rate", type=float, default=200.0, help="serving arrivals per second")
     ap.add_argument("--split-nodes", type=int, default=2048,
                     help="nodes of the annealed split leg (64 restarts each)")
+    ap.add_argument("--max-split-parts", type=int, default=0,
+                    help="cap the workgroups per node of the device split search (0: the "
+                         "library's policy, up to 8; 1: one workgroup per node)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after timing, run a plain 1 GiB device copy three times: the "
                          "calibration kernel for FETCH_SIZE / WRITE_SIZE counter runs")
@@ -990,6 +993,9 @@ def run_legs(args):
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a HIP device")
     sys.path.insert(0, str(ROOT))
+    if args.max_split_parts > 0:
+        from art import batched_optimizer
+        batched_optimizer.MAX_PARTS = min(batched_optimizer.MAX_PARTS, args.max_split_parts)
     if args.only == "ridge_kernels":
         _LINE["line"] = bench_ridge_kernels(args)
         return
