@@ -310,9 +310,9 @@ constexpr int kWaveSolveP = 16;
 // status contract as the workgroup form below (-1 bad gates, -2 non-finite,
 // k+1 first pivot <= 0, -3 non-finite solution).  col: 64 doubles of LDS for
 // this wave.  Writes beta[0..p) and lam; returns the status (uniform).
+template <int PMAX>
 __device__ inline int wave_side_solve(double* M, int p, double ridge, int bad, int lane,
                                       double* col, double* beta, double& lam_out) {
-  constexpr int PMAX = kWaveSolveP;
   const double tr = wave_sum(lane < p ? M[tri(lane, lane)] : 0.0);
   const double lam = ridge == 0.0 ? 0.0 : ridge * (tr / p);
   int st = bad ? -1 : ((!isfinite(tr) || !isfinite(lam)) ? -2 : 0);
@@ -550,10 +550,15 @@ __device__ void fit_sides(const SplitCtx& C, SplitShared<T>& S, PartSync& P, boo
   if (p <= kWaveSolveP) {
     // Small child systems: wave `side` factors and solves side `side` in
     // registers (no barrier per column), both sides at once.
+    // Width 8 for p <= 8: the same results (steps past p change nothing),
+    // half the column and substitution steps.
     if (wave < 2) {
       const int side = wave;
-      S.status[side] = wave_side_solve(S.M[side], p, C.ridge, bad, lane,
-                                       S.red + 64 * side, S.beta[side], S.lam[side]);
+      S.status[side] =
+          p <= 8 ? wave_side_solve<8>(S.M[side], p, C.ridge, bad, lane, S.red + 64 * side,
+                                      S.beta[side], S.lam[side])
+                 : wave_side_solve<kWaveSolveP>(S.M[side], p, C.ridge, bad, lane,
+                                                S.red + 64 * side, S.beta[side], S.lam[side]);
     }
     __syncthreads();
     return;
