@@ -51,7 +51,12 @@ enum StopReason {
 };
 
 __device__ inline double gate(double score) {
-  return 1.0 / (1.0 + exp(-fmin(50.0, fmax(-50.0, score))));
+  // 1 / (1 + e) by v_rcp_f64 and two Newton steps (within an ulp of the
+  // quotient; e + 1 lies in [1, 5.2e21], so the reciprocal is normal).
+  const double v = 1.0 + exp(-fmin(50.0, fmax(-50.0, score)));
+  double r = __builtin_amdgcn_rcp(v);
+  r = fma(r, fma(-v, r, 1.0), r);
+  return fma(r, fma(-v, r, 1.0), r);
 }
 
 // Workgroup and staged-chunk geometry.  Rows are staged in LDS with an odd
@@ -947,14 +952,22 @@ __global__ __launch_bounds__(kThreadsT<T>) void split_optimize_kernel(art_split_
       while (true) {
         // Candidates alpha_k = a0 rho^k by repeated multiplication
         // (art/_cuda_batched_optimizer.cu:47-57); a logical batch of H
-        // candidates is evaluated in passes of at most kSplitMaxH.
+        // candidates is evaluated in passes of at most kSplitMaxH.  With one
+        // workgroup per problem (launches of many problems, bound by row
+        // work) the first candidate, usually accepted, goes alone and the
+        // rest follow only when it fails; with several parts per problem
+        // (bound by the passes' latency) the batch goes at once.  The batch's
+        // choice -- its first candidate that satisfies Armijo -- and every
+        // loss are the same either way; `physical` counts the batch, as the
+        // reference's batched search does.
+        const int lead = P.parts == 1 ? 1 : kSplitMaxH;
         double acur = a->adaptive ? alpha_ctrl : a->alpha0;
         selected = -1;
         for (int offset = 0; offset < S_; offset += H) {
           const int count = min(H, S_ - offset);
           int sel = -1;
-          for (int sub = 0; sub < count; sub += kSplitMaxH) {
-            const int c2 = min(kSplitMaxH, count - sub);
+          for (int sub = 0; sub < count && sel < 0;) {
+            const int c2 = min(sub == 0 ? lead : kSplitMaxH, count - sub);
             __syncthreads();  // every thread has finished reading the last pass
             if (tid == 0) {
               double t = acur;
@@ -972,6 +985,7 @@ __global__ __launch_bounds__(kThreadsT<T>) void split_optimize_kernel(art_split_
               if (isfinite(value) && value <= S.packet[0] + a->armijo_c * step * S.packet[2])
                 sel = offset + sub + k;
             }
+            sub += c2;
           }
           physical += count;
           logical += sel >= 0 ? sel - offset + 1 : count;
