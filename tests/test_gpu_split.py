@@ -625,3 +625,40 @@ def test_node_rows_spread_over_parts_match_one_workgroup(torch_gpu, degree, d):
     for j in (0, 3):
         lo, hi = offsets[j], offsets[j + 1]
         check_against_oracle(runs[4][0][j], s, X[lo:hi], y[lo:hi], theta0[j], .3)
+
+
+def test_annealed_mode_over_parts_matches_one_workgroup_and_the_oracle(torch_gpu):
+    # The ragged workspace anneals one start per problem on the optimizer
+    # kernel (mode 1); with each node's rows over 4 workgroups the annealed
+    # trajectories, hard refits and losses agree with one workgroup per node
+    # and with the oracle.
+    from art import AffineRidgeModel, SoftObliqueSplitter
+    from art.batched_optimizer import RaggedOptimizerWorkspace
+
+    rng = np.random.default_rng(131)
+    sizes = [6000, 3100]
+    X = rng.uniform(-1, 1, (sum(sizes), 3))
+    y = np.where(X[:, 0] - .3 * X[:, 2] > 0, 1. + X[:, 1], -.4 * X[:, 0]) \
+        + .05 * rng.standard_normal(len(X))
+    offsets = np.concatenate([[0], np.cumsum(sizes)])
+    s = SoftObliqueSplitter(AffineRidgeModel(1e-4), .5, n_restarts=1, min_side_points=5)
+    starts = np.array([[[.8, .6, 0., .05], [.6, 0., .8, -.1]]])
+    runs = {}
+    with RaggedOptimizerWorkspace(s, X, y, offsets) as ws:
+        for parts in (1, 4):
+            ws.split_parts = parts
+            runs[parts] = ws.anneal(starts, t0=.5, t1=.05, iters=20, step=.2)
+            assert ws.last_launch_parts == parts
+        assert ws.part_retries == 0
+    a, b = runs[1], runs[4]
+    np.testing.assert_allclose(b["theta"], a["theta"], rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(b["soft_loss_history"], a["soft_loss_history"], rtol=1e-10)
+    np.testing.assert_array_equal(b["ok"], a["ok"])
+    np.testing.assert_allclose(b["loss"], a["loss"], rtol=1e-10)
+    so = settings_of(s, .5)
+    for j in range(2):
+        lo, hi = offsets[j], offsets[j + 1]
+        th, losses = O.annealed(X[lo:hi], y[lo:hi], starts[0, j], so, iters=20, t0=.5, t1=.05,
+                                step=.2)
+        np.testing.assert_allclose(b["theta"][0, j], O.project_unit_w(th), rtol=1e-9, atol=1e-10)
+        np.testing.assert_allclose(b["soft_loss_history"][0, j], losses, rtol=1e-9, atol=1e-12)
