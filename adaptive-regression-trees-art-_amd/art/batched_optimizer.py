@@ -7,7 +7,10 @@ art/_cuda_batched_optimizer.py:88-660: same constructor, ``allocate``,
 and metadata keys.  Where the reference runs a host loop with one
 device->host transfer per iteration and per backtrack batch, this workspace
 makes one launch of ``art_split_optimize`` for every (node, restart) problem
-of the call and downloads the results once.  Random starts (Gaussian unit w,
+of the call and downloads the results once; when a launch has too few
+problems to fill the device, each problem's rows are spread over several
+workgroups of one cooperative launch (``art_split_optimize_parts``, same
+results up to the order of the partial sums).  Random starts (Gaussian unit w,
 z = median(X w), art/splitters.py:692-700) are drawn on the device
 (``art_split_init``: Philox4x32-10 normals seeded from the splitter's RNG,
 radix-select median).
