@@ -36,10 +36,16 @@ namespace art {
 // Relative margin kept inside each end of a chord (2^-40).
 constexpr double kChordMargin = 0x1p-40;
 
-__device__ __forceinline__ double chord_point(double tlo, double thi, double u) {
-  const double m = kChordMargin * (thi - tlo);
-  const double lo = tlo + m, hi = thi - m;
-  return (hi - lo) * u + lo;
+// The point of the chord [tlo, thi] shrunk by kChordMargin of its length at
+// each end: tlo + (thi - tlo) u' with u' = m + u (1 - 2 m) (chord_fraction,
+// taken once per move, off the chain's dependent path), which is
+// lo + u (hi - lo) for lo = tlo + m (thi - tlo), hi = thi - m (thi - tlo).
+// 1 - 2 m is exact, and u' lies in [m, 1 - m] for u in [0, 1).
+__device__ __forceinline__ double chord_fraction(double u) {
+  return fma(u, 1.0 - 2.0 * kChordMargin, kChordMargin);
+}
+__device__ __forceinline__ double chord_point(double tlo, double thi, double uf) {
+  return fma(thi - tlo, uf, tlo);
 }
 }  // namespace art
 
@@ -238,7 +244,7 @@ __device__ __attribute__((always_inline)) inline void har_chain_lane(const art_h
       tlo = (ad < 0.0 && ratio > tlo) ? ratio : tlo;
       thi = (ad > 0.0 && ratio < thi) ? ratio : thi;
     }
-    const double t = chord_point(tlo, thi, u);
+    const double t = chord_point(tlo, thi, chord_fraction(u));
 #pragma unroll
     for (int j = 0; j < DM; ++j) x[j] += dir[j] * t;
     const int done = s + 1;
@@ -268,7 +274,7 @@ __global__ __launch_bounds__(64) void har_kernel(art_har_args a) {
 // in every lane), so a move's critical path is only the slacks, the chord
 // and the step.  Each quantity is computed by the same code as in
 // har_kernel, so the chain is bit-identical to har_kernel's.
-// LDS entry of a move: dir[DM], 1/dir[DM], u, A_i.d[m], 1/(A_i.d)[m].
+// LDS entry of a move: dir[DM], 1/dir[DM], the chord fraction u', A_i.d[m], 1/(A_i.d)[m].
 constexpr int kChainBlock = 64;
 
 // One chain per workgroup, M >= 0: exactly M general rows, held in registers
@@ -344,7 +350,7 @@ __device__ __attribute__((always_inline)) inline void chain_run(const art_har_ar
           e[2 * DM + 1 + j] = nz ? (up ? blo[j] : bhi[j]) : -INFINITY;
           e[3 * DM + 1 + j] = nz ? (up ? bhi[j] : blo[j]) : INFINITY;
         }
-      e[2 * DM] = u;
+      e[2 * DM] = chord_fraction(u);
       // Per row, 1 / (A_i.d) on the side of the chord it bounds and NaN on
       // the other (a row with A_i.d = 0 bounds neither): the replay's
       // slack * reciprocal is then the row's chord end or NaN, and fmax /

@@ -126,11 +126,13 @@ DEVICE_CHORD_MARGIN = 2.0 ** -40
 
 
 def chord_point(t_lo, t_hi, u, margin=0.0):
-    """t_lo + u (t_hi - t_lo) on the chord shrunk by margin (t_hi - t_lo) at
-    each end (art/sampling.py:877-962 at margin 0)."""
-    m = margin * (t_hi - t_lo)
-    lo, hi = t_lo + m, t_hi - m
-    return (hi - lo) * u + lo
+    """The uniform point of the chord shrunk by margin (t_hi - t_lo) at each
+    end: t_lo + (t_hi - t_lo) u' with u' = margin + u (1 - 2 margin), i.e.
+    lo + u (hi - lo) for lo = t_lo + margin (t_hi - t_lo), hi = t_hi -
+    margin (t_hi - t_lo).  At margin 0, u' = u: the reference's step
+    t_lo + u (t_hi - t_lo) (art/sampling.py:877-962)."""
+    uf = margin + u * (1.0 - 2.0 * margin)
+    return t_lo + (t_hi - t_lo) * uf
 
 
 def advance(A, points, slack, directions, uniforms, margin=0.0):
