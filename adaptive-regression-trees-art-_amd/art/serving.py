@@ -1068,6 +1068,12 @@ class ServingEngine:
             Xs, ys, _, dst = gather_strided(g.X, g.y, None, begin, end, counts, stream=g.stream,
                                             staging=g.staging)
             with RaggedOptimizerWorkspace(splitter, Xs, ys, dst, temperature=temps) as ws:
+                # One workgroup per node: with several, the number of parts
+                # (and so the order in which a pass's partial sums are added)
+                # would depend on how many nodes share the launch, and a
+                # request served in a mix must get bitwise the tree it gets
+                # alone.
+                ws.split_parts = 1
                 slots = [k for _ in range(R) for k in range(count)]
                 theta0 = ws._device_starts(slots, keys=keys).reshape(R, count, d + 1)
                 theta, ok, reason = ws.hyperplanes(theta0)

@@ -135,6 +135,11 @@ def test_results_are_the_same_alone_and_in_the_mix(torch_gpu):
     others = [_request(rng, n, d, seed=s, max_depth=4, min_leaf=64)
               for n, d, s in ((5000, 4, 1), (30000, 8, 2), (1500, 2, 4), (12000, 4, 5),
                               (8000, 16, 6))]
+    # Sixteen more fits of the target's basis, so that its levels share
+    # search launches of many nodes (a launch's size must not change a node's
+    # arithmetic).
+    others += [_request(rng, 9000, 4, seed=20 + i, max_depth=4, min_leaf=64)
+               for i in range(16)]
     Xn = rng.uniform(-1, 1, (1500, 4))
     yn = np.cos(Xn.sum(axis=1))
     cfg = ServingConfig(split_rows=2048)
