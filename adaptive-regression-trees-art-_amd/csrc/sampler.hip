@@ -209,7 +209,11 @@ __device__ __attribute__((always_inline)) inline void har_chain_lane(const art_h
       // need no multiplication, and the tighter of repeated rows always wins
       // the chord's min/max, so one bound per side gives the same chord.  The
       // two ratios share one reciprocal (v_rcp_f64 plus two Newton steps, a
-      // few ulp from the quotient); a zero direction component is skipped.
+      // few ulp from the quotient), and the direction's sign says which face
+      // bounds which end.  A zero component (reciprocal +-inf, the same sign)
+      // gives the ends -inf and +inf, or NaN for a point on a face, which
+      // fmax / fmin drop: no bound, as for the skipped component of the
+      // single-chain replay.
 #pragma unroll
       for (int j = 0; j < DM; ++j)
         if (EXACT || j < d) {
@@ -219,12 +223,9 @@ __device__ __attribute__((always_inline)) inline void har_chain_lane(const art_h
           r = fma(r, fma(-dj, r, 1.0), r);
           const double rl = (box_lo[j] - x[j]) * r;
           const double rh = (box_hi[j] - x[j]) * r;
-          const bool order = rl < rh;
-          const double lo_c = order ? rl : rh;
-          const double hi_c = order ? rh : rl;
-          const bool nz = dj != 0.0;
-          tlo = (nz && lo_c > tlo) ? lo_c : tlo;
-          thi = (nz && hi_c < thi) ? hi_c : thi;
+          const bool up = !__builtin_signbit(dj);
+          tlo = fmax(tlo, up ? rl : rh);
+          thi = fmin(thi, up ? rh : rl);
         }
     }
 #pragma unroll 2
