@@ -43,3 +43,35 @@ def test_feature_count_is_host_only(d, degree, bias, last, p):
 
     assert _hip.load().art_feature_count(d, degree, bias, last) == p
     assert _hip.load().art_feature_count(3, 2, 1, 1) == -1
+
+
+@pytest.mark.parametrize("p,parts,doubles", [
+    # Two slots of parts x stride doubles; the stride holds the longest
+    # exchanged vector -- both sides' packed (p+1)(p+2)/2 systems and a flag,
+    # 3 + 31 statistics, 8 trial losses -- rounded up to 32.
+    (6, 8, 2 * 8 * 64), (1, 2, 2 * 2 * 64), (16, 4, 2 * 4 * 320), (32, 2, 2 * 2 * 1152),
+])
+def test_split_exchange_space_is_host_only(p, parts, doubles):
+    from art import _hip
+
+    lib = _hip.load()
+    assert lib.art_split_part_doubles(p, parts) == doubles
+    assert lib.art_split_part_doubles(0, 2) == -1 and lib.art_split_part_doubles(33, 2) == -1
+    assert lib.art_split_part_doubles(6, 0) == -1
+
+
+def test_split_parts_entry_refuses_bad_arguments_before_any_device_call():
+    import ctypes
+
+    from art import _hip
+
+    lib = _hip.load()
+    a = _hip.SplitArgs()
+    null = ctypes.c_void_p()
+    assert lib.art_split_max_parts(None, 4) == 1
+    assert lib.art_split_optimize_parts(None, 4, 2, null, null, null, null) == _hip.ERR_ARG
+    # parts outside [1, 8] or missing exchange buffers
+    assert lib.art_split_optimize_parts(ctypes.byref(a), 4, 9, null, null, null,
+                                        null) == _hip.ERR_ARG
+    assert lib.art_split_optimize_parts(ctypes.byref(a), 4, 2, null, null, null,
+                                        null) == _hip.ERR_ARG
