@@ -945,6 +945,7 @@ def while_arriving(events, lat, start=1.0):
     arrivals = int(((arrive >= start) & (arrive <= t1)).sum())
     return {"window_s": [start, t1], "completions_per_s": completions / span,
             "arrivals_per_s": arrivals / span,
+            "completions_per_arrival": completions / max(arrivals, 1),
             "backlog_at_last_arrival": int(len(events) - (done <= t1).sum()),
             "definition": "completions and arrivals inside the window (seconds from the first "
                           "arrival); backlog = requests not finished at the last arrival"}
